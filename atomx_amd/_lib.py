@@ -1,0 +1,125 @@
+"""ctypes binding of libatomx_slater.so (the C ABI declared in include/atomx_slater.h).
+
+The product path has no CPU fallback: importing a compute entry point without
+the built library, or calling one without a HIP device, raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+from ._build import LIB_PATH
+
+AX_OK = 0
+AX_ERR_VALUE = 1
+AX_ERR_TYPE = 2
+AX_ERR_MEMORY = 3
+AX_ERR_NONFINITE = 4
+AX_ERR_HIP = 5
+
+AX_PREC_FP64 = 0
+AX_PREC_SPLIT = 1
+
+PRECISIONS = {"fp64": AX_PREC_FP64, "split": AX_PREC_SPLIT}
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+# name -> (restype, argtypes); every symbol declared in include/atomx_slater.h
+SIGNATURES = {
+    "ax_last_error": (ctypes.c_char_p, []),
+    "ax_version": (_int, []),
+    "ax_split_slices": (_int, []),
+    "ax_split_ktile": (_int, []),
+    "ax_split_products": (_int, []),
+    "ax_device_count": (_int, [ctypes.POINTER(_int)]),
+    "ax_yk_partial_matrices": (_int, [_int, _dp, _dp]),
+    "ax_grid_rpow": (_int, [_vp, _i64, _int, _vp, _vp, _vp]),
+    "ax_pair_density": (_int, [_vp, _vp, _i64, _i64, _int, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "ax_yk_batch": (_int, [_vp, _i64, _vp, _vp, _vp, _i64, _int, _int, _int, _vp, _vp, _int,
+                           _vp, _vp, _vp, _vp, _i64, _vp]),
+    "ax_rk_dot": (_int, [_vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _vp, _vp]),
+    "ax_bspline_eval": (_int, [_vp, _i64, _int, _vp, _int, _int, _int, _int, _vp, _i64, _vp]),
+    "ax_exchange_matrix_yk": (_int, [_vp, _vp, _vp, _vp, _int, _int, _int, _int, _int, _vp,
+                                     _vp]),
+    "ax_slater_rk_direct": (_int, [_vp, _vp, _vp, _vp, _int, _int, _int, _int, _vp, _vp, _vp,
+                                   _vp, ctypes.c_double, _vp, _vp, _vp]),
+    "ax_rk_gemm_workspace_bytes": (_i64, [_int, _i64, _int]),
+    "ax_rk_gemm": (_int, [_vp, _i64, _vp, _vp, _i64, _int, _i64, _int, _vp, _vp, _vp]),
+    "ax_rk_tensor_create": (_int, [_dp, _dp, _int, _int, _int, _ip, _int, _int,
+                                   ctypes.POINTER(_vp)]),
+    "ax_rk_tensor_destroy": (_int, [_vp]),
+    "ax_rk_tensor_size": (_i64, [_vp]),
+    "ax_rk_tensor_pairs": (_int, [_vp, _int]),
+    "ax_rk_tensor_offset": (_i64, [_vp, _int]),
+    "ax_rk_tensor_footprint": (_i64, [_vp]),
+    "ax_rk_tensor_run": (_int, [_vp, _vp, _vp, _vp, _vp]),
+    "ax_rk_tensor_run_stage": (_int, [_vp, _int, _vp, _vp, _vp, _vp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    """libatomx_slater.so has not been built (run __graft_entry__.build())."""
+
+
+def load():
+    """Loads the shared library (no GPU needed) and binds every symbol."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise NativeLibraryMissing(
+                    f"{LIB_PATH} not found: build the HIP extension first "
+                    "(python -c 'import __graft_entry__ as g; g.build()')")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def register(name, restype, argtypes):
+    """Binds an additional symbol of the library (used by sibling modules)."""
+    lib = load()
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = argtypes
+    SIGNATURES[name] = (restype, argtypes)
+    return fn
+
+
+def last_error() -> str:
+    msg = load().ax_last_error()
+    return msg.decode() if msg else ""
+
+
+_EXC = {
+    AX_ERR_VALUE: ValueError,
+    AX_ERR_TYPE: TypeError,
+    AX_ERR_MEMORY: MemoryError,
+    AX_ERR_NONFINITE: FloatingPointError,
+    AX_ERR_HIP: RuntimeError,
+}
+
+
+def check(status: int, what: str = "") -> None:
+    if status != AX_OK:
+        exc = _EXC.get(status, RuntimeError)
+        raise exc(f"{what}: {last_error()}" if what else last_error())
+
+
+def call(name: str, *args):
+    """Calls an ax_* entry point that returns a status, raising on failure."""
+    fn = getattr(load(), name)
+    check(fn(*args), name)

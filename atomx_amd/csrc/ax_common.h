@@ -1,0 +1,55 @@
+// Internal helpers shared by the HIP sources of libatomx_slater.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#include "atomx_slater.h"
+
+namespace ax {
+
+void set_error(const char *fmt, ...);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int grid_for(int64_t work, int threads, int cap = 65535) {
+    int64_t g = (work + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<int>(g);
+}
+
+// Gauss-Legendre nodes and weights on [-1, 1] (ascending), computed on the
+// host in long double by Newton iteration on P_n.
+void gauss_legendre(int n, double *x, double *w);
+
+}  // namespace ax
+
+#define AX_CHECK_HIP(expr)                                                        \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ax::set_error("%s failed: %s", #expr, hipGetErrorString(e_));         \
+            return AX_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define AX_CHECK_LAUNCH()                                                         \
+    do {                                                                          \
+        hipError_t e_ = hipGetLastError();                                        \
+        if (e_ != hipSuccess) {                                                   \
+            ax::set_error("kernel launch failed: %s", hipGetErrorString(e_));     \
+            return AX_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define AX_REQUIRE(cond, ...)                                                     \
+    do {                                                                          \
+        if (!(cond)) {                                                            \
+            ax::set_error(__VA_ARGS__);                                           \
+            return AX_ERR_VALUE;                                                  \
+        }                                                                         \
+    } while (0)

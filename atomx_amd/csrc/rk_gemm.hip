@@ -1,0 +1,210 @@
+// R^k contraction R[i][j] = sum_x D[row_i][x] * Y[j][x] (numerics/bspline/
+// slater.py:129-130 for a whole pair block), upper triangle only.
+//
+// Work decomposition: a list of segments (one per rank k, per request, ...),
+// each an m x m symmetric-structured output; every upper 64x64 tile of every
+// segment is one work entry; K is split into `n_split` contiguous chunks.
+// Partial tiles are written to a workspace and summed over the splits in a
+// fixed order by rk_reduce_kernel, so a result depends only on its own
+// segment's data (never on what else is in the batch).
+//
+// FP64 path: v_mfma_f64_16x16x4_f64.  256 threads = 4 waves in a 2x2 layout,
+// each wave owns a 32x32 sub-tile = 2x2 MFMA tiles; K is staged through LDS in
+// chunks of 16 in k-major layout.
+#include "ax_common.h"
+#include "rk_gemm.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64;
+constexpr int BK = 16;
+constexpr int LDS_LD = BM + 4;  // padded row (in doubles) of the k-major LDS tiles
+
+// Loads one BK x BM k-major tile of rows [row0, row0+64) x [k0, k0+16) into
+// registers: thread t -> row t/4, k sub-block (t%4)*4 .. +3.
+__device__ inline void load_tile_regs(const double *__restrict__ base, int64_t ld,
+                                      const int32_t *__restrict__ rows, int row0, int m,
+                                      int64_t k0, int64_t kend, double (&v)[4]) {
+    const int t = threadIdx.x;
+    const int r = t >> 2;
+    const int kq = (t & 3) * 4;
+    const int gr = row0 + r;
+    if (gr < m) {
+        const int64_t rr = rows ? (int64_t)rows[gr] : (int64_t)gr;
+        const double *p = base + rr * ld;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t kx = k0 + kq + q;
+            v[q] = kx < kend ? p[kx] : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = 0.0;
+    }
+}
+
+__device__ inline void store_tile_lds(double *__restrict__ s, const double (&v)[4]) {
+    const int t = threadIdx.x;
+    const int r = t >> 2;
+    const int kq = (t & 3) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[(kq + q) * LDS_LD + r] = v[q];
+}
+
+__global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::RkSeg *__restrict__ segs,
+                                                          const ax::RkWork *__restrict__ work,
+                                                          int n_work, int n_split,
+                                                          double *__restrict__ partial) {
+    __shared__ double As[2][BK * LDS_LD];
+    __shared__ double Bs[2][BK * LDS_LD];
+
+    const int e = blockIdx.x;
+    const int split = blockIdx.y;
+    if (e >= n_work) return;
+    const ax::RkWork wk = work[e];
+    const ax::RkSeg sg = segs[wk.seg];
+    const int64_t chunk = ((sg.n + n_split - 1) / n_split + BK - 1) / BK * BK;
+    const int64_t kbeg = (int64_t)split * chunk;
+    const int64_t kend = kbeg + chunk < sg.n ? kbeg + chunk : sg.n;
+    const int row0 = wk.ti * BM, col0 = wk.tj * BM;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int fr = lane & 15, fk = lane >> 4;
+
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+    double va[4], vb[4];
+    int buf = 0;
+    if (kbeg < kend) {
+        load_tile_regs(sg.D, sg.ld_d, sg.rows, row0, sg.m, kbeg, kend, va);
+        load_tile_regs(sg.Y, sg.ld_y, nullptr, col0, sg.m, kbeg, kend, vb);
+        store_tile_lds(As[0], va);
+        store_tile_lds(Bs[0], vb);
+    }
+    __syncthreads();
+    for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+        const int64_t kn = k0 + BK;
+        if (kn < kend) {
+            load_tile_regs(sg.D, sg.ld_d, sg.rows, row0, sg.m, kn, kend, va);
+            load_tile_regs(sg.Y, sg.ld_y, nullptr, col0, sg.m, kn, kend, vb);
+        }
+        const double *as = As[buf];
+        const double *bs = Bs[buf];
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            const int kr = (kk * 4 + fk) * LDS_LD;
+            const double a0 = as[kr + wr + fr];
+            const double a1 = as[kr + wr + 16 + fr];
+            const double b0 = bs[kr + wc + fr];
+            const double b1 = bs[kr + wc + 16 + fr];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        if (kn < kend) {
+            store_tile_lds(As[buf ^ 1], va);
+            store_tile_lds(Bs[buf ^ 1], vb);
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    // C layout of the 16x16x4 f64 MFMA: result register r of lane l holds
+    // row (l/16) + 4r, column l%16 (one-row groups, unlike the 4-row groups
+    // of the i32/f32 16x16 MFMAs).
+    double *dst = partial + ((int64_t)e * n_split + split) * (BM * BM);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = wr + a * 16 + fk + 4 * r;
+                const int j = wc + b * 16 + fr;
+                dst[i * BM + j] = acc[a][b][r];
+            }
+}
+
+// Sums the split partials of each tile in split order and writes the packed
+// upper triangle of the segment.
+__global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::RkSeg *__restrict__ segs,
+                                                        const ax::RkWork *__restrict__ work,
+                                                        int n_work, int n_split,
+                                                        const double *__restrict__ partial,
+                                                        double *__restrict__ out_base) {
+    const int e = blockIdx.x;
+    if (e >= n_work) return;
+    const ax::RkWork wk = work[e];
+    const ax::RkSeg sg = segs[wk.seg];
+    const int64_t m = sg.m;
+    const double *src = partial + (int64_t)e * n_split * (BM * BM);
+    for (int t = threadIdx.x; t < BM * BM; t += blockDim.x) {
+        const int li = t / BM, lj = t % BM;
+        const int64_t i = (int64_t)wk.ti * BM + li;
+        const int64_t j = (int64_t)wk.tj * BM + lj;
+        if (i >= m || j >= m || j < i) continue;
+        double s = src[t];
+        for (int sp = 1; sp < n_split; ++sp) s += src[(int64_t)sp * BM * BM + t];
+        out_base[sg.out_off + i * m - i * (i - 1) / 2 + (j - i)] = s;
+    }
+}
+
+}  // namespace
+
+namespace ax {
+
+int gemm_tile() { return BM; }
+
+int64_t rk_gemm_partial_doubles(int n_work, int n_split) {
+    return (int64_t)n_work * n_split * BM * BM;
+}
+
+ax_status rk_gemm_contract_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work,
+                                 int n_split, int precision, double *partial, hipStream_t stream) {
+    if (n_work <= 0) return AX_OK;
+    if (precision == AX_PREC_FP64) {
+        hipLaunchKernelGGL(rk_gemm_f64_kernel, dim3(n_work, n_split), dim3(256), 0, stream,
+                           segs_dev, work_dev, n_work, n_split, partial);
+        AX_CHECK_LAUNCH();
+        return AX_OK;
+    }
+    if (precision == AX_PREC_SPLIT)
+        return rk_gemm_split_launch(segs_dev, work_dev, n_work, n_split, partial, stream);
+    set_error("unknown precision mode %d", precision);
+    return AX_ERR_VALUE;
+}
+
+ax_status rk_gemm_reduce_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work,
+                               int n_split, const double *partial, double *out_base,
+                               hipStream_t stream) {
+    if (n_work <= 0) return AX_OK;
+    hipLaunchKernelGGL(rk_reduce_kernel, dim3(n_work), dim3(256), 0, stream, segs_dev, work_dev,
+                       n_work, n_split, partial, out_base);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status rk_gemm_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work, int n_split,
+                         int precision, double *partial, double *out_base, hipStream_t stream) {
+    ax_status st = rk_gemm_contract_launch(segs_dev, work_dev, n_work, n_split, precision, partial,
+                                           stream);
+    if (st != AX_OK) return st;
+    return rk_gemm_reduce_launch(segs_dev, work_dev, n_work, n_split, partial, out_base, stream);
+}
+
+void upper_tiles(int m, int seg, std::vector<RkWork> &out) {
+    const int T = (m + BM - 1) / BM;
+    for (int ti = 0; ti < T; ++ti)
+        for (int tj = ti; tj < T; ++tj) out.push_back(RkWork{seg, ti, tj, 0});
+}
+
+}  // namespace ax

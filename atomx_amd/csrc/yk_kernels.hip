@@ -1,0 +1,542 @@
+// (Y) Poisson discretisation on the Gauss grid: pair densities, r-power
+// tables, the cell-blocked Y^k scan, quartet contraction, B-spline orbital
+// evaluation and the Y^k-form exchange matrix.
+//
+// Data layout in HBM: every radial function is one contiguous row of n =
+// nv*ng FP64 samples (cell-major, node-minor), rows of a batch stacked with a
+// leading dimension.  Kernels map 64-lane wavefronts onto whole cells
+// (floor(64/ng) cells per wave) so that every global load of a wave is one
+// contiguous run of doubles.
+#include "ax_common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxNg = 16;
+
+// ---------------------------------------------------------------------------
+// r^k and r^-(k+1) tables (numerics/yk.py:113-117)
+// ---------------------------------------------------------------------------
+__global__ void rpow_kernel(const double *__restrict__ r, int64_t n, int kmax,
+                            double *__restrict__ rk, double *__restrict__ rkm) {
+    int64_t total = n * (kmax + 1);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int k = static_cast<int>(t / n);
+        int64_t x = t - (int64_t)k * n;
+        double rv = r[x];
+        rk[t] = pow(rv, static_cast<double>(k));
+        rkm[t] = pow(rv, static_cast<double>(-(k + 1)));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pair densities rho_p = P_a P_b + Q_a Q_b  (optionally times w)
+// ---------------------------------------------------------------------------
+__global__ void pair_density_kernel(const double *__restrict__ P, const double *__restrict__ Q,
+                                    int64_t ld_pq, int64_t n, int n_pairs,
+                                    const int32_t *__restrict__ pa,
+                                    const int32_t *__restrict__ pb,
+                                    const double *__restrict__ w, double *__restrict__ out,
+                                    int64_t ld_out) {
+    for (int p = blockIdx.y; p < n_pairs; p += gridDim.y) {
+        const double *Pa = P + (int64_t)pa[p] * ld_pq;
+        const double *Pb = P + (int64_t)pb[p] * ld_pq;
+        const double *Qa = Q ? Q + (int64_t)pa[p] * ld_pq : nullptr;
+        const double *Qb = Q ? Q + (int64_t)pb[p] * ld_pq : nullptr;
+        double *o = out + (int64_t)p * ld_out;
+        for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+             x += (int64_t)gridDim.x * blockDim.x) {
+            double v = Pa[x] * Pb[x];
+            if (Q) v += Qa[x] * Qb[x];
+            if (w) v = w[x] * v;
+            o[x] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Y^k scan (numerics/yk.py:87-154).
+//
+// One work-group per (density row, k) item, grid-strided over items.
+//   pass 1: per-cell full sums S_out[c] = sum_j d_j r_j^k and
+//           S_in[c] = sum_j d_j r_j^-(k+1)  (d = w*rho), into LDS;
+//   scan:   exclusive prefix of S_out (outward) and exclusive suffix of S_in
+//           (inward) over cells: segmented sequential sums per thread, then a
+//           Hillis-Steele scan of the 256 segment totals;
+//   pass 2: I_out[i] = base_out[c] + sum_j mo[i,j] g_out_j,
+//           I_in[i]  = base_in[c]  + sum_j mi[i,j] g_in_j,
+//           Y[i] = r_i^-(k+1) I_out[i] + r_i^k I_in[i].
+// The row is read twice (the second read hits L2) and Y written once.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void yk_batch_kernel(
+    const double *__restrict__ rho, int64_t ld_rho, const double *__restrict__ w,
+    const double *__restrict__ rk_tab, const double *__restrict__ rkm_tab, int64_t n, int nv,
+    int ng, int kmax, const double *__restrict__ mo, const double *__restrict__ mi, int n_items,
+    const int32_t *__restrict__ row, const int32_t *__restrict__ kk,
+    const int32_t *__restrict__ out_row, double *__restrict__ y, int64_t ld_y) {
+    extern __shared__ double cell_sums[];  // [2*nv]: S_out then S_in
+    __shared__ double mo_s[kMaxNg * kMaxNg], mi_s[kMaxNg * kMaxNg];
+    __shared__ double g_out[kThreads], g_in[kThreads];
+    __shared__ double tot_a[kThreads], tot_b[kThreads];
+
+    double *s_out = cell_sums;
+    double *s_in = cell_sums + nv;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < ng * ng; t += kThreads) {
+        mo_s[t] = mo[t];
+        mi_s[t] = mi[t];
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    const int cpw = 64 / ng;           // cells per wave
+    const int cpb = cpw * (kThreads / 64);  // cells per block pass
+    const int cell_in_wave = lane / ng, node = lane - cell_in_wave * ng;
+    const bool lane_used = lane < cpw * ng;
+    const int lane_cell_base = tid - node;  // LDS slot of node 0 of this lane's cell
+    const int seg = (nv + kThreads - 1) / kThreads;
+    __syncthreads();
+
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const double *src = rho + (int64_t)row[it] * ld_rho;
+        const int k = kk[it];
+        if (k < 0 || k > kmax) {
+            // rank outside the r-power tables: poison the row instead of reading past them
+            double *bad = y + (int64_t)out_row[it] * ld_y;
+            for (int64_t x = tid; x < n; x += kThreads) bad[x] = __builtin_nan("");
+            continue;
+        }
+        const double *rkp = rk_tab + (int64_t)k * n;
+        const double *rkmp = rkm_tab + (int64_t)k * n;
+
+        // pass 1: cell sums
+        for (int c0 = 0; c0 < nv; c0 += cpb) {
+            const int cell = c0 + wave * cpw + cell_in_wave;
+            double go = 0.0, gi = 0.0;
+            if (lane_used && cell < nv) {
+                const int64_t x = (int64_t)cell * ng + node;
+                double d = src[x];
+                if (w) d = w[x] * d;
+                go = d * rkp[x];
+                gi = d * rkmp[x];
+            }
+            g_out[tid] = go;
+            g_in[tid] = gi;
+            __syncthreads();
+            if (lane_used && node == 0 && cell < nv) {
+                double so = 0.0, si = 0.0;
+                for (int j = 0; j < ng; ++j) {
+                    so += g_out[tid + j];
+                    si += g_in[tid + j];
+                }
+                s_out[cell] = so;
+                s_in[cell] = si;
+            }
+            __syncthreads();
+        }
+
+        // scan of cell sums: per-thread contiguous segments
+        const int c_lo = tid * seg;
+        const int c_hi = min(c_lo + seg, nv);
+        double a = 0.0, b = 0.0;
+        for (int c = c_lo; c < c_hi; ++c) {
+            a += s_out[c];
+            b += s_in[c];
+        }
+        tot_a[tid] = a;
+        tot_b[kThreads - 1 - tid] = b;  // reversed: suffix becomes prefix
+        __syncthreads();
+        for (int off = 1; off < kThreads; off <<= 1) {
+            double va = (tid >= off) ? tot_a[tid - off] : 0.0;
+            double vb = (tid >= off) ? tot_b[tid - off] : 0.0;
+            __syncthreads();
+            tot_a[tid] += va;
+            tot_b[tid] += vb;
+            __syncthreads();
+        }
+        // exclusive bases read from the neighbouring inclusive sums (no
+        // subtraction of nearly equal partial sums)
+        double base_o = (tid == 0) ? 0.0 : tot_a[tid - 1];
+        double base_i = (tid == kThreads - 1) ? 0.0 : tot_b[kThreads - 2 - tid];
+        __syncthreads();
+        for (int c = c_lo; c < c_hi; ++c) {
+            double s = s_out[c];
+            s_out[c] = base_o;
+            base_o += s;
+        }
+        for (int c = c_hi - 1; c >= c_lo; --c) {
+            double s = s_in[c];
+            s_in[c] = base_i;
+            base_i += s;
+        }
+        __syncthreads();
+
+        // pass 2: in-cell partials and Y
+        double *dst = y + (int64_t)out_row[it] * ld_y;
+        for (int c0 = 0; c0 < nv; c0 += cpb) {
+            const int cell = c0 + wave * cpw + cell_in_wave;
+            const bool live = lane_used && cell < nv;
+            const int64_t x = (int64_t)cell * ng + node;
+            double rkv = 0.0, rkmv = 0.0;
+            double go = 0.0, gi = 0.0;
+            if (live) {
+                double d = src[x];
+                if (w) d = w[x] * d;
+                rkv = rkp[x];
+                rkmv = rkmp[x];
+                go = d * rkv;
+                gi = d * rkmv;
+            }
+            g_out[tid] = go;
+            g_in[tid] = gi;
+            __syncthreads();
+            if (live) {
+                double po = 0.0, pi = 0.0;
+                for (int j = 0; j < ng; ++j) {
+                    po += mo_s[node * ng + j] * g_out[lane_cell_base + j];
+                    pi += mi_s[node * ng + j] * g_in[lane_cell_base + j];
+                }
+                const double iout = s_out[cell] + po;
+                const double iin = s_in[cell] + pi;
+                dst[x] = rkmv * iout + rkv * iin;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// R[t] = sum_x D[p_t, x] * Y[q_t, x]  (numerics/bspline/slater.py:129-130)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void rk_dot_kernel(
+    const double *__restrict__ D, int64_t ld_d, const double *__restrict__ Y, int64_t ld_y,
+    int64_t n, int n_quart, const int32_t *__restrict__ p, const int32_t *__restrict__ q,
+    double *__restrict__ out) {
+    __shared__ double red[kThreads];
+    for (int t = blockIdx.x; t < n_quart; t += gridDim.x) {
+        const double *a = D + (int64_t)p[t] * ld_d;
+        const double *b = Y + (int64_t)q[t] * ld_y;
+        double s = 0.0;
+        for (int64_t x = threadIdx.x; x < n; x += kThreads) s += a[x] * b[x];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = kThreads / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[t] = red[0];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Orbital samples f(x) = sum_s c[iv+s] B_s(x)  (numerics/bspline/evaluate.py:71-97)
+// ---------------------------------------------------------------------------
+__global__ void bspline_eval_kernel(const double *__restrict__ coeffs, int64_t ld_c, int n_orb,
+                                    const double *__restrict__ bsp, int nv, int ng, int ks,
+                                    int ns, double *__restrict__ out, int64_t ld_out) {
+    const int64_t n = (int64_t)nv * ng;
+    for (int o = blockIdx.y; o < n_orb; o += gridDim.y) {
+        const double *c = coeffs + (int64_t)o * ld_c;
+        for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+             x += (int64_t)gridDim.x * blockDim.x) {
+            const int iv = static_cast<int>(x / ng);
+            const double *b = bsp + x * ks;
+            double s = 0.0;
+            for (int sidx = 0; sidx < ks; ++sidx) {
+                const int gi = iv + sidx;
+                const double cv = gi < ns ? c[gi] : 0.0;
+                s += cv * b[sidx];
+            }
+            out[(int64_t)o * ld_out + x] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Y^k-form exchange matrix (numerics/bspline/slater_yk.py:35-122).
+// One work-group per column n_idx: the product density fb*B_n is nonzero on
+// at most ks cells, so the node-wise cumulative integrals are constant
+// outside that support; the column is assembled over all cells.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void exchange_yk_kernel(
+    const double *__restrict__ fb, const double *__restrict__ bsp,
+    const double *__restrict__ r, const double *__restrict__ w, int nv, int ng, int ks, int ns,
+    const double *__restrict__ rk, const double *__restrict__ rkm, double *__restrict__ K) {
+    extern __shared__ double sm[];
+    const int64_t n = (int64_t)nv * ng;
+    for (int nidx = blockIdx.x; nidx < ns; nidx += gridDim.x) {
+        int lo = nidx - ks + 1;
+        if (lo < 0) lo = 0;
+        int hi = nidx + 1;
+        if (hi > nv) hi = nv;
+        const int64_t x_lo = (int64_t)lo * ng, x_hi = (int64_t)hi * ng;
+        const int len = hi > lo ? static_cast<int>(x_hi - x_lo) : 0;
+        double *iout = sm;        // [len] inclusive prefix inside the support
+        double *iin = sm + len;   // [len] inclusive suffix inside the support
+        __shared__ double tot_out_s, tot_in_s;
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int t = 0; t < len; ++t) {
+                const int64_t x = x_lo + t;
+                const int iv = static_cast<int>(x / ng);
+                const double rho = fb[x] * bsp[x * ks + (nidx - iv)];
+                s += w[x] * rho * rk[x];
+                iout[t] = s;
+            }
+            tot_out_s = s;
+            s = 0.0;
+            for (int t = len - 1; t >= 0; --t) {
+                const int64_t x = x_lo + t;
+                const int iv = static_cast<int>(x / ng);
+                const double rho = fb[x] * bsp[x * ks + (nidx - iv)];
+                s += w[x] * rho * rkm[x];
+                iin[t] = s;
+            }
+            tot_in_s = s;
+        }
+        __syncthreads();
+        const double tot_out = tot_out_s, tot_in = tot_in_s;
+        // K[m, nidx] = sum_{iv+s=m} sum_g (w fb)(x) B_s(x) Y(x), iv ascending
+        for (int m = threadIdx.x; m < ns; m += blockDim.x) {
+            double acc_m = 0.0;
+            for (int sidx = ks - 1; sidx >= 0; --sidx) {
+                const int iv = m - sidx;
+                if (iv < 0 || iv >= nv) continue;
+                double acc = 0.0;
+                for (int g = 0; g < ng; ++g) {
+                    const int64_t x = (int64_t)iv * ng + g;
+                    double io, ii;
+                    if (x < x_lo) {
+                        io = 0.0;
+                        ii = tot_in;
+                    } else if (x >= x_hi) {
+                        io = tot_out;
+                        ii = 0.0;
+                    } else {
+                        io = iout[x - x_lo];
+                        ii = iin[x - x_lo];
+                    }
+                    const double yk = rkm[x] * io + rk[x] * ii;
+                    acc += (w[x] * fb[x]) * bsp[x * ks + sidx] * yk;
+                }
+                acc_m += acc;
+            }
+            K[(int64_t)m * ns + nidx] = acc_m;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// "direct" O(N^2) double quadrature with the coincident-cell cusp split
+// (numerics/bspline/slater.py:51-115).  Disjoint cells: product rule with the
+// bare kernel; coincident cells: the triangle rule r = a + h x, s = a + h x y
+// with densities from their cell interpolants (tables outer/inner built on
+// the host).  Partial sums per block/cell, then a fixed-order final sum.
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double *red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kThreads) void direct_disjoint_kernel(
+    const double *__restrict__ rho13, const double *__restrict__ rho24,
+    const double *__restrict__ r, const double *__restrict__ w, int64_t n, int ng, int k,
+    double *__restrict__ partial) {
+    __shared__ double red[kThreads];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int64_t j0 = (i / ng + 1) * ng;
+        const double ri = r[i], wi = w[i], a13 = rho13[i], a24 = rho24[i];
+        for (int64_t j = j0 + threadIdx.x; j < n; j += kThreads) {
+            const double kern = pow(ri / r[j], static_cast<double>(k)) / r[j];
+            acc += wi * w[j] * kern * (a13 * rho24[j] + rho13[j] * a24);
+        }
+    }
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void direct_diag_kernel(
+    const double *__restrict__ rho13, const double *__restrict__ rho24,
+    const double *__restrict__ gr, const double *__restrict__ grw, int nv, int ng, int k,
+    int order, const double *__restrict__ xq, const double *__restrict__ wq,
+    const double *__restrict__ outer, const double *__restrict__ inner, double node0,
+    double *__restrict__ partial) {
+    __shared__ double red[kThreads];
+    __shared__ double fo[64], go[64];
+    for (int c = blockIdx.x; c < nv; c += gridDim.x) {
+        const double *f = rho13 + (int64_t)c * ng;
+        const double *g = rho24 + (int64_t)c * ng;
+        double h = 0.0;
+        for (int m = 0; m < ng; ++m) h += grw[(int64_t)c * ng + m];
+        const double a = gr[(int64_t)c * ng] - h * (1.0 + node0) / 2.0;
+        for (int x = threadIdx.x; x < order; x += blockDim.x) {
+            double sf = 0.0, sg = 0.0;
+            for (int j = 0; j < ng; ++j) {
+                sf += f[j] * outer[x * ng + j];
+                sg += g[j] * outer[x * ng + j];
+            }
+            fo[x] = sf;
+            go[x] = sg;
+        }
+        __syncthreads();
+        double acc = 0.0;
+        for (int t = threadIdx.x; t < order * order; t += blockDim.x) {
+            const int x = t / order, y = t - (t / order) * order;
+            const double *in = inner + (int64_t)t * ng;
+            double fi = 0.0, gi = 0.0;
+            for (int j = 0; j < ng; ++j) {
+                fi += f[j] * in[j];
+                gi += g[j] * in[j];
+            }
+            const double rr = a + h * xq[x];
+            const double ss = a + h * xq[x] * xq[y];
+            const double kj = pow(ss / rr, static_cast<double>(k)) / rr * (h * h) * xq[x];
+            const double dens = fo[x] * gi + go[x] * fi;
+            acc += dens * kj * wq[x] * wq[y];
+        }
+        const double s = block_sum(acc, red);
+        if (threadIdx.x == 0) partial[c] = s;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double *__restrict__ a,
+                                                                int na,
+                                                                const double *__restrict__ b,
+                                                                int nb, double *__restrict__ out) {
+    __shared__ double red[kThreads];
+    double s = 0.0;
+    for (int t = threadIdx.x; t < na; t += kThreads) s += a[t];
+    const double sa = block_sum(s, red);
+    s = 0.0;
+    for (int t = threadIdx.x; t < nb; t += kThreads) s += b[t];
+    const double sb = block_sum(s, red);
+    if (threadIdx.x == 0) out[0] = sa + sb;
+}
+
+}  // namespace
+
+extern "C" {
+
+ax_status ax_grid_rpow(const double *r, int64_t n, int kmax, double *rk, double *rkm,
+                       void *stream) {
+    AX_REQUIRE(n > 0 && kmax >= 0, "ax_grid_rpow: bad sizes n=%lld kmax=%d", (long long)n, kmax);
+    hipLaunchKernelGGL(rpow_kernel, dim3(ax::grid_for(n * (kmax + 1), 256, 4096)), dim3(256), 0,
+                       ax::as_stream(stream), r, n, kmax, rk, rkm);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_pair_density(const double *P, const double *Q, int64_t ld_pq, int64_t n,
+                          int n_pairs, const int32_t *pa, const int32_t *pb, const double *w,
+                          double *out, int64_t ld_out, void *stream) {
+    AX_REQUIRE(n > 0 && n_pairs >= 0 && ld_pq >= n && ld_out >= n,
+               "ax_pair_density: bad sizes");
+    if (n_pairs == 0) return AX_OK;
+    dim3 grid(ax::grid_for(n, 256, 64), n_pairs < 65535 ? n_pairs : 65535);
+    hipLaunchKernelGGL(pair_density_kernel, grid, dim3(256), 0, ax::as_stream(stream), P, Q,
+                       ld_pq, n, n_pairs, pa, pb, w, out, ld_out);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_yk_batch(const double *rho, int64_t ld_rho, const double *w, const double *rk,
+                      const double *rkm, int64_t n, int nv, int ng, int kmax, const double *mo,
+                      const double *mi, int n_items, const int32_t *row, const int32_t *kk,
+                      const int32_t *out_row, double *y, int64_t ld_y, void *stream) {
+    AX_REQUIRE(ng >= 1 && ng <= kMaxNg, "ax_yk_batch: n_gauss must be in [1, %d], got %d",
+               kMaxNg, ng);
+    AX_REQUIRE(nv >= 1 && n == (int64_t)nv * ng, "ax_yk_batch: n (%lld) != nv*ng (%d*%d)",
+               (long long)n, nv, ng);
+    AX_REQUIRE(ld_rho >= n && ld_y >= n, "ax_yk_batch: leading dimension below n");
+    AX_REQUIRE((size_t)2 * nv * sizeof(double) <= 48 * 1024,
+               "ax_yk_batch: nv=%d exceeds the LDS cell-sum capacity", nv);
+    if (n_items <= 0) return AX_OK;
+    const size_t shmem = (size_t)2 * nv * sizeof(double);
+    const int grid = n_items < 4096 ? n_items : 4096;
+    hipLaunchKernelGGL(yk_batch_kernel, dim3(grid), dim3(kThreads), shmem, ax::as_stream(stream),
+                       rho, ld_rho, w, rk, rkm, n, nv, ng, kmax, mo, mi, n_items, row, kk, out_row,
+                       y, ld_y);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_rk_dot(const double *D, int64_t ld_d, const double *Y, int64_t ld_y, int64_t n,
+                    int n_quart, const int32_t *p, const int32_t *q, double *out, void *stream) {
+    AX_REQUIRE(n > 0 && n_quart >= 0, "ax_rk_dot: bad sizes");
+    if (n_quart == 0) return AX_OK;
+    const int grid = n_quart < 8192 ? n_quart : 8192;
+    hipLaunchKernelGGL(rk_dot_kernel, dim3(grid), dim3(kThreads), 0, ax::as_stream(stream), D,
+                       ld_d, Y, ld_y, n, n_quart, p, q, out);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_bspline_eval(const double *coeffs, int64_t ld_c, int n_orb, const double *bsp,
+                          int nv, int ng, int ks, int ns, double *out, int64_t ld_out,
+                          void *stream) {
+    AX_REQUIRE(nv > 0 && ng > 0 && ks > 0 && ns > 0 && n_orb >= 0 && ld_c >= ns &&
+                   ld_out >= (int64_t)nv * ng,
+               "ax_bspline_eval: bad sizes");
+    if (n_orb == 0) return AX_OK;
+    dim3 grid(ax::grid_for((int64_t)nv * ng, 256, 64), n_orb < 65535 ? n_orb : 65535);
+    hipLaunchKernelGGL(bspline_eval_kernel, grid, dim3(256), 0, ax::as_stream(stream), coeffs,
+                       ld_c, n_orb, bsp, nv, ng, ks, ns, out, ld_out);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_exchange_matrix_yk(const double *fb, const double *bsp, const double *r,
+                                const double *w, int nv, int ng, int ks, int ns, int k,
+                                double *K, void *stream) {
+    AX_REQUIRE(nv > 0 && ng > 0 && ks > 0 && ns > 0 && k >= 0, "ax_exchange_matrix_yk: bad sizes");
+    AX_REQUIRE((size_t)2 * ks * ng * sizeof(double) <= 32 * 1024,
+               "ax_exchange_matrix_yk: ks*n_gauss too large");
+    const int64_t n = (int64_t)nv * ng;
+    // r^j and r^-(j+1) for j = 0..k; the kernel reads the rank-k rows
+    double *tab = nullptr;
+    AX_CHECK_HIP(hipMallocAsync(&tab, sizeof(double) * 2 * n * (k + 1), ax::as_stream(stream)));
+    double *rk_all = tab, *rkm_all = tab + n * (k + 1);
+    hipLaunchKernelGGL(rpow_kernel, dim3(ax::grid_for(n * (k + 1), 256, 4096)), dim3(256), 0,
+                       ax::as_stream(stream), r, n, k, rk_all, rkm_all);
+    AX_CHECK_LAUNCH();
+    const size_t shmem = (size_t)2 * ks * ng * sizeof(double);
+    hipLaunchKernelGGL(exchange_yk_kernel, dim3(ns < 1024 ? ns : 1024), dim3(kThreads), shmem,
+                       ax::as_stream(stream), fb, bsp, r, w, nv, ng, ks, ns, rk_all + n * k,
+                       rkm_all + n * k, K);
+    AX_CHECK_LAUNCH();
+    AX_CHECK_HIP(hipFreeAsync(tab, ax::as_stream(stream)));
+    return AX_OK;
+}
+
+ax_status ax_slater_rk_direct(const double *rho13, const double *rho24, const double *gr,
+                              const double *grw, int nv, int ng, int k, int order,
+                              const double *xq, const double *wq, const double *outer,
+                              const double *inner, double node0, double *workspace, double *out,
+                              void *stream) {
+    AX_REQUIRE(nv > 0 && ng > 0 && k >= 0 && order > 0 && order <= 64,
+               "ax_slater_rk_direct: bad sizes (nv=%d ng=%d k=%d order=%d)", nv, ng, k, order);
+    const int64_t n = (int64_t)nv * ng;
+    const int g1 = n < 1024 ? static_cast<int>(n) : 1024;
+    hipLaunchKernelGGL(direct_disjoint_kernel, dim3(g1), dim3(kThreads), 0, ax::as_stream(stream),
+                       rho13, rho24, gr, grw, n, ng, k, workspace);
+    AX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(direct_diag_kernel, dim3(nv < 4096 ? nv : 4096), dim3(kThreads), 0,
+                       ax::as_stream(stream), rho13, rho24, gr, grw, nv, ng, k, order, xq, wq,
+                       outer, inner, node0, workspace + 1024);
+    AX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kThreads), 0, ax::as_stream(stream),
+                       workspace, g1, workspace + 1024, nv, out);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+"""Batched all-quartet R^k tensor on one grid (device-resident plan).
+
+Wraps ``ax_rk_tensor_*``: for orbital samples P, Q (n_orb, n) on the grid it
+returns every R^k(ab;cd), k = 0..kmax, over pair-pairs valid by the triangle
+and parity rules — R^k(ab;cd) = sum_x grw rho_ab Y^k[rho_cd] for pair(ab) <=
+pair(cd) (numerics/bspline/slater.py:118-130 per integral).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import device as dv
+from ._lib import PRECISIONS, call, load
+from .workloads import pair_list, valid_pairs
+
+
+class SlaterTensor:
+    def __init__(self, gr, grw, ls, kmax, precision="fp64"):
+        dv.require_gpu()
+        lib = load()
+        gr = np.ascontiguousarray(gr, dtype=np.float64)
+        grw = np.ascontiguousarray(grw, dtype=np.float64)
+        if gr.ndim != 2 or gr.shape != grw.shape:
+            raise ValueError("grid and weights must both be (nv, n_gauss)")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {tuple(PRECISIONS)}")
+        self.nv, self.ng = gr.shape
+        self.n = self.nv * self.ng
+        self.ls = np.ascontiguousarray(ls, dtype=np.int32)
+        self.n_orb = int(self.ls.size)
+        self.kmax = int(kmax)
+        self.precision = precision
+        self._grw = grw.ravel().copy()
+        h = ctypes.c_void_p()
+        dp = ctypes.POINTER(ctypes.c_double)
+        call("ax_rk_tensor_create", gr.ravel().ctypes.data_as(dp), grw.ravel().ctypes.data_as(dp),
+             self.nv, self.ng, self.n_orb,
+             self.ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), self.kmax,
+             PRECISIONS[precision], ctypes.byref(h))
+        self._h = h
+        self._lib = lib
+        self.size = int(lib.ax_rk_tensor_size(h))
+        self.m = [int(lib.ax_rk_tensor_pairs(h, k)) for k in range(self.kmax + 1)]
+        self.offsets = [int(lib.ax_rk_tensor_offset(h, k)) for k in range(self.kmax + 1)]
+        self.footprint = int(lib.ax_rk_tensor_footprint(h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.ax_rk_tensor_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, P, Q, out=None, stream=None):
+        """P, Q device tensors (n_orb, n) FP64 (or host arrays); returns the
+        packed output (device tensor of ``size`` values)."""
+        P = dv.f64(P)
+        Q = dv.f64(Q)
+        if P.shape != (self.n_orb, self.n) or Q.shape != (self.n_orb, self.n):
+            raise ValueError(f"orbitals must be ({self.n_orb}, {self.n})")
+        if out is None:
+            out = dv.empty((self.size,))
+        call("ax_rk_tensor_run", self._h, dv.ptr(P), dv.ptr(Q), dv.ptr(out),
+             dv.stream_handle(stream))
+        return out
+
+    STAGES = ("density", "scan", "slice", "contract", "reduce")
+
+    def run_stage(self, stage, P=None, Q=None, out=None, stream=None):
+        """Launches one stage of ``run`` (per-kernel timing); P/Q/out must be
+        the same device tensors as in the run being repeated."""
+        call("ax_rk_tensor_run_stage", self._h, self.STAGES.index(stage), dv.ptr(P), dv.ptr(Q),
+             dv.ptr(out), dv.stream_handle(stream))
+
+    # -- host-side views of the packed output -----------------------------
+    def rank_pairs(self, k):
+        return valid_pairs(self.ls, k)
+
+    def quartets(self, k):
+        """(p, q) pair indices of rank k's packed values (p <= q in V_k order)."""
+        V = self.rank_pairs(k)
+        i, j = np.triu_indices(len(V))
+        return V[i], V[j]
+
+    def unpack(self, out_host, k):
+        """Rank k's values as a symmetric (m_k, m_k) matrix over its valid pairs."""
+        m = self.m[k]
+        vals = out_host[self.offsets[k]:self.offsets[k] + m * (m + 1) // 2]
+        M = np.zeros((m, m))
+        i, j = np.triu_indices(m)
+        M[i, j] = vals
+        M[j, i] = vals
+        return M
+
+    @staticmethod
+    def pairs(n_orb):
+        return pair_list(n_orb)

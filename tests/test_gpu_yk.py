@@ -1,0 +1,223 @@
+"""GPU parity of the (Y) Poisson path against the CPU oracle and closed forms.
+
+Tolerances: FP64 GPU results agree with the FP64 oracle to 1e-12 relative
+(summation order differs, the arithmetic does not); the split-precision
+contraction to 1e-11 relative to the absolute integrand sum_x |D||Y| and to
+1e-11 elementwise wherever |R| >= 1e-3 sum_x |D||Y| (no heavy cancellation).
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import RK_1S, physics_workspace, rho1s
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b, floor=0.0):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+@pytest.fixture(scope="module")
+def c2_small():
+    from atomx_amd.workloads import synthetic_grid, synthetic_orbitals
+    gr, grw = synthetic_grid(nv=100, ng=8, r_min=1e-5, r_max=60.0)
+    P, Q, ls = synthetic_orbitals(gr, grw, n_orb=12, seed=11)
+    return gr, grw, P, Q, ls
+
+
+@pytest.fixture(scope="module")
+def h_grid():
+    return physics_workspace(Z=1, ks=9)[0]
+
+
+def test_yk_flat_matches_oracle(c2_small):
+    from atomx_amd import yk
+    gr, grw, P, Q, ls = c2_small
+    nv, ng = gr.shape
+    r, w = gr.ravel(), grw.ravel()
+    mo, mi = O.yk_partial_matrices(ng)
+    for (a, b) in ((0, 0), (1, 5), (3, 7), (10, 11)):
+        rho = P[a] * P[b] + Q[a] * Q[b]
+        for k in (0, 1, 2, 5, 8):
+            got = yk.yk_flat(rho, r, w, r.size, k, nv, ng, mo, mi)
+            ref = O.yk_flat(rho, r, w, r.size, k, nv, ng, mo, mi)
+            scale = np.abs(ref).max()
+            big = np.abs(ref) >= 1e-10 * scale
+            assert _rel(got[big], ref[big]) < 1e-12, (a, b, k)
+            assert np.max(np.abs(got - ref)) <= 1e-12 * scale
+
+
+def test_compute_yk_complex_density(c2_small):
+    from atomx_amd import yk
+    gr, grw, P, Q, ls = c2_small
+    nv, ng = gr.shape
+    rho = (P[0] * P[2] + Q[0] * Q[2]) + 0.3j * (P[1] * P[1] + Q[1] * Q[1])
+    got = yk.compute_yk(rho.reshape(nv, ng), gr, grw, nv, ng, 2)
+    ref = (O.compute_yk(rho.real.reshape(nv, ng), gr, grw, nv, ng, 2)
+           + 1j * O.compute_yk(rho.imag.reshape(nv, ng), gr, grw, nv, ng, 2))
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_hydrogenic_closed_forms_gpu(h_grid, k):
+    from atomx_amd.slater import slater_rk
+    rho = rho1s(h_grid["gr"])
+    got = slater_rk(rho, rho, h_grid, k, method="poisson")
+    assert got == pytest.approx(RK_1S[k], rel=1e-11)
+    ref = O.slater_rk_poisson(rho, rho, h_grid["gr"], h_grid["grw"], h_grid["nv"], 9, k)
+    assert got == pytest.approx(ref, rel=1e-13)
+
+
+def test_yk_pointwise_hydrogenic_gpu(h_grid):
+    from atomx_amd.yk import compute_yk
+    gr, grw, nv = h_grid["gr"], h_grid["grw"], h_grid["nv"]
+    for k in (0, 1, 2):
+        got = compute_yk(rho1s(gr), gr, grw, nv, 9, k)
+        ref = O.compute_yk(rho1s(gr), gr, grw, nv, 9, k)
+        assert _rel(got[1:], ref[1:]) < 1e-13
+
+
+def test_slater_direct_gpu_matches_oracle(h_grid):
+    from atomx_amd.slater import slater_rk, slater_rk_direct
+    gr, grw = h_grid["gr"], h_grid["grw"]
+    r = gr
+    a = r * np.exp(-2.0 * r) * (1 - 0.5 * r) * r ** 2 * np.exp(-0.9 * r)
+    b = r * np.exp(-1.3 * r) * r ** 3 * np.exp(-0.7 * r)
+    for k in (0, 1, 3, 6):
+        got = slater_rk(a, b, h_grid, k, method="direct")
+        ref = O.slater_rk_direct(a, b, gr, grw, k)
+        assert got == pytest.approx(ref, rel=1e-12)
+    with pytest.raises(ValueError, match="unknown Slater integrator"):
+        slater_rk(a, b, h_grid, 1, method="simpson")
+    x, w = np.polynomial.legendre.leggauss(9)
+    rr, ww = (0.7 * (x + 1) / 2)[None, :], (0.7 * w / 2)[None, :]
+    expected = 0.7 ** 11 / 11 * (1 / 8 + 1 / 10)
+    assert slater_rk_direct(rr ** 4, rr ** 6, rr, ww, 3) == pytest.approx(expected, rel=2e-13)
+
+
+def test_compute_slater_rk_from_coefficients(h_grid):
+    from atomx_amd import slater_yk
+    ns = h_grid["ns"]
+    rng = np.random.default_rng(17)
+    vecs = [rng.normal(size=ns) * np.exp(-np.arange(ns) / 20.0) for _ in range(8)]
+    for k in (0, 2, 4):
+        got = slater_yk.compute_slater_rk(*vecs, h_grid, k)
+        ref = O.compute_slater_rk(*vecs, h_grid, k)
+        assert got == pytest.approx(ref, rel=1e-12, abs=1e-13 * abs(ref) + 1e-300)
+    gk = slater_yk.compute_slater_gk(vecs[0], vecs[1], vecs[2], vecs[3], h_grid, 1)
+    ref = O.compute_slater_rk(vecs[0], vecs[1], vecs[2], vecs[3], vecs[2], vecs[3], vecs[0],
+                              vecs[1], h_grid, 1)
+    assert gk == pytest.approx(ref, rel=1e-12)
+
+
+def test_exchange_matrix_gpu_matches_oracle(h_grid):
+    from atomx_amd import slater_yk
+    ns = h_grid["ns"]
+    rng = np.random.default_rng(3)
+    P = rng.normal(size=ns) * np.exp(-np.arange(ns) / 15.0)
+    Q = 0.01 * rng.normal(size=ns) * np.exp(-np.arange(ns) / 15.0)
+    for k in (0, 1, 2):
+        kpp, kqq = slater_yk.compute_exchange_matrix(P, Q, h_grid, k)
+        rpp, rqq = O.compute_exchange_matrix(P, Q, h_grid, k)
+        assert np.max(np.abs(kpp - rpp)) <= 1e-12 * np.abs(rpp).max()
+        assert np.max(np.abs(kqq - rqq)) <= 1e-12 * np.abs(rqq).max()
+
+
+def _tensor_ref(plan, P, Q, gr, grw, k):
+    a, b = plan.pairs(P.shape[0])
+    rho = P[a] * P[b] + Q[a] * Q[b]
+    p, q = plan.quartets(k)
+    ref = O.rk_quartets(rho, gr, grw, gr.shape[0], gr.shape[1], p, q, np.full(len(p), k))
+    w = grw.ravel()
+    return ref, rho, p, q
+
+
+def _abs_scale(rho, w, gr, grw, p, q, k):
+    """sum_x |w rho_p| |Y^k[rho_q]| per quartet (the conditioning scale)."""
+    nv, ng = gr.shape
+    mo, mi = O.yk_partial_matrices(ng)
+    ys = {}
+    out = np.empty(len(p))
+    for t in range(len(p)):
+        if q[t] not in ys:
+            ys[q[t]] = np.abs(O.yk_flat(rho[q[t]], gr.ravel(), w, w.size, k, nv, ng, mo, mi))
+        out[t] = np.sum(np.abs(w * rho[p[t]]) * ys[q[t]])
+    return out
+
+
+def test_tensor_fp64_matches_oracle(c2_small):
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    gr, grw, P, Q, ls = c2_small
+    plan = SlaterTensor(gr, grw, ls, 8, precision="fp64")
+    out = dv.host(plan.run(P, Q))
+    assert out.size == plan.size
+    for k in range(9):
+        if plan.m[k] == 0:
+            continue
+        ref, rho, p, q = _tensor_ref(plan, P, Q, gr, grw, k)
+        got = out[plan.offsets[k]:plan.offsets[k] + len(p)]
+        scale = _abs_scale(rho, grw.ravel(), gr, grw, p, q, k)
+        assert np.max(np.abs(got - ref) / scale) < 1e-13, k
+        big = np.abs(ref) >= 1e-6 * scale
+        assert _rel(got[big], ref[big]) < 1e-12, k
+    plan.close()
+
+
+def test_tensor_split_matches_oracle(c2_small):
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    gr, grw, P, Q, ls = c2_small
+    plan = SlaterTensor(gr, grw, ls, 8, precision="split")
+    out = dv.host(plan.run(P, Q))
+    worst_scaled = worst_rel = 0.0
+    for k in range(9):
+        if plan.m[k] == 0:
+            continue
+        ref, rho, p, q = _tensor_ref(plan, P, Q, gr, grw, k)
+        got = out[plan.offsets[k]:plan.offsets[k] + len(p)]
+        scale = _abs_scale(rho, grw.ravel(), gr, grw, p, q, k)
+        worst_scaled = max(worst_scaled, float(np.max(np.abs(got - ref) / scale)))
+        big = np.abs(ref) >= 1e-3 * scale
+        worst_rel = max(worst_rel, _rel(got[big], ref[big]))
+    print(f"split: max err / sum|D||Y| = {worst_scaled:.2e}, max rel (|R|>=1e-3 scale) = {worst_rel:.2e}")
+    assert worst_scaled < 1e-11
+    assert worst_rel < 1e-11
+    plan.close()
+
+
+def test_rk_gemm_block_matches_numpy():
+    """ax_rk_gemm on random operands: packed upper triangle of D[rows] Y^T."""
+    import ctypes
+
+    from atomx_amd import device as dv
+    from atomx_amd._lib import PRECISIONS, call, load
+    rng = np.random.default_rng(2)
+    for m, n in ((1, 7), (37, 100), (130, 1000)):
+        D = rng.normal(size=(m + 5, n))
+        Y = rng.normal(size=(m, n))
+        rows = rng.permutation(m + 5)[:m].astype(np.int32)
+        ref = D[rows] @ Y.T
+        i, j = np.triu_indices(m)
+        for prec in ("fp64", "split"):
+            nbytes = load().ax_rk_gemm_workspace_bytes(m, n, PRECISIONS[prec])
+            ws = dv.empty((max(nbytes, 8) // 8 + 1,))
+            out = dv.empty((m * (m + 1) // 2,))
+            Dd, Yd, rd = dv.f64(D), dv.f64(Y), dv.i32(rows)
+            call("ax_rk_gemm", dv.ptr(Dd), n, dv.ptr(rd), dv.ptr(Yd), n, m, n, PRECISIONS[prec],
+                 dv.ptr(ws), dv.ptr(out), dv.stream_handle())
+            got = dv.host(out)
+            absref = (np.abs(D[rows]) @ np.abs(Y).T)[i, j]
+            tol = 1e-14 if prec == "fp64" else 1e-11
+            assert np.max(np.abs(got - ref[i, j]) / absref) < tol, (m, n, prec)
+
+
+def test_invalid_arguments_raise():
+    from atomx_amd import yk
+    with pytest.raises(ValueError):
+        yk.yk_flat(np.ones(10), np.ones(10), np.ones(10), 10, -1, 2, 5)
+    with pytest.raises(ValueError):
+        yk.yk_flat(np.ones(10), np.ones(10), np.ones(10), 10, 0, 3, 5)
