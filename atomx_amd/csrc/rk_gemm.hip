@@ -19,38 +19,40 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64;
-constexpr int BK = 16;
+constexpr int BK = 32;
 constexpr int LDS_LD = BM + 4;  // padded row (in doubles) of the k-major LDS tiles
 
-// Loads one BK x BM k-major tile of rows [row0, row0+64) x [k0, k0+16) into
-// registers: thread t -> row t/4, k sub-block (t%4)*4 .. +3.
+constexpr int KPT = BK * BM / 256;  // k values per thread in one staged tile (8)
+
+// Loads one BK x BM k-major tile of rows [row0, row0+64) x [k0, k0+BK) into
+// registers: thread t -> row t/4, k sub-block (t%4)*KPT .. +KPT-1.
 __device__ inline void load_tile_regs(const double *__restrict__ base, int64_t ld,
                                       const int32_t *__restrict__ rows, int row0, int m,
-                                      int64_t k0, int64_t kend, double (&v)[4]) {
+                                      int64_t k0, int64_t kend, double (&v)[KPT]) {
     const int t = threadIdx.x;
     const int r = t >> 2;
-    const int kq = (t & 3) * 4;
+    const int kq = (t & 3) * KPT;
     const int gr = row0 + r;
     if (gr < m) {
         const int64_t rr = rows ? (int64_t)rows[gr] : (int64_t)gr;
         const double *p = base + rr * ld;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < KPT; ++q) {
             const int64_t kx = k0 + kq + q;
             v[q] = kx < kend ? p[kx] : 0.0;
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = 0.0;
+        for (int q = 0; q < KPT; ++q) v[q] = 0.0;
     }
 }
 
-__device__ inline void store_tile_lds(double *__restrict__ s, const double (&v)[4]) {
+__device__ inline void store_tile_lds(double *__restrict__ s, const double (&v)[KPT]) {
     const int t = threadIdx.x;
     const int r = t >> 2;
-    const int kq = (t & 3) * 4;
+    const int kq = (t & 3) * KPT;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) s[(kq + q) * LDS_LD + r] = v[q];
+    for (int q = 0; q < KPT; ++q) s[(kq + q) * LDS_LD + r] = v[q];
 }
 
 __global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::RkSeg *__restrict__ segs,
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::RkSeg *__res
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
-    double va[4], vb[4];
+    double va[KPT], vb[KPT];
     int buf = 0;
     if (kbeg < kend) {
         load_tile_regs(sg.D, sg.ld_d, sg.rows, row0, sg.m, kbeg, kend, va);
@@ -147,7 +149,8 @@ __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::RkSeg *__restr
     const ax::RkSeg sg = segs[wk.seg];
     const int64_t m = sg.m;
     const double *src = partial + (int64_t)e * n_split * (BM * BM);
-    for (int t = threadIdx.x; t < BM * BM; t += blockDim.x) {
+    const int per = (BM * BM) / gridDim.y;
+    for (int t = blockIdx.y * per + threadIdx.x; t < (blockIdx.y + 1) * per; t += blockDim.x) {
         const int li = t / BM, lj = t % BM;
         const int64_t i = (int64_t)wk.ti * BM + li;
         const int64_t j = (int64_t)wk.tj * BM + lj;
@@ -187,7 +190,7 @@ ax_status rk_gemm_reduce_launch(const RkSeg *segs_dev, const RkWork *work_dev, i
                                int n_split, const double *partial, double *out_base,
                                hipStream_t stream) {
     if (n_work <= 0) return AX_OK;
-    hipLaunchKernelGGL(rk_reduce_kernel, dim3(n_work), dim3(256), 0, stream, segs_dev, work_dev,
+    hipLaunchKernelGGL(rk_reduce_kernel, dim3(n_work, 16), dim3(256), 0, stream, segs_dev, work_dev,
                        n_work, n_split, partial, out_base);
     AX_CHECK_LAUNCH();
     return AX_OK;

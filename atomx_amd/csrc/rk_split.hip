@@ -59,10 +59,45 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restric
 
 __device__ inline v4i load16(const int8_t *p) { return *reinterpret_cast<const v4i *>(p); }
 
+// LDS tile of one operand for one K-tile: [S][64 rows][ROWB bytes] int8
+// (64 data bytes + 16 pad per row) and the 64 row exponents.
+constexpr int ROWB = KT + 16;
+constexpr int PLANEB = BM * ROWB;
+
+// Global -> register staging: thread t moves S chunks of 16 bytes, chunk
+// (s, row = t/4, q = t%4) of the K-tile.
+__device__ inline void stage_load(const int8_t *__restrict__ base, int64_t plane, int64_t ld,
+                                  const int32_t *__restrict__ rows, int row0, int m, int64_t t,
+                                  v4i (&reg)[S]) {
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const int gr = row0 + r;
+    const v4i zero = {0, 0, 0, 0};
+    if (gr < m) {
+        const int64_t rr = rows ? (int64_t)rows[gr] : (int64_t)gr;
+        const int8_t *p = base + rr * ld + t * KT + q * 16;
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2) reg[s2] = load16(p + s2 * plane);
+    } else {
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2) reg[s2] = zero;
+    }
+}
+
+__device__ inline void stage_store(int8_t *__restrict__ lds, const v4i (&reg)[S]) {
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+#pragma unroll
+    for (int s2 = 0; s2 < S; ++s2)
+        *reinterpret_cast<v4i *>(lds + s2 * PLANEB + r * ROWB + q * 16) = reg[s2];
+}
+
 __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::RkSeg *__restrict__ segs,
                                                             const ax::RkWork *__restrict__ work,
                                                             int n_work, int n_split,
                                                             double *__restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) int8_t As[S * PLANEB];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[S * PLANEB];
+    __shared__ int eAs[BM], eBs[BM];
+
     const int e = blockIdx.x;
     const int split = blockIdx.y;
     if (e >= n_work) return;
@@ -72,31 +107,25 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::RkSeg *__r
     const int64_t per = (n_tiles + n_split - 1) / n_split;
     const int64_t tb = (int64_t)split * per;
     const int64_t te = tb + per < n_tiles ? tb + per : n_tiles;
+    const int row0 = wk.ti * BM, col0 = wk.tj * BM;
 
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     const int fr = lane & 15, fg = lane >> 4;
 
-    // operand rows of this lane (A: output rows, B: output columns)
-    int64_t arow[2], brow[2];
-    bool aok[2], bok[2];
-    int64_t erow[2][4];  // exponent rows of the 4 output rows this lane holds
-    bool eok[2][4];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        const int i = wk.ti * BM + wr + a * 16 + fr;
-        aok[a] = i < sg.m;
-        arow[a] = aok[a] ? (sg.rows ? (int64_t)sg.rows[i] : (int64_t)i) : 0;
-        const int j = wk.tj * BM + wc + a * 16 + fr;
-        bok[a] = j < sg.m;
-        brow[a] = bok[a] ? (int64_t)j : 0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int io = wk.ti * BM + wr + a * 16 + 4 * fg + r;
-            eok[a][r] = io < sg.m;
-            erow[a][r] = eok[a][r] ? (sg.rows ? (int64_t)sg.rows[io] : (int64_t)io) : 0;
-        }
+    // exponent row of output-row / output-column slot `tid` (tid < 64 / < 128)
+    int64_t erow = 0;
+    bool eok = false;
+    if (tid < BM) {
+        const int i = row0 + tid;
+        eok = i < sg.m;
+        erow = eok ? (sg.rows ? (int64_t)sg.rows[i] : (int64_t)i) : 0;
+    } else if (tid < 2 * BM) {
+        const int j = col0 + tid - BM;
+        eok = j < sg.m;
+        erow = eok ? (int64_t)j : 0;
     }
 
     d4 acc[2][2];
@@ -105,48 +134,61 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::RkSeg *__r
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
+    v4i ra[S], rb[S];
+    int rexp = 0;
+    if (tb < te) {
+        stage_load(sg.Ds, sg.ds_plane, sg.ds_ld, sg.rows, row0, sg.m, tb, ra);
+        stage_load(sg.Ys, sg.ys_plane, sg.ys_ld, nullptr, col0, sg.m, tb, rb);
+        if (tid < BM) rexp = eok ? sg.De[erow * sg.de_ld + tb] : 0;
+        else if (tid < 2 * BM) rexp = eok ? sg.Ye[erow * sg.ye_ld + tb] : 0;
+    }
     const v4i zero = {0, 0, 0, 0};
     for (int64_t t = tb; t < te; ++t) {
-        const int64_t xo = t * KT + fg * 16;
-        v4i av[2][S], bv[2][S];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                av[a][s] = aok[a] ? load16(sg.Ds + s * sg.ds_plane + arow[a] * sg.ds_ld + xo) : zero;
-                bv[a][s] = bok[a] ? load16(sg.Ys + s * sg.ys_plane + brow[a] * sg.ys_ld + xo) : zero;
-            }
-        int ea[2][4], eb[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) ea[a][r] = eok[a][r] ? sg.De[erow[a][r] * sg.de_ld + t] : 0;
-            eb[a] = bok[a] ? sg.Ye[brow[a] * sg.ye_ld + t] : 0;
+        __syncthreads();  // previous tile's LDS reads are complete
+        stage_store(As, ra);
+        stage_store(Bs, rb);
+        if (tid < BM) eAs[tid] = rexp;
+        else if (tid < 2 * BM) eBs[tid - BM] = rexp;
+        __syncthreads();
+        if (t + 1 < te) {  // prefetch the next K-tile while this one is multiplied
+            stage_load(sg.Ds, sg.ds_plane, sg.ds_ld, sg.rows, row0, sg.m, t + 1, ra);
+            stage_load(sg.Ys, sg.ys_plane, sg.ys_ld, nullptr, col0, sg.m, t + 1, rb);
+            if (tid < BM) rexp = eok ? sg.De[erow * sg.de_ld + t + 1] : 0;
+            else if (tid < 2 * BM) rexp = eok ? sg.Ye[erow * sg.ye_ld + t + 1] : 0;
         }
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a) {
+            v4i av[S];
+#pragma unroll
+            for (int s2 = 0; s2 < S; ++s2)
+                av[s2] = *reinterpret_cast<const v4i *>(As + s2 * PLANEB + (wr + a * 16 + fr) * ROWB + fg * 16);
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                // levels L = 2..S+1, highest first for Horner
+                v4i bv[S];
+#pragma unroll
+                for (int s2 = 0; s2 < S; ++s2)
+                    bv[s2] = *reinterpret_cast<const v4i *>(Bs + s2 * PLANEB + (wc + b * 16 + fr) * ROWB + fg * 16);
                 double lvl[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int L = S + 1; L >= 2; --L) {
                     v4i c = zero;
 #pragma unroll
-                    for (int s = 1; s <= S; ++s) {
-                        const int s2 = L - s;
+                    for (int s1 = 1; s1 <= S; ++s1) {
+                        const int s2 = L - s1;
                         if (s2 < 1 || s2 > S) continue;
-                        c = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[a][s - 1], bv[b][s2 - 1], c,
-                                                                  0, 0, 0);
+                        c = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[s1 - 1], bv[s2 - 1], c, 0, 0, 0);
                     }
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
                         lvl[r] = (L == S + 1) ? static_cast<double>(c[r])
                                               : fma(lvl[r], 0x1p-7, static_cast<double>(c[r]));
                 }
+                const int eb = eBs[wc + b * 16 + fr];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[a][b][r] += ldexp(lvl[r], ea[a][r] + eb[b] - 14);
+                for (int r = 0; r < 4; ++r)
+                    acc[a][b][r] += ldexp(lvl[r], eAs[wr + a * 16 + 4 * fg + r] + eb - 14);
             }
+        }
     }
 
     double *dst = partial + ((int64_t)e * n_split + split) * (BM * BM);

@@ -205,6 +205,153 @@ __global__ __launch_bounds__(kThreads) void yk_batch_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// LDS-resident Y^k scan (the production variant of the same arithmetic).
+//   1. one coalesced pass loads the row and writes g_out = d r^k and
+//      g_in = d r^-(k+1) to LDS, each cell padded to NG+1 doubles;
+//   2. thread t owns cells t, t+T, t+2T, ... (round-major): per-cell sums and
+//      the exclusive prefix (outward) / suffix (inward) bases over cells via
+//      wave shuffle scans and the per-wave totals; bases go to LDS;
+//   3. lanes map onto (cell, node) with NG fixed per lane, so each lane keeps
+//      its rows of mo/mi in registers; the cell's g values are LDS broadcasts
+//      and Y is stored with contiguous, coalesced wave stores.
+// One global load round trip and three barriers per (row, k) item.
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 512;
+constexpr int kScanWaves = kScanThreads / 64;
+constexpr int kMaxRounds = 4;  // nv <= kScanThreads * kMaxRounds
+
+template <int NG>
+__global__ __launch_bounds__(kScanThreads) void yk_scan_lds_kernel(
+    const double *__restrict__ rho, int64_t ld_rho, const double *__restrict__ w,
+    const double *__restrict__ rk_tab, const double *__restrict__ rkm_tab, int64_t n, int nv,
+    int kmax, const double *__restrict__ mo, const double *__restrict__ mi, int n_items,
+    const int32_t *__restrict__ row, const int32_t *__restrict__ kk,
+    const int32_t *__restrict__ out_row, double *__restrict__ y, int64_t ld_y) {
+    extern __shared__ double sm[];
+    constexpr int P = NG + 1;
+    double *gos = sm;                          // [nv * P] g_out
+    double *gis = sm + (int64_t)nv * P;        // [nv * P] g_in
+    double *bo = sm + (int64_t)2 * nv * P;     // [nv] outward base per cell
+    double *bi = bo + nv;                      // [nv] inward base per cell
+    __shared__ double wt_o[kMaxRounds][kScanWaves], wt_i[kMaxRounds][kScanWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // phase-3 lane role: cell_in_wave, node; rows of the partial matrices
+    constexpr int cpw = 64 / NG;
+    const int ciw = lane / NG, node = lane - (lane / NG) * NG;
+    const bool lane_used = lane < cpw * NG;
+    double mo_r[NG], mi_r[NG];
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+        mo_r[j] = mo[node * NG + j];
+        mi_r[j] = mi[node * NG + j];
+    }
+    const int rounds = (nv + kScanThreads - 1) / kScanThreads;
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int k = kk[it];
+        double *dst = y + (int64_t)out_row[it] * ld_y;
+        if (k < 0 || k > kmax) {
+            for (int64_t x = tid; x < n; x += kScanThreads) dst[x] = __builtin_nan("");
+            continue;
+        }
+        const double *src = rho + (int64_t)row[it] * ld_rho;
+        const double *rkp = rk_tab + (int64_t)k * n;
+        const double *rkmp = rkm_tab + (int64_t)k * n;
+        __syncthreads();  // the previous item is done with the LDS
+        for (int64_t x = tid; x < n; x += kScanThreads) {
+            double d = src[x];
+            if (w) d = w[x] * d;
+            const int64_t px = x + x / NG;
+            gos[px] = d * rkp[x];
+            gis[px] = d * rkmp[x];
+        }
+        __syncthreads();
+        double so[kMaxRounds], si[kMaxRounds], ex_o[kMaxRounds], ex_i[kMaxRounds];
+#pragma unroll
+        for (int r = 0; r < kMaxRounds; ++r) {
+            so[r] = si[r] = ex_o[r] = ex_i[r] = 0.0;
+            if (r < rounds) {
+                const int c = r * kScanThreads + tid;
+                if (c < nv) {
+                    double sa = 0.0, sb = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NG; ++j) {
+                        sa += gos[(int64_t)c * P + j];
+                        sb += gis[(int64_t)c * P + j];
+                    }
+                    so[r] = sa;
+                    si[r] = sb;
+                }
+                double pa = so[r], sb2 = si[r];
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const double va = __shfl_up(pa, off, 64);
+                    const double vb = __shfl_down(sb2, off, 64);
+                    if (lane >= off) pa += va;
+                    if (lane + off < 64) sb2 += vb;
+                }
+                if (lane == 63) wt_o[r][wave] = pa;
+                if (lane == 0) wt_i[r][wave] = sb2;
+                const double ea = __shfl_up(pa, 1, 64);
+                const double eb = __shfl_down(sb2, 1, 64);
+                ex_o[r] = lane == 0 ? 0.0 : ea;
+                ex_i[r] = lane == 63 ? 0.0 : eb;
+            }
+        }
+        __syncthreads();
+        double carry = 0.0;
+#pragma unroll
+        for (int r = 0; r < kMaxRounds; ++r) {
+            if (r < rounds) {
+                double before = carry;
+                for (int v = 0; v < wave; ++v) before += wt_o[r][v];
+                for (int v = 0; v < kScanWaves; ++v) carry += wt_o[r][v];
+                const int c = r * kScanThreads + tid;
+                if (c < nv) bo[c] = ex_o[r] + before;
+            }
+        }
+        carry = 0.0;
+#pragma unroll
+        for (int r = kMaxRounds - 1; r >= 0; --r) {
+            if (r < rounds) {
+                double after = carry;
+                for (int v = kScanWaves - 1; v > wave; --v) after += wt_i[r][v];
+                for (int v = 0; v < kScanWaves; ++v) carry += wt_i[r][v];
+                const int c = r * kScanThreads + tid;
+                if (c < nv) bi[c] = ex_i[r] + after;
+            }
+        }
+        __syncthreads();
+        for (int c0 = 0; c0 < nv; c0 += cpw * kScanWaves) {
+            const int c = c0 + wave * cpw + ciw;
+            if (lane_used && c < nv) {
+                const double *a = gos + (int64_t)c * P;
+                const double *b = gis + (int64_t)c * P;
+                double po = 0.0, pi = 0.0;
+#pragma unroll
+                for (int j = 0; j < NG; ++j) {
+                    po += mo_r[j] * a[j];
+                    pi += mi_r[j] * b[j];
+                }
+                const int64_t x = (int64_t)c * NG + node;
+                dst[x] = rkmp[x] * (bo[c] + po) + rkp[x] * (bi[c] + pi);
+            }
+        }
+    }
+}
+
+template <int NG>
+void launch_scan_lds(size_t lds, int grid, hipStream_t s, const double *rho, int64_t ld_rho,
+                     const double *w, const double *rk, const double *rkm, int64_t n, int nv,
+                     int kmax, const double *mo, const double *mi, int n_items, const int32_t *row,
+                     const int32_t *kk, const int32_t *out_row, double *y, int64_t ld_y) {
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(yk_scan_lds_kernel<NG>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipLaunchKernelGGL(yk_scan_lds_kernel<NG>, dim3(grid), dim3(kScanThreads), lds, s, rho, ld_rho,
+                       w, rk, rkm, n, nv, kmax, mo, mi, n_items, row, kk, out_row, y, ld_y);
+}
+
+// ---------------------------------------------------------------------------
 // R[t] = sum_x D[p_t, x] * Y[q_t, x]  (numerics/bspline/slater.py:129-130)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void rk_dot_kernel(
@@ -460,6 +607,34 @@ ax_status ax_yk_batch(const double *rho, int64_t ld_rho, const double *w, const 
     AX_REQUIRE((size_t)2 * nv * sizeof(double) <= 48 * 1024,
                "ax_yk_batch: nv=%d exceeds the LDS cell-sum capacity", nv);
     if (n_items <= 0) return AX_OK;
+    if (ng == 7 || ng == 8 || ng == 9) {
+        static int lds_cap = -1;
+        if (lds_cap < 0) {
+            int dev = 0, v = 0;
+            if (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+                lds_cap = v;
+            else
+                lds_cap = 65536;
+        }
+        const size_t lds = ((size_t)2 * nv * (ng + 1) + 2 * (size_t)nv) * sizeof(double);
+        const size_t static_lds = 2 * kMaxRounds * kScanWaves * sizeof(double);
+        if (nv <= kScanThreads * kMaxRounds && lds + static_lds <= (size_t)lds_cap) {
+            const int grid = n_items < 8192 ? n_items : 8192;
+            hipStream_t s = ax::as_stream(stream);
+            if (ng == 7)
+                launch_scan_lds<7>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi,
+                                   n_items, row, kk, out_row, y, ld_y);
+            else if (ng == 8)
+                launch_scan_lds<8>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi,
+                                   n_items, row, kk, out_row, y, ld_y);
+            else
+                launch_scan_lds<9>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi,
+                                   n_items, row, kk, out_row, y, ld_y);
+            AX_CHECK_LAUNCH();
+            return AX_OK;
+        }
+    }
     const size_t shmem = (size_t)2 * nv * sizeof(double);
     const int grid = n_items < 4096 ? n_items : 4096;
     hipLaunchKernelGGL(yk_batch_kernel, dim3(grid), dim3(kThreads), shmem, ax::as_stream(stream),

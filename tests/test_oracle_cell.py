@@ -1,0 +1,194 @@
+"""Pins the CPU oracle of the (C) DBSR cell algorithm before it is trusted.
+
+Anchors: the DBSR_HF Ne two-electron energy on fixed spinors (2e-10 Ha, the
+reference's own gate), the L2-projected hydrogenic 1s against closed forms,
+signed monomial cell integrals for all four P/Q blocks, bank symmetry, the
+case1/case2 transpose relation, direct/exchange blocks contracting to the
+scalar R^k, and the semiseparable factorisation equal to the full bank
+(reference tests/test_dbsr_common_spinors.py, test_slater_kernel_crosscheck.py,
+test_convol.py).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from atomx_amd import bspline as B
+from helpers import RK_1S, dbsr_state, physics_workspace, project_1s
+
+
+@pytest.fixture(scope="module")
+def dbsr():
+    return dbsr_state()
+
+
+def test_dbsr_two_electron_energy_semiseparable(dbsr):
+    bd, P, Q, occ, kap, ref = dbsr
+    assert bd["ns"] == 218 and bd["ksp"] == 8 and bd["ksq"] == 9
+    e2, ej, ek = O.two_electron_energy(P, Q, occ, kap, bd)
+    assert e2 == pytest.approx(ref["components"]["E_2e"], abs=2e-10, rel=0)
+
+
+@pytest.fixture(scope="module")
+def h_cell():
+    bd, t = physics_workspace(Z=1, ks=9)
+    coeffs, overlap = project_1s(bd, t)
+    return bd, coeffs
+
+
+def test_projection_is_faithful(h_cell):
+    bd, coeffs = h_cell
+    p = O.eval_orbital_on_grid(coeffs, bd["bsp_p"], bd["nv"], bd["ks"], bd["ns"])
+    assert np.abs(p - 2.0 * bd["gr"] * np.exp(-bd["gr"])).max() < 1e-9
+
+
+@pytest.mark.parametrize("path", ["classic", "semiseparable"])
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_projected_1s_cell_closed_forms(h_cell, path, k):
+    """Cell R^k of the projected 1s (Q = 0) at <= 1e-12 (measured ~2e-15)."""
+    bd, c = h_cell
+    q0 = np.zeros(bd["ns"])
+    banks = O.Banks(bd)
+    v = O.compute_slater_rk_cell(c, q0, c, q0, c, q0, c, q0, bd, k, banks, force=path)
+    assert v == pytest.approx(RK_1S[k], rel=1e-12)
+
+
+@pytest.mark.parametrize("ks,balanced,p_power,q_power,ranks", [
+    (4, False, 1, 1, (0, 1, 6)),
+    (6, False, 4, 5, (6,)),
+    (6, True, 4, 5, (6,)),
+])
+def test_signed_monomial_components(ks, balanced, p_power, q_power, ranks):
+    """-24 R^(a+b+1)/(a+b+1) (1/(a+k+1) + 1/(b+k+1)) for every P/Q block
+    through the full bank and the factored form (reference test_convol.py:435-490)."""
+    radius = 0.7
+    knots = np.r_[np.zeros(ks), np.full(ks, radius)]
+    data = B.prepare_bspline_data(knots, ks, ks, n_gauss=16, kinetic_balance=balanced)
+    orders = data["ksp"], data["ksq"]
+    coefficients = []
+    for order, power in zip(orders, (p_power, q_power)):
+        vector = np.zeros(data["ns"])
+        for i in range(power, order):
+            vector[i] = radius ** power * math.comb(i, power) / math.comb(order - 1, power)
+        coefficients.append(vector)
+    waves = [(s * coefficients[0], s * coefficients[1]) for s in (1.0, -2.0, 3.0, 4.0)]
+    target = [O.density_sym(waves[0][i], waves[1][i], data["ns"], orders[i]) for i in range(2)]
+    source = [O.density_sym(waves[2][i], waves[3][i], data["ns"], orders[i]) for i in range(2)]
+    banks = O.Banks(data)
+    for rank in ranks:
+        for left, right in ((0, 0), (1, 1), (0, 1), (1, 0)):
+            comp = ("p", "q")[left] + ("p", "q")[right]
+            comp += comp
+            bank, extent, _ = banks.get_rkb(rank, comp)
+            full = O.convol_case1(bank, source[right], extent, orders[left], orders[right])
+            factored = O.convol_case1_semiseparable(*banks.get_coulomb_factors(rank, comp),
+                                                    source[right], data["ns"], orders[left],
+                                                    orders[right])
+            a = 2 * (p_power, q_power)[left]
+            b = 2 * (p_power, q_power)[right]
+            expected = -24 * radius ** (a + b + 1) / (a + b + 1) * (1 / (a + rank + 1) + 1 / (b + rank + 1))
+            for pot in (full, factored):
+                v = O.sum_amb_banded(target[left], pot, data["ns"], orders[left])
+                assert v == pytest.approx(expected, rel=2e-13, abs=0.0)
+
+
+@pytest.fixture(scope="module")
+def he_like():
+    """A small exponential-grid workspace with random-ish smooth coefficients."""
+    t = B.make_knots(50, 9, 0.025, 40.0)
+    bd = B.prepare_bspline_data(t, 50, 9)
+    rng = np.random.default_rng(713)
+    ns = bd["ns"]
+    vecs = [tuple(rng.normal(size=ns) for _ in range(2)) for _ in range(3)]
+    return bd, vecs
+
+
+def test_bank_symmetry_and_case1_case2(he_like):
+    bd, _ = he_like
+    banks = O.Banks(bd)
+    ks, ns = bd["ks"], bd["ns"]
+    for k in (0, 2):
+        rkb, _, _ = banks.get_rkb(k, "pppp")
+        sym = rkb.transpose(1, 0, 3, 2)
+        assert np.max(np.abs(rkb - sym)) <= 1e-14 * np.abs(rkb).max()
+        p = np.random.default_rng(1).normal(size=ns)
+        dens = O.density_sym(p, p, ns, ks)
+        r1 = O.convol_case1(rkb, dens, ns, ks, ks)
+        r2 = O.convol_case2(rkb, dens, ns, ks, ks)
+        assert np.allclose(r1, r2, rtol=0, atol=1e-12 * np.abs(r1).max())
+
+
+@pytest.mark.parametrize("rank", [0, 1, 2])
+def test_direct_exchange_blocks_contract_to_slater(he_like, rank):
+    """Direct and exchange blocks reproduce the scalar cell R^k (rel 3e-13)."""
+    bd, (a, b, c) = he_like
+    banks = O.Banks(bd)
+    jpp, jqq = O.compute_direct_potential_cell(*c, bd, rank, banks)
+    kpp, kqq, kpq, kqp = O.compute_exchange_matrix_cell(*c, *c, bd, rank, banks)
+    direct = a[0] @ jpp @ b[0] + a[1] @ jqq @ b[1]
+    exchange = a[0] @ kpp @ b[0] + a[1] @ kqq @ b[1] + a[0] @ kpq @ b[1] + a[1] @ kqp @ b[0]
+    ej = O.compute_slater_rk_cell(*a, *b, *c, *c, bd, rank, banks)
+    ek = O.compute_slater_rk_cell(*a, *c, *c, *b, bd, rank, banks)
+    assert direct == pytest.approx(ej, rel=3e-13, abs=1e-11)
+    assert exchange == pytest.approx(ek, rel=3e-13, abs=1e-11)
+    assert np.array_equal(kpq, kqp.T)
+
+
+@pytest.mark.parametrize("kb", [False, True])
+def test_semiseparable_equals_bank(kb):
+    """Factored direct and exchange contractions equal the full-bank ones."""
+    t = B.make_knots(40, 7, 1e-3, 30.0)
+    bd = B.prepare_bspline_data(t, 40, 7, kinetic_balance=kb)
+    banks = O.Banks(bd)
+    ns = bd["ns"]
+    orders = {"p": bd["ksp"], "q": bd["ksq"]}
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=ns) * np.exp(-np.arange(ns) / 9.0)
+    y = rng.normal(size=ns) * np.exp(-np.arange(ns) / 7.0)
+    for k in (0, 1, 3):
+        for comp in ("pppp", "qqqq", "pqpq", "qpqp"):
+            l, r = comp[0], comp[1]
+            rkb, _, _ = banks.get_rkb(k, comp)
+            f = banks.get_coulomb_factors(k, comp)
+            d = O.density_sym(y, y, ns, orders[r])
+            full = O.convol_case1(rkb, d, ns, orders[l], orders[r])
+            semi = O.convol_case1_semiseparable(*f, d, ns, orders[l], orders[r])
+            assert np.max(np.abs(full - semi)) <= 2e-12 * np.abs(full).max()
+            xx = np.outer(x, y)
+            if l == r:
+                xs = np.outer(x, x)
+                kfull = O.convol_case4(rkb, xs, ns, orders[l], orders[r], symmetric_output=True)
+                ksemi = O.convol_case4_semiseparable(f[0], f[1], f[2], f[1], f[2], xs, ns,
+                                                     orders[l], orders[r], True)
+            else:
+                kfull = O.convol_case4(rkb, xx, ns, orders[l], orders[r])
+                ksemi = O.convol_case4_semiseparable(*f, xx, ns, orders[l], orders[r], False)
+            assert np.max(np.abs(kfull - ksemi)) <= 2e-12 * np.abs(kfull).max()
+
+
+def test_case4_transposed_order_equals_gather(he_like):
+    bd, (a, b, c) = he_like
+    banks = O.Banks(bd)
+    rkb, _, _ = banks.get_rkb(1, "pppp")
+    d = np.outer(a[0], b[0])
+    g = O.convol_case4_gather(rkb, d, bd["ns"], bd["ks"])
+    t = O.convol_case4(rkb, d, bd["ns"], bd["ks"])
+    assert np.array_equal(g, t)
+
+
+def test_angular_coefficients_closed_values():
+    """Lambda for s1/2-s1/2 (k=0): (2j+1)^2 (1/2 0 1/2; -1/2 0 1/2)^2 = 2;
+    p3/2-s1/2 exchange at k=1 and the 3j table value 1/sqrt(6)."""
+    assert O.exchange_lambda(-1, -1, 0) == pytest.approx(2.0, rel=1e-15)
+    # (1/2 1 3/2; -1/2 0 1/2) = -1/sqrt(6) (Racah; standard tables)
+    assert O.wigner_3j_exact(1, 2, 3, -1, 0, 1) == pytest.approx(-1 / math.sqrt(6), rel=1e-14)
+    # sum rule: sum_k (2k+1) (ja k jb; -1/2 0 1/2)^2 over all k = 1 (orthogonality)
+    for ta in (1, 3, 5):
+        for tb in (1, 3, 5):
+            s = sum((2 * k + 1) * O.wigner_3j_exact(ta, 2 * k, tb, -1, 0, 1) ** 2
+                    for k in range(0, (ta + tb) // 2 + 1))
+            assert s == pytest.approx(1.0, rel=1e-14)
+    assert O.exchange_k_range(-1, -2) == [1]
+    assert O.exchange_k_range(-2, -2) == [0, 2]
