@@ -61,6 +61,30 @@ SIGNATURES = {
     "ax_rk_tensor_footprint": (_i64, [_vp]),
     "ax_rk_tensor_run": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "ax_rk_tensor_run_stage": (_int, [_vp, _int, _vp, _vp, _vp, _vp]),
+    "ax_gauss_legendre_01": (_int, [_int, _dp, _dp]),
+    "ax_cell_moments": (_int, [_vp, _vp, _vp, _vp, _int, _int, _int, _int, _int, _int, _vp, _vp]),
+    "ax_cell_triang": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _int, _int, _int, _int, _int, _int,
+                              _vp, _vp, _vp, _vp]),
+    "ax_cell_pair_integrals": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _int, _int, _int, _int, _vp,
+                                      _vp]),
+    "ax_cell_global_moments": (_int, [_vp, _int, _int, _int, _vp, _vp]),
+    "ax_density_sym": (_int, [_vp, _i64, _vp, _vp, _int, _int, _int, _int, _int, _vp, _vp]),
+    "ax_combine_rows": (_int, [_vp, _i64, _int, _vp, _int, _vp, _vp]),
+    "ax_exchange_density": (_int, [_vp, _vp, _i64, _int, _int, _vp, _int, _vp, _vp]),
+    "ax_convol_case1": (_int, [_vp, _int, _int, _int, _vp, _i64, _int, _vp, _i64, _vp]),
+    "ax_convol_case2": (_int, [_vp, _int, _int, _int, _int, _vp, _i64, _int, _vp, _i64, _vp]),
+    "ax_convol_case4": (_int, [_vp, _int, _int, _int, _vp, _i64, _int, _int, _vp, _i64, _vp]),
+    "ax_convol_semi_workspace": (_i64, [_int, _int, _int]),
+    "ax_convol_case1_semi": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _int, _int, _vp, _i64, _int,
+                                    _vp, _vp, _i64, _vp]),
+    "ax_convol_case4_semi": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _int, _int, _vp, _i64, _int,
+                                    _int, _vp, _i64, _vp]),
+    "ax_sum_amb_banded": (_int, [_vp, _i64, _int, _vp, _i64, _int, _int, _int, _int, _vp, _vp]),
+    "ax_banded_to_full_add": (_int, [_vp, _i64, _vp, _i64, _int, _int, _int, ctypes.c_double, _vp]),
+    "ax_axpy": (_int, [_vp, _vp, _i64, ctypes.c_double, _vp]),
+    "ax_mul_rows": (_int, [_vp, _vp, _vp, _int, _i64, _vp]),
+    "ax_bilinear_stage2": (_int, [_vp, _int, _int, _int, _vp, _int, _vp, _vp]),
+    "ax_fock_finalize": (_int, [_vp, _vp, _vp, _vp, _int, ctypes.c_double, _int, _vp]),
 }
 
 _lock = threading.Lock()

@@ -1,0 +1,137 @@
+"""Dirac-Fock two-electron operator on the GPU: the per-target Fock build and
+the average-level two-electron energy of the DBSR cell algorithm.
+
+``FockBuilder.build_all`` is methods/dirac_fock.py:1128-1319
+(_build_hf_matrix_dbsr) for many targets at once: for every (component
+block, rank k) the direct densities of all targets that need k are one
+row-combination launch, their contraction one batched case-1 launch on the
+shared resident bank (semiseparable for ns >= 96), and likewise the exchange
+densities and case-4 contractions; each target's blocks accumulate in the
+reference's (block, k, direct-then-exchange) order and are symmetrised and
+divided by the target's occupation on the device.
+
+``FockBuilder.energy`` is dirac_fock.py:1618-1661 (E_J, E_K, E_2e) with all
+F^k / G^k integrals of one rank in one batched cell contraction.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import device as dv
+from ._lib import call
+from .angular import AngularTables
+from .convol import ConvolCache, _cell_rk_many, _pad_rows, axpy, density_sym_device
+
+_BLOCKS = (((1, 1), "pppp"), ((1, 2), "pqpq"), ((2, 1), "qpqp"), ((2, 2), "qqqq"))
+
+
+class FockBuilder:
+    def __init__(self, bsp_data, kappas, occupations, cache=None, tables=None, **seam):
+        self.bsp_data = bsp_data
+        self.cache = cache if cache is not None else ConvolCache(bsp_data)
+        self.tables = tables if tables is not None else AngularTables(kappas, occupations, **seam)
+        self.ns = self.cache.ns
+        self.n_orb = self.tables.n
+
+    # ------------------------------------------------------------------
+    def _stacks(self, P_stack, Q_stack):
+        ns = self.ns
+        P = dv.f64(_pad_rows(list(np.asarray(P_stack, dtype=np.float64)), ns))
+        Q = dv.f64(_pad_rows(list(np.asarray(Q_stack, dtype=np.float64)), ns))
+        return P, Q
+
+    def build_all_device(self, P_stack, Q_stack, targets=None, skip_pq_exchange=False):
+        """Fock blocks of every target as device tensors: dict t -> (ns,ns)x4
+        (PP, QQ, PQ, QP)."""
+        tab, cache, ns = self.tables, self.cache, self.ns
+        n_orb = self.n_orb
+        targets = list(range(n_orb)) if targets is None else [int(t) for t in targets]
+        P, Q = self._stacks(P_stack, Q_stack)
+        order = {"p": cache.ksp, "q": cache.ksq}
+        dens = {c: density_sym_device(M, [(j, j) for j in range(n_orb)], ns, order[c])
+                for c, M in (("p", P), ("q", Q))}
+        H = {t: [dv.zeros((ns, ns)) for _ in range(4)] for t in targets}
+        ks_all = sorted({k for t in targets for k in tab.k_for_target[t]})
+        for (ii, jj), block in _BLOCKS:
+            c_bra = "q" if ii == 2 else "p"
+            c_ket = "q" if jj == 2 else "p"
+            ks_bra, ks_ket = order[c_bra], order[c_ket]
+            cii = Q if ii == 2 else P
+            cjj = Q if jj == 2 else P
+            direct_slot = 0 if ii == 1 else 1
+            exch_slot = {(1, 1): 0, (1, 2): 2, (2, 1): 3, (2, 2): 1}[(ii, jj)]
+            for k in ks_all:
+                tk = [t for t in targets if k in tab.k_for_target[t]]
+                # --- direct: one combination + one contraction for all targets
+                tdir = [t for t in tk if np.any(tab.direct[t, :, k])]
+                if tdir:
+                    coef = np.zeros((len(tdir), n_orb))
+                    for r, t in enumerate(tdir):
+                        c = tab.direct[t, :, k].copy()
+                        c[t] = c[t] + c[t]
+                        coef[r] = c
+                    cd = dv.f64(coef)
+                    dd = dv.empty((len(tdir), ns, ks_ket))
+                    call("ax_combine_rows", dv.ptr(dens[c_ket]), ns * ks_ket, n_orb, dv.ptr(cd),
+                         len(tdir), dv.ptr(dd), dv.stream_handle())
+                    v = cache.case1(k, block, dd)
+                    for r, t in enumerate(tdir):
+                        call("ax_banded_to_full_add", dv.ptr(H[t][direct_slot]), ns * ns,
+                             dv.ptr(v[r]), ns * ks_bra, 1, ns, ks_bra, 1.0, dv.stream_handle())
+                # --- exchange
+                if skip_pq_exchange and ii == 1 and jj == 2:
+                    continue
+                tex = [t for t in tk if np.any(tab.exchange[t, :, k])]
+                if tex:
+                    coef = np.stack([tab.exchange[t, :, k] for t in tex])
+                    cd = dv.f64(coef)
+                    xx = dv.empty((len(tex), ns, ns))
+                    call("ax_exchange_density", dv.ptr(cii), dv.ptr(cjj), ns, n_orb, ns,
+                         dv.ptr(cd), len(tex), dv.ptr(xx), dv.stream_handle())
+                    kk = cache.case4(k, block, xx, symmetric=(ii == jj))
+                    for r, t in enumerate(tex):
+                        axpy(H[t][exch_slot], kk[r])
+        for t in targets:
+            pp, qq, pq, qp = H[t]
+            call("ax_fock_finalize", dv.ptr(pp), dv.ptr(qq), dv.ptr(pq), dv.ptr(qp), ns,
+                 float(tab.divisor(t)), 1 if skip_pq_exchange else 0, dv.stream_handle())
+        return H
+
+    def build_all(self, P_stack, Q_stack, targets=None, skip_pq_exchange=False):
+        H = self.build_all_device(P_stack, Q_stack, targets, skip_pq_exchange)
+        return {t: tuple(dv.host(x) for x in blocks) for t, blocks in H.items()}
+
+    def build(self, target_i, P_stack, Q_stack, skip_pq_exchange=False):
+        """hfm_PP, hfm_QQ, hfm_PQ, hfm_QP of one target (dirac_fock.py:1128-1319)."""
+        return self.build_all(P_stack, Q_stack, [target_i], skip_pq_exchange)[int(target_i)]
+
+    # ------------------------------------------------------------------
+    def energy(self, P_stack, Q_stack):
+        """(E_2e, E_J, E_K) of the average-level functional (dirac_fock.py:1618-1661)."""
+        P = np.asarray(P_stack, dtype=np.float64)
+        Q = np.asarray(Q_stack, dtype=np.float64)
+        terms = self.tables.energy_terms()
+        by_k = {}
+        for idx, (_, kind, i, j, k, _) in enumerate(terms):
+            by_k.setdefault(k, []).append(idx)
+        vals = np.empty(len(terms))
+        for k, idxs in by_k.items():
+            quartets = []
+            for idx in idxs:
+                _, kind, i, j, _, _ = terms[idx]
+                if kind == "F":
+                    quartets.append((P[i], Q[i], P[i], Q[i], P[j], Q[j], P[j], Q[j]))
+                else:
+                    quartets.append((P[i], Q[i], P[j], Q[j], P[j], Q[j], P[i], Q[i]))
+            v = dv.host(_cell_rk_many(self.cache, k, quartets))
+            if not np.all(np.isfinite(v)):
+                raise FloatingPointError(f"non-finite two-electron integral at k={k}")
+            vals[idxs] = v
+        EJ = EK = 0.0
+        for (w, _, _, _, _, is_direct), v in zip(terms, vals):
+            if is_direct:
+                EJ += w * v
+            else:
+                EK += w * v
+        return EJ - EK, EJ, EK

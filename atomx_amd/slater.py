@@ -36,12 +36,12 @@ def _dot_rows(D, Y, p, q):
 def slater_rk_many(rho13, rho24, tables, ks):
     """R^k for many (rho13[t], rho24[t], ks[t]) on one grid; device tensors in,
     device tensor (n,) out."""
-    from . import device as _dv  # noqa: F401
-    t = dv.torch()
     Y = yk_rows(rho24, tables, ks)
-    D = rho13 * tables.w[None, :]
+    D = dv.empty(tuple(rho13.shape))
+    call("ax_mul_rows", dv.ptr(D), dv.ptr(rho13), dv.ptr(tables.w), rho13.shape[0], rho13.shape[1],
+         dv.stream_handle())
     idx = np.arange(rho13.shape[0], dtype=np.int32)
-    return _dot_rows(D.contiguous(), Y, idx, idx)
+    return _dot_rows(D, Y, idx, idx)
 
 
 def slater_rk_poisson(rho13, rho24, gr, grw, nv, n_gauss, k):

@@ -59,11 +59,13 @@ def compute_slater_rk(Pa, Qa, Pb, Qb, Pc, Qc, Pd, Qd, bsp_data, k):
     gr, grw = bd["gr"], bd["grw"]
     nv, ng = bd["nv"], gr.shape[1]
     fp, fq = _pq_samples([Pa, Pb, Pc, Pd], [Qa, Qb, Qc, Qd], bd)
-    rho_ab = fp[0] * fp[1] + fq[0] * fq[1]
-    rho_cd = fp[2] * fp[3] + fq[2] * fq[3]
+    n = nv * ng
+    rho = dv.empty((2, n))
+    pa, pb = dv.i32([0, 2]), dv.i32([1, 3])
+    call("ax_pair_density", dv.ptr(fp), dv.ptr(fq), n, n, 2, dv.ptr(pa), dv.ptr(pb), None,
+         dv.ptr(rho), n, dv.stream_handle())
     tables = _grid.get(gr, grw, nv, ng, int(k))
-    v = slater_rk_many(rho_ab[None, :].contiguous(), rho_cd[None, :].contiguous(), tables,
-                       [int(k)])
+    v = slater_rk_many(rho[0:1].contiguous(), rho[1:2].contiguous(), tables, [int(k)])
     return float(dv.host(v)[0])
 
 

@@ -162,6 +162,111 @@ enum {
 ax_status ax_rk_tensor_run_stage(ax_rk_tensor *plan, int stage, const double *P,
                                  const double *Q, double *out, void *stream);
 
+/* ---- (C) DBSR cell algorithm ------------------------------------------------
+ * Bank layout: bankT[ip][jp][i][j] = R^k(B_i B_{i+ip}; B_j B_{j+jp}), i.e. the
+ * reference's transposed bank rkbT (convol.py:1572-1591), (ksl, ksr, ns, ns).
+ * Contractions take nb densities that share one bank (leading dims ld_*). */
+
+/* numerics/bspline/convol.py:252-291 _gauss_legendre_01 (host, n nodes on [0,1]) */
+ax_status ax_gauss_legendre_01(int n, double *x_host, double *w_host);
+
+/* convol.py:57-90 _compute_moments_sym (sym=1, packed upper triangle, ks1 == ks2)
+ * and :93-128 _compute_moments_nonsym (sym=0): out (kk, nv). */
+ax_status ax_cell_moments(const double *bsp1, const double *bsp2, const double *gr,
+                          const double *grw, int nv, int ng, int ks1, int ks2, int k, int sym,
+                          double *out, void *stream);
+
+/* convol.py:135-249 _triang_sym (left == right basis) and :387-529
+ * _triang_pqpq: diagonal-cell integrals out (kkl, kkr, nv); n_sub sub-nodes
+ * x01/w01 from ax_gauss_legendre_01 (n_sub = max(ksl,ksr) + k/2 for k >= 0). */
+ax_status ax_cell_triang(const double *bsp_l, const double *bsp_r, const double *gr,
+                         const double *grw, const double *t_l, const double *t_r, int nv, int ng,
+                         int ksl, int ksr, int k, int n_sub, const double *x01, const double *w01,
+                         double *out, void *stream);
+
+/* convol.py:323-380 _assemble_rkb_sym / :532-572 _assemble_rkb_pqpq (mode 0:
+ * the bank bankT) and :1076-1122 _assemble_near_rkb (mode 1: the overlap
+ * strip near (ksl, ksr, ns, ksl+ksr-1)), from r^k moments lo_*, r^-(k+1)
+ * moments hi_* and diagonal cells. */
+ax_status ax_cell_pair_integrals(const double *lo_l, const double *hi_r, const double *lo_r,
+                                 const double *hi_l, const double *diag, int nv, int ksl, int ksr,
+                                 int ns, int mode, double *out, void *stream);
+
+/* convol.py:1061-1073 _global_pair_moments: out (ks, ns). */
+ax_status ax_cell_global_moments(const double *local, int nv, int ks, int ns, double *out,
+                                 void *stream);
+
+/* convol.py:688-753 _density_sym for np coefficient-row pairs (A[pa], A[pb]),
+ * zero-padded beyond n1 / n2: out (np, ns, ks). */
+ax_status ax_density_sym(const double *A, int64_t lda, const int32_t *pa, const int32_t *pb,
+                         int np, int n1, int n2, int ns, int ks, double *out, void *stream);
+
+/* convol.py:646-685 _accumulate_direct_density_batch as a row combination:
+ * out[t] = sum_j coef[t, j] dens[j] (rows of L values). */
+ax_status ax_combine_rows(const double *dens, int64_t L, int n_src, const double *coef, int nt,
+                          double *out, void *stream);
+
+/* convol.py:688-709 _accumulate_exchange_density_batch for nt targets:
+ * out[t][i][k] = sum_j (coef[t,j] A[j][i]) B[j][k]. */
+ax_status ax_exchange_density(const double *A, const double *B, int64_t ld, int n_src, int ns,
+                              const double *coef, int nt, double *out, void *stream);
+
+/* convol.py:788-849 / :971-997 _convol_case1[_mixed|_t]: out (nb, ns, ksl). */
+ax_status ax_convol_case1(const double *bankT, int ns, int ksl, int ksr, const double *d,
+                          int64_t ld_d, int nb, double *out, int64_t ld_o, void *stream);
+
+/* convol.py:852-874 / :1310-1345 _convol_case2[_mixed]: left densities
+ * (nb, n_left, ksl) -> out (nb, ns, ksr). */
+ax_status ax_convol_case2(const double *bankT, int ns, int ksl, int ksr, int n_left,
+                          const double *d, int64_t ld_d, int nb, double *out, int64_t ld_o,
+                          void *stream);
+
+/* convol.py:877-948 / :1000-1058 _convol_case4[_t]: full densities
+ * (nb, ns, ns) -> out (nb, ns, ns); symmetric = symmetric_output. */
+ax_status ax_convol_case4(const double *bankT, int ns, int ksl, int ksr, const double *d,
+                          int64_t ld_d, int nb, int symmetric, double *out, int64_t ld_o,
+                          void *stream);
+
+/* convol.py:1125-1162 _convol_case1_semiseparable_t; workspace holds
+ * ax_convol_semi_workspace(nb, ns, ksr) doubles. */
+int64_t ax_convol_semi_workspace(int nb, int ns, int ksr);
+ax_status ax_convol_case1_semi(const double *near, const double *rl, const double *fl,
+                               const double *rr, const double *fr, int ns, int ksl, int ksr,
+                               const double *d, int64_t ld_d, int nb, double *workspace,
+                               double *out, int64_t ld_o, void *stream);
+
+/* convol.py:1187-1222 _convol_case4_semiseparable_t (symmetric = 1) and
+ * :1250-1303 _convol_case4_semiseparable_mixed_t (symmetric = 0). */
+ax_status ax_convol_case4_semi(const double *near, const double *rl, const double *fl,
+                               const double *rr, const double *fr, int ns, int ksl, int ksr,
+                               const double *d, int64_t ld_d, int nb, int symmetric, double *out,
+                               int64_t ld_o, void *stream);
+
+/* convol.py:1348-1356 _sum_amb_banded for nb pairs (row strides sa, sb). */
+ax_status ax_sum_amb_banded(const double *a, int64_t lda, int sa, const double *b, int64_t ldb,
+                            int sb, int nb, int ns, int ks, double *out, void *stream);
+
+/* convol.py:814-825 _banded_to_full_add for nb matrices. */
+ax_status ax_banded_to_full_add(double *V, int64_t ldv, const double *vb, int64_t ldb, int nb,
+                                int ns, int ks, double coeff, void *stream);
+
+/* y += alpha x (the "hfm_XX += Kk" accumulations of methods/dirac_fock.py:1293-1301) */
+ax_status ax_axpy(double *y, const double *x, int64_t n, double alpha, void *stream);
+
+/* y[r][x] = a[r][x] w[x]: weighted densities (numerics/bspline/slater.py:130) */
+ax_status ax_mul_rows(double *y, const double *a, const double *w, int nrows, int64_t n,
+                      void *stream);
+
+/* convol.py:1933-1942 (_rkb_bilinear stage 2): M (nm, ns) from the case-2
+ * contractions T (nm, ns, ksr) of the symmetrised unit densities and wb. */
+ax_status ax_bilinear_stage2(const double *T, int ns, int ksr, int n_right, const double *wb,
+                             int nm, double *M, void *stream);
+
+/* methods/dirac_fock.py:1303-1317: symmetrise the four Fock blocks and divide
+ * by the target's occupation divisor (skip_pq: PQ = QP^T). */
+ax_status ax_fock_finalize(double *pp, double *qq, double *pq, double *qp, int ns, double norm,
+                           int skip_pq, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
