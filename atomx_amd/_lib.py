@@ -46,6 +46,7 @@ SIGNATURES = {
                            _vp, _vp, _vp, _vp, _i64, _vp]),
     "ax_rk_dot": (_int, [_vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _vp, _vp]),
     "ax_bspline_eval": (_int, [_vp, _i64, _int, _vp, _int, _int, _int, _int, _vp, _i64, _vp]),
+    "ax_assemble_integral": (_int, [_int, _int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp]),
     "ax_exchange_matrix_yk": (_int, [_vp, _vp, _vp, _vp, _int, _int, _int, _int, _int, _vp,
                                      _vp]),
     "ax_slater_rk_direct": (_int, [_vp, _vp, _vp, _vp, _int, _int, _int, _int, _vp, _vp, _vp,
@@ -55,12 +56,20 @@ SIGNATURES = {
     "ax_rk_tensor_create": (_int, [_dp, _dp, _int, _int, _int, _ip, _int, _int,
                                    ctypes.POINTER(_vp)]),
     "ax_rk_tensor_destroy": (_int, [_vp]),
+    "ax_rk_tensor_estimate": (_i64, [_int, _int, _int, _ip, _int, _int]),
     "ax_rk_tensor_size": (_i64, [_vp]),
     "ax_rk_tensor_pairs": (_int, [_vp, _int]),
     "ax_rk_tensor_offset": (_i64, [_vp, _int]),
     "ax_rk_tensor_footprint": (_i64, [_vp]),
     "ax_rk_tensor_run": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "ax_rk_tensor_run_stage": (_int, [_vp, _int, _vp, _vp, _vp, _vp]),
+    "ax_rk_tensor_run_batch": (_int, [ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp),
+                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ax_rk_tensor_set_schedule": (_int, [_vp, _int, _int]),
+    "ax_rk_tensor_get_schedule": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "ax_rk_tensor_tune": (_int, [_vp, _vp, _vp, _vp, _int, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "ax_pool_cached_bytes": (_i64, []),
+    "ax_pool_trim": (_int, []),
     "ax_gauss_legendre_01": (_int, [_int, _dp, _dp]),
     "ax_cell_moments": (_int, [_vp, _vp, _vp, _vp, _int, _int, _int, _int, _int, _int, _vp, _vp]),
     "ax_cell_triang": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _int, _int, _int, _int, _int, _int,

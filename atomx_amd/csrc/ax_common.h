@@ -26,6 +26,15 @@ inline int grid_for(int64_t work, int threads, int cap = 65535) {
 // host in long double by Newton iteration on P_n.
 void gauss_legendre(int n, double *x, double *w);
 
+// Caching device allocator (pool.hip): blocks are rounded up to a size class
+// and returned to a per-device free list on pool_free.
+hipError_t pool_alloc(void **ptr, size_t bytes);
+void pool_free(void *ptr);
+size_t pool_class(size_t bytes);
+// Non-blocking stream of the calling thread on the current device, for
+// set-up copies (never destroyed).
+hipStream_t setup_stream();
+
 }  // namespace ax
 
 #define AX_CHECK_HIP(expr)                                                        \

@@ -1,0 +1,112 @@
+// Caching device allocator for the engine's own workspaces.
+//
+// Plans are created and destroyed per request in a serving loop; hipMalloc /
+// hipFree of their dozen buffers would dominate small requests.  Freed blocks
+// are kept per (device, size class) and reused; on allocation failure the
+// cache is released and the allocation retried once.  Size classes: powers
+// of two up to 2 MiB, multiples of 2 MiB above.
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "ax_common.h"
+
+namespace ax {
+
+namespace {
+
+struct Block {
+    size_t bytes;
+    int device;
+};
+
+std::mutex g_mu;
+std::multimap<std::pair<int, size_t>, void *> g_free;  // (device, class) -> block
+std::unordered_map<void *, Block> g_live;
+size_t g_cached = 0;
+
+size_t size_class(size_t n) {
+    if (n <= 256) return 256;
+    const size_t mb2 = size_t(2) << 20;
+    if (n >= mb2) return (n + mb2 - 1) / mb2 * mb2;
+    size_t c = 256;
+    while (c < n) c <<= 1;
+    return c;
+}
+
+void release_cached_locked() {
+    for (auto &kv : g_free) (void)hipFree(kv.second);
+    g_free.clear();
+    g_cached = 0;
+}
+
+}  // namespace
+
+hipError_t pool_alloc(void **ptr, size_t bytes) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const size_t cls = size_class(bytes == 0 ? 1 : bytes);
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_free.find({dev, cls});
+    if (it != g_free.end()) {
+        *ptr = it->second;
+        g_free.erase(it);
+        g_cached -= cls;
+        g_live[*ptr] = Block{cls, dev};
+        return hipSuccess;
+    }
+    hipError_t e = hipMalloc(ptr, cls);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        release_cached_locked();
+        e = hipMalloc(ptr, cls);
+    }
+    if (e != hipSuccess) return e;
+    g_live[*ptr] = Block{cls, dev};
+    return hipSuccess;
+}
+
+void pool_free(void *ptr) {
+    if (!ptr) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_live.find(ptr);
+    if (it == g_live.end()) {
+        (void)hipFree(ptr);
+        return;
+    }
+    g_free.insert({{it->second.device, it->second.bytes}, ptr});
+    g_cached += it->second.bytes;
+    g_live.erase(it);
+}
+
+size_t pool_class(size_t bytes) { return size_class(bytes == 0 ? 1 : bytes); }
+
+hipStream_t setup_stream() {
+    // one non-blocking stream per (host thread, device) for plan set-up
+    // copies, so that creating a plan never waits on work queued by others
+    static thread_local hipStream_t streams[64] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) return nullptr;
+    if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
+        streams[dev] = nullptr;
+    return streams[dev];
+}
+
+}  // namespace ax
+
+extern "C" {
+
+int64_t ax_pool_cached_bytes(void) {
+    std::lock_guard<std::mutex> lk(ax::g_mu);
+    return static_cast<int64_t>(ax::g_cached);
+}
+
+ax_status ax_pool_trim(void) {
+    AX_CHECK_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> lk(ax::g_mu);
+    ax::release_cached_locked();
+    return AX_OK;
+}
+
+}  // extern "C"

@@ -41,21 +41,164 @@ struct RkWork {
 
 constexpr int kSplitSlices = 7;   // int8 slices per operand (7 bits each)
 constexpr int kSplitKTile = 64;   // K-tile that shares one exponent
+constexpr int kMaxBatch = 16;     // plans per batched launch
+
+// Per-plan arguments of the batched kernels, passed by value in the launch
+// table (kernel arguments), so a work-group reaches its plan's arrays without
+// a chain of dependent loads.
+struct DensArg {             // weighted pair densities (+ their slices)
+    const double *P, *Q;     // orbital samples (n_orb, ld); Q may be null
+    int64_t ld, n;
+    const int32_t *pa, *pb;
+    const double *w;
+    double *D;               // (n_pairs, n)
+    int8_t *Ds;              // split precision: slice planes of D, else null
+    int32_t *De;
+    int64_t npad, ds_plane;
+};
+struct ScanArg {             // Y^k items of one plan
+    const double *D;
+    double *Y;               // (n_yrows, n) FP64 rows, or null when sliced here
+    const double *rk, *rkm;
+    const int32_t *item_row, *item_k, *item_out;
+    int64_t n;
+    int nv, kmax;
+    int8_t *Ys;              // slices written by the scan (n_gauss = 8, split)
+    int32_t *Ye;
+    int64_t npad, ys_plane;
+};
+struct SliceArg {            // rows sliced by a separate pass
+    const double *src;
+    int64_t n, npad, plane;
+    int8_t *dst;
+    int32_t *e;
+};
+struct GemmArg {             // contraction tiles and split reduction
+    const RkSeg *segs;
+    const RkWork *work;
+    double *partial;         // n_work * n_split * 64*64
+    double *out;             // packed R^k output
+    int n_split;
+    int pad;
+};
+
+// Launch table: end[] is the exclusive prefix of the launch's items per plan
+// (pairs, scan items, slice tiles or contraction tiles).
+template <typename T>
+struct BatchOf {
+    int n;
+    int max_split;  // contraction launches: K splits per tile in the grid
+    int order;      // contraction launches: block order (kOrder*)
+    int pad;
+    int64_t end[kMaxBatch];
+    T a[kMaxBatch];
+};
+
+// Block orders of a contraction launch over tiles x K splits (results do not
+// depend on the order, only the L2 locality does):
+enum {
+    kOrderLinear = 0,    // block b -> (split b / tiles, tile b % tiles)
+    kOrderXcd = 1,       // same, after the XCD-contiguous renumbering
+    kOrderXcdSplit = 2,  // XCD-contiguous, K split fastest
+    kOrderCount = 3
+};
+
+// Work-group renumbering for L2 locality: the dispatcher deals consecutive
+// work-groups round-robin to the 8 XCDs, each with its own L2.  Mapping
+// hardware block b of nb to this logical index gives every XCD one
+// contiguous range of logical blocks (a bijection on [0, nb) for any nb, so
+// correctness never depends on the dispatch order).
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+    constexpr int64_t X = 8;
+    const int64_t xcd = b % X, idx = b / X, q = nb / X, r = nb % X;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// (tile, split) of contraction block b of nb under the launch's order.
+template <typename T>
+__device__ inline int64_t gemm_block(const BatchOf<T> &A, int &split) {
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t lb = A.order == kOrderLinear ? b : xcd_block(b, nb);
+    if (A.order == kOrderXcdSplit) {
+        split = static_cast<int>(lb % A.max_split);
+        return lb / A.max_split;
+    }
+    const int64_t tiles = A.end[A.n - 1];
+    split = static_cast<int>(lb / tiles);
+    return lb % tiles;
+}
+
+// Plan of item `it` (uniform across the caller's wave) and the item's index
+// inside that plan.
+template <typename T>
+__device__ inline int batch_find(const BatchOf<T> &B, int64_t it, int64_t &local) {
+    int p = 0;
+    while (p + 1 < B.n && it >= B.end[p]) ++p;
+    local = it - (p ? B.end[p - 1] : 0);
+    return p;
+}
+
+// Split-precision slicing of one value per lane of a wave whose 64 lanes
+// hold one 64-wide K-tile (lane l = element t*64 + l): power-of-two exponent
+// e with max|v| < 2^e over the tile, then kSplitSlices signed 7-bit slices
+// of v*2^-e by truncation, plane s at dst_row[s*plane + x]; lane 0 writes e.
+// Every lane of the wave must call it (it reduces across the wave).
+__device__ inline void slice_value(double v, int64_t x, int8_t *__restrict__ dst_row,
+                                   int64_t plane, int32_t *__restrict__ e_slot) {
+    double mx = fabs(v);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    const int ex = mx > 0.0 ? ilogb(mx) + 1 : 0;
+    double rho = ldexp(v, 7 - ex);  // v * 2^-e * 2^7, |rho| < 128
+    int8_t *d = dst_row + x;
+#pragma unroll
+    for (int s = 0; s < kSplitSlices; ++s) {
+        const double c = trunc(rho);
+        d[(int64_t)s * plane] = static_cast<int8_t>(c);
+        rho = (rho - c) * 128.0;
+    }
+    if ((threadIdx.x & 63) == 0) *e_slot = ex;
+}
+
+// One wave slices K-tile t of a row of n values (zero beyond n).
+__device__ inline void slice_tile(const double *__restrict__ src_row, int64_t n, int64_t t,
+                                  int8_t *__restrict__ dst_row, int64_t dst_plane,
+                                  int32_t *__restrict__ e_slot) {
+    const int64_t x = t * kSplitKTile + (threadIdx.x & 63);
+    slice_value(x < n ? src_row[x] : 0.0, x, dst_row, dst_plane, e_slot);
+}
 
 int gemm_tile();
 int64_t rk_gemm_partial_doubles(int n_work, int n_split);
 void upper_tiles(int m, int seg, std::vector<RkWork> &out);
 
-// Launches the contraction (and the deterministic split reduction).
-ax_status rk_gemm_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work, int n_split,
-                         int precision, double *partial, double *out_base, hipStream_t stream);
-ax_status rk_gemm_contract_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work,
-                                 int n_split, int precision, double *partial, hipStream_t stream);
-ax_status rk_gemm_reduce_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work,
-                               int n_split, const double *partial, double *out_base,
-                               hipStream_t stream);
-ax_status rk_gemm_split_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work,
-                               int n_split, double *partial, hipStream_t stream);
+// Host copy of a plan: sizes and the device pointers the launchers need.
+struct PlanHost {
+    int64_t n, npad;
+    int nv, ng, kmax, n_pairs, n_items, n_yrows, n_work, n_split, precision;
+    int order;              // contraction block order (kOrder*)
+    const double *mo, *mi;  // partial matrices of ng (device)
+    const double *w, *rk, *rkm;
+    const int32_t *pa, *pb, *item_row, *item_k, *item_out;
+    double *D, *Y;          // Y null when the scan writes slices directly
+    int8_t *Ds, *Ys;
+    int32_t *De, *Ye;
+    const RkSeg *segs;
+    const RkWork *work;
+    double *partial;
+};
+// Whether the scan of a split-precision plan on n_gauss nodes per cell writes
+// the int8 slices of Y^k itself (one wave covers exactly one 64-node K-tile).
+bool scan_slices_fused(int ng, int nv);
+
+ax_status density_batch_launch(const PlanHost *const *plans, int n, const double *const *P,
+                               const double *const *Q, hipStream_t stream);
+ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream);
+ax_status slice_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream);
+ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream);
+ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const *out,
+                              hipStream_t stream);
+ax_status rk_gemm_split_launch(const BatchOf<GemmArg> &A, dim3 grid, hipStream_t stream);
 
 // Slices n_rows rows of src (row r at src + rows[r]*ld, or r*ld) into
 // kSplitSlices int8 planes dst[s*plane + r*dst_ld + x] (x < npad, zero

@@ -55,18 +55,20 @@ __device__ inline void store_tile_lds(double *__restrict__ s, const double (&v)[
     for (int q = 0; q < KPT; ++q) s[(kq + q) * LDS_LD + r] = v[q];
 }
 
-__global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::RkSeg *__restrict__ segs,
-                                                          const ax::RkWork *__restrict__ work,
-                                                          int n_work, int n_split,
-                                                          double *__restrict__ partial) {
+__global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::BatchOf<ax::GemmArg> A) {
     __shared__ double As[2][BK * LDS_LD];
     __shared__ double Bs[2][BK * LDS_LD];
 
-    const int e = blockIdx.x;
-    const int split = blockIdx.y;
-    if (e >= n_work) return;
-    const ax::RkWork wk = work[e];
-    const ax::RkSeg sg = segs[wk.seg];
+    // one 1-D grid of tiles x max_split blocks in the launch's block order
+    int split;
+    const int64_t tile = ax::gemm_block(A, split);
+    int64_t e;
+    const int p = ax::batch_find(A, tile, e);
+    const ax::GemmArg &g = A.a[p];
+    const int n_split = g.n_split;
+    if (split >= n_split) return;
+    const ax::RkWork wk = g.work[e];
+    const ax::RkSeg sg = g.segs[wk.seg];
     const int64_t chunk = ((sg.n + n_split - 1) / n_split + BK - 1) / BK * BK;
     const int64_t kbeg = (int64_t)split * chunk;
     const int64_t kend = kbeg + chunk < sg.n ? kbeg + chunk : sg.n;
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::RkSeg *__res
     // C layout of the 16x16x4 f64 MFMA: result register r of lane l holds
     // row (l/16) + 4r, column l%16 (one-row groups, unlike the 4-row groups
     // of the i32/f32 16x16 MFMAs).
-    double *dst = partial + ((int64_t)e * n_split + split) * (BM * BM);
+    double *dst = g.partial + (e * n_split + split) * (BM * BM);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -138,17 +140,16 @@ __global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::RkSeg *__res
 
 // Sums the split partials of each tile in split order and writes the packed
 // upper triangle of the segment.
-__global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::RkSeg *__restrict__ segs,
-                                                        const ax::RkWork *__restrict__ work,
-                                                        int n_work, int n_split,
-                                                        const double *__restrict__ partial,
-                                                        double *__restrict__ out_base) {
-    const int e = blockIdx.x;
-    if (e >= n_work) return;
-    const ax::RkWork wk = work[e];
-    const ax::RkSeg sg = segs[wk.seg];
+__global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::BatchOf<ax::GemmArg> A) {
+    int64_t e;
+    const int p = ax::batch_find(A, blockIdx.x, e);
+    const ax::GemmArg &g = A.a[p];
+    const ax::RkWork wk = g.work[e];
+    const ax::RkSeg sg = g.segs[wk.seg];
+    const int n_split = g.n_split;
     const int64_t m = sg.m;
-    const double *src = partial + (int64_t)e * n_split * (BM * BM);
+    const double *src = g.partial + e * n_split * (BM * BM);
+    double *out = g.out + sg.out_off;
     const int per = (BM * BM) / gridDim.y;
     for (int t = blockIdx.y * per + threadIdx.x; t < (blockIdx.y + 1) * per; t += blockDim.x) {
         const int li = t / BM, lj = t % BM;
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::RkSeg *__restr
         if (i >= m || j >= m || j < i) continue;
         double s = src[t];
         for (int sp = 1; sp < n_split; ++sp) s += src[(int64_t)sp * BM * BM + t];
-        out_base[sg.out_off + i * m - i * (i - 1) / 2 + (j - i)] = s;
+        out[i * m - i * (i - 1) / 2 + (j - i)] = s;
     }
 }
 
@@ -171,37 +172,72 @@ int64_t rk_gemm_partial_doubles(int n_work, int n_split) {
     return (int64_t)n_work * n_split * BM * BM;
 }
 
-ax_status rk_gemm_contract_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work,
-                                 int n_split, int precision, double *partial, hipStream_t stream) {
-    if (n_work <= 0) return AX_OK;
+namespace {
+
+// Fills the per-plan prefix of contraction tiles; returns the total tile
+// count (the largest K split of the batch goes to A.max_split).
+int64_t fill_work(BatchOf<GemmArg> &A, const PlanHost *const *plans, int n,
+                  double *const *out) {
+    A.n = n;
+    A.max_split = 1;
+    A.order = plans[0]->order;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const PlanHost *h = plans[i];
+        total += h->n_work;
+        A.end[i] = total;
+        A.a[i] = GemmArg{h->segs, h->work, h->partial, out ? out[i] : nullptr, h->n_split, 0};
+        if (h->n_split > A.max_split) A.max_split = h->n_split;
+    }
+    return total;
+}
+
+}  // namespace
+
+ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream) {
+    if (n <= 0) return AX_OK;
+    if (n > kMaxBatch) {
+        set_error("contract_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
+        return AX_ERR_VALUE;
+    }
+    const int precision = plans[0]->precision;
+    for (int i = 1; i < n; ++i)
+        if (plans[i]->precision != precision) {
+            set_error("a batched launch needs one precision mode for all plans");
+            return AX_ERR_VALUE;
+        }
+    BatchOf<GemmArg> A{};
+    const int64_t total = fill_work(A, plans, n, nullptr);
+    if (total == 0) return AX_OK;
+    if (total * A.max_split > 0x7fffffff) {
+        set_error("contract_batch_launch: %lld tiles exceed one launch", (long long)total);
+        return AX_ERR_VALUE;
+    }
+    const dim3 grid(static_cast<unsigned>(total * A.max_split));
     if (precision == AX_PREC_FP64) {
-        hipLaunchKernelGGL(rk_gemm_f64_kernel, dim3(n_work, n_split), dim3(256), 0, stream,
-                           segs_dev, work_dev, n_work, n_split, partial);
+        hipLaunchKernelGGL(rk_gemm_f64_kernel, grid, dim3(256), 0, stream, A);
         AX_CHECK_LAUNCH();
         return AX_OK;
     }
-    if (precision == AX_PREC_SPLIT)
-        return rk_gemm_split_launch(segs_dev, work_dev, n_work, n_split, partial, stream);
+    if (precision == AX_PREC_SPLIT) return rk_gemm_split_launch(A, grid, stream);
     set_error("unknown precision mode %d", precision);
     return AX_ERR_VALUE;
 }
 
-ax_status rk_gemm_reduce_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work,
-                               int n_split, const double *partial, double *out_base,
-                               hipStream_t stream) {
-    if (n_work <= 0) return AX_OK;
-    hipLaunchKernelGGL(rk_reduce_kernel, dim3(n_work, 16), dim3(256), 0, stream, segs_dev, work_dev,
-                       n_work, n_split, partial, out_base);
+ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const *out,
+                              hipStream_t stream) {
+    if (n <= 0) return AX_OK;
+    if (n > kMaxBatch) {
+        set_error("reduce_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
+        return AX_ERR_VALUE;
+    }
+    BatchOf<GemmArg> A{};
+    const int64_t total = fill_work(A, plans, n, out);
+    if (total == 0) return AX_OK;
+    hipLaunchKernelGGL(rk_reduce_kernel, dim3(static_cast<unsigned>(total), 16), dim3(256), 0,
+                       stream, A);
     AX_CHECK_LAUNCH();
     return AX_OK;
-}
-
-ax_status rk_gemm_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work, int n_split,
-                         int precision, double *partial, double *out_base, hipStream_t stream) {
-    ax_status st = rk_gemm_contract_launch(segs_dev, work_dev, n_work, n_split, precision, partial,
-                                           stream);
-    if (st != AX_OK) return st;
-    return rk_gemm_reduce_launch(segs_dev, work_dev, n_work, n_split, partial, out_base, stream);
 }
 
 void upper_tiles(int m, int seg, std::vector<RkWork> &out) {

@@ -31,29 +31,32 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restric
                                                          int8_t *__restrict__ dst, int64_t dst_ld,
                                                          int64_t dst_plane,
                                                          int32_t *__restrict__ e, int64_t e_ld) {
-    const int lane = threadIdx.x & 63;
     const int64_t n_tiles = npad / KT;
     const int64_t total = (int64_t)n_rows * n_tiles;
     for (int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); item < total;
          item += (int64_t)gridDim.x * 4) {
         const int r = static_cast<int>(item / n_tiles);
         const int64_t t = item - (int64_t)r * n_tiles;
-        const int64_t x = t * KT + lane;
         const int64_t rr = rows ? (int64_t)rows[r] : (int64_t)r;
-        const double v = x < n ? src[rr * ld + x] : 0.0;
-        double mx = fabs(v);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-        const int ex = mx > 0.0 ? ilogb(mx) + 1 : 0;
-        double rho = ldexp(v, 7 - ex);  // v * 2^-e * 2^7, |rho| < 128
-        int8_t *d = dst + (int64_t)r * dst_ld + x;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const double c = trunc(rho);
-            d[(int64_t)s * dst_plane] = static_cast<int8_t>(c);
-            rho = (rho - c) * 128.0;
-        }
-        if (lane == 0) e[(int64_t)r * e_ld + t] = ex;
+        ax::slice_tile(src + rr * ld, n, t, dst + (int64_t)r * dst_ld, dst_plane,
+                   e + (int64_t)r * e_ld + t);
+    }
+}
+
+// Batched slicing of the rows that no producing kernel sliced (Y^k rows of
+// grids whose scan does not slice in place).  Four waves per block, one
+// 64-wide K-tile per wave; the plan lookup is per wave.
+__global__ __launch_bounds__(256) void slice_batch_kernel(const ax::BatchOf<ax::SliceArg> A) {
+    const int64_t total = A.end[A.n - 1];
+    for (int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); item < total;
+         item += (int64_t)gridDim.x * 4) {
+        int64_t local;
+        const int p = ax::batch_find(A, item, local);
+        const ax::SliceArg &g = A.a[p];
+        const int64_t n_tiles = g.npad / KT;
+        const int64_t r = local / n_tiles;
+        const int64_t t = local - r * n_tiles;
+        ax::slice_tile(g.src + r * g.n, g.n, t, g.dst + r * g.npad, g.plane, g.e + r * n_tiles + t);
     }
 }
 
@@ -90,19 +93,21 @@ __device__ inline void stage_store(int8_t *__restrict__ lds, const v4i (&reg)[S]
         *reinterpret_cast<v4i *>(lds + s2 * PLANEB + r * ROWB + q * 16) = reg[s2];
 }
 
-__global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::RkSeg *__restrict__ segs,
-                                                            const ax::RkWork *__restrict__ work,
-                                                            int n_work, int n_split,
-                                                            double *__restrict__ partial) {
+__global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax::GemmArg> A) {
     __shared__ __attribute__((aligned(16))) int8_t As[S * PLANEB];
     __shared__ __attribute__((aligned(16))) int8_t Bs[S * PLANEB];
     __shared__ int eAs[BM], eBs[BM];
 
-    const int e = blockIdx.x;
-    const int split = blockIdx.y;
-    if (e >= n_work) return;
-    const ax::RkWork wk = work[e];
-    const ax::RkSeg sg = segs[wk.seg];
+    // one 1-D grid of tiles x max_split blocks in the launch's block order
+    int split;
+    const int64_t tile = ax::gemm_block(A, split);
+    int64_t e;
+    const int p = ax::batch_find(A, tile, e);
+    const ax::GemmArg &g = A.a[p];
+    const int n_split = g.n_split;
+    if (split >= n_split) return;
+    const ax::RkWork wk = g.work[e];
+    const ax::RkSeg sg = g.segs[wk.seg];
     const int64_t n_tiles = (sg.n + KT - 1) / KT;
     const int64_t per = (n_tiles + n_split - 1) / n_split;
     const int64_t tb = (int64_t)split * per;
@@ -191,7 +196,7 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::RkSeg *__r
         }
     }
 
-    double *dst = partial + ((int64_t)e * n_split + split) * (BM * BM);
+    double *dst = g.partial + (e * n_split + split) * (BM * BM);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -225,10 +230,32 @@ ax_status slice_rows_launch(const double *src, int64_t ld, const int32_t *rows, 
     return AX_OK;
 }
 
-ax_status rk_gemm_split_launch(const RkSeg *segs_dev, const RkWork *work_dev, int n_work,
-                               int n_split, double *partial, hipStream_t stream) {
-    hipLaunchKernelGGL(rk_gemm_split_kernel, dim3(n_work, n_split), dim3(256), 0, stream, segs_dev,
-                       work_dev, n_work, n_split, partial);
+ax_status rk_gemm_split_launch(const BatchOf<GemmArg> &A, dim3 grid, hipStream_t stream) {
+    hipLaunchKernelGGL(rk_gemm_split_kernel, grid, dim3(256), 0, stream, A);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status slice_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream) {
+    if (n <= 0) return AX_OK;
+    if (n > kMaxBatch) {
+        set_error("slice_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
+        return AX_ERR_VALUE;
+    }
+    BatchOf<SliceArg> A{};
+    A.n = n;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const PlanHost *h = plans[i];
+        const bool todo = h->precision == AX_PREC_SPLIT && h->Y != nullptr;
+        if (todo) total += (int64_t)h->n_yrows * (h->npad / KT);
+        A.end[i] = total;
+        const int64_t yr = h->n_yrows > 0 ? h->n_yrows : 1;
+        A.a[i] = SliceArg{h->Y, h->n, h->npad, yr * h->npad, h->Ys, h->Ye};
+    }
+    if (total == 0) return AX_OK;
+    const int grid = static_cast<int>((total + 3) / 4 < 65535 ? (total + 3) / 4 : 65535);
+    hipLaunchKernelGGL(slice_batch_kernel, dim3(grid), dim3(256), 0, stream, A);
     AX_CHECK_LAUNCH();
     return AX_OK;
 }

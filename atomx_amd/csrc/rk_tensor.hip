@@ -1,12 +1,19 @@
 // Batched R^k tensor over many orbitals on one grid (the all-quartet
-// Slater-integral workload), and the single-block ax_rk_gemm entry point.
+// Slater-integral workload), batches of such plans, and the single-block
+// ax_rk_gemm entry point.
 //
 // Pipeline of one run (all on one stream, capturable in a HIP graph):
 //   1. pair_density   D[p] = w * (P_a P_b + Q_a Q_b)  for all pairs a <= b
-//   2. yk_batch       Y^k[slot(p,k)] for every pair valid at rank k
+//   2. yk scan        Y^k[slot(p,k)] for every pair valid at rank k
 //   3. (split) slice  int8 planes + exponents of D and of every Y^k row
-//   4. rk_gemm        packed upper triangle of D[V_k] Y_k^T for every k,
+//   4. contraction    packed upper triangle of D[V_k] Y_k^T for every k,
 //                     all ranks in one launch (one segment per rank)
+//   5. reduction      split partials summed in a fixed order
+// A batch of plans (different grids, orbital counts, k ranges) runs the same
+// five launches once for all of them; every kernel looks its plan up from a
+// prefix table in its arguments.  A plan's tiles, K splits and summation
+// order do not depend on the batch, so its results are bitwise the same
+// alone and in any batch.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -17,67 +24,327 @@
 struct ax_rk_tensor {
     int nv = 0, ng = 0, n_orb = 0, kmax = 0, precision = 0;
     int64_t n = 0, npad = 0;
-    int n_pairs = 0, n_yrows = 0, n_items = 0, n_work = 0, n_split = 1;
+    int n_pairs = 0, n_yrows = 0, n_items = 0, n_work = 0, n_split = 1, split_cap = 1;
     std::vector<int> m_k;
     std::vector<int64_t> out_off;
     int64_t out_size = 0;
     int64_t footprint = 0;
-    // device buffers
-    double *r = nullptr, *w = nullptr, *rk = nullptr, *rkm = nullptr, *mo = nullptr, *mi = nullptr;
-    int32_t *pa = nullptr, *pb = nullptr, *rows_all = nullptr;
-    int32_t *item_row = nullptr, *item_k = nullptr, *item_out = nullptr;
-    double *D = nullptr, *Y = nullptr, *partial = nullptr;
-    ax::RkSeg *segs = nullptr;
-    ax::RkWork *work = nullptr;
-    int8_t *Ds = nullptr, *Ys = nullptr;
-    int32_t *De = nullptr, *Ye = nullptr;
+    ax::PlanHost host{};
+    hipEvent_t done = nullptr;  // recorded after the plan's last launch
+    bool captured = false;      // launched into a stream capture at least once
     std::vector<void *> allocs;
 };
 
 namespace {
 
-template <typename T>
-ax_status dev_alloc(ax_rk_tensor *p, T **ptr, size_t count) {
-    void *q = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc(&q, count * sizeof(T));
-    if (e != hipSuccess) {
-        ax::set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
-        return e == hipErrorOutOfMemory ? AX_ERR_MEMORY : AX_ERR_HIP;
-    }
-    p->allocs.push_back(q);
-    p->footprint += static_cast<int64_t>(count * sizeof(T));
-    *ptr = static_cast<T *>(q);
-    return AX_OK;
-}
+int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
-template <typename T>
-ax_status upload(T *dst, const T *src, size_t count) {
-    if (count == 0) return AX_OK;
-    AX_CHECK_HIP(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice));
-    return AX_OK;
-}
+// Layout of a plan's single device block: every array at a 256-byte aligned
+// offset.  `add` reserves and returns the offset.
+struct Layout {
+    int64_t size = 0;
+    int64_t add(int64_t bytes) {
+        const int64_t off = size;
+        size = align256(size + (bytes > 0 ? bytes : 1));
+        return off;
+    }
+};
 
 void free_plan(ax_rk_tensor *p) {
-    for (void *q : p->allocs) (void)hipFree(q);
+    for (void *q : p->allocs) ax::pool_free(q);
     p->allocs.clear();
+    if (p->done) (void)hipEventDestroy(p->done);
+    p->done = nullptr;
 }
-
-#define AX_TRY(expr)                   \
-    do {                               \
-        ax_status st_ = (expr);        \
-        if (st_ != AX_OK) {            \
-            free_plan(plan);           \
-            delete plan;               \
-            return st_;                \
-        }                              \
-    } while (0)
 
 int choose_split(int n_work, int64_t n) {
     // enough work-groups for 256 CUs (about four per CU), K chunks >= 256
     int s = 1;
     while ((int64_t)n_work * s < 1024 && n / (2 * s) >= 256 && s < 16) s *= 2;
     return s;
+}
+
+// Records the completion event of every plan after a launch sequence
+// (skipped under stream capture, where destroy synchronises the device).
+ax_status mark_done(ax_rk_tensor *const *plans, int n, hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    AX_CHECK_HIP(hipStreamIsCapturing(s, &cs));
+    for (int i = 0; i < n; ++i) {
+        if (cs != hipStreamCaptureStatusNone) {
+            plans[i]->captured = true;
+            continue;
+        }
+        AX_CHECK_HIP(hipEventRecord(plans[i]->done, s));
+    }
+    return AX_OK;
+}
+
+ax_status run_stage_batch(ax_rk_tensor *const *plans, int n, int stage, const double *const *P,
+                          const double *const *Q, double *const *out, hipStream_t s) {
+    const ax::PlanHost *h[ax::kMaxBatch];
+    for (int i = 0; i < n; ++i) h[i] = &plans[i]->host;
+    switch (stage) {
+        case AX_STAGE_DENSITY:
+            return ax::density_batch_launch(h, n, P, Q, s);
+        case AX_STAGE_SCAN:
+            return ax::scan_batch_launch(h, n, s);
+        case AX_STAGE_SLICE:
+            return ax::slice_batch_launch(h, n, s);
+        case AX_STAGE_CONTRACT:
+            return ax::contract_batch_launch(h, n, s);
+        case AX_STAGE_REDUCE:
+            for (int i = 0; i < n; ++i)
+                AX_REQUIRE(out[i] != nullptr, "R^k tensor run: null output for plan %d", i);
+            return ax::reduce_batch_launch(h, n, out, s);
+        default:
+            ax::set_error("R^k tensor run: unknown stage %d", stage);
+            return AX_ERR_VALUE;
+    }
+}
+
+}  // namespace
+
+namespace {
+
+// Builds a plan, or (estimate != nullptr) only computes the device bytes it
+// would hold, without touching the device.
+ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng, int n_orb,
+                      const int32_t *l_host, int kmax, int precision, ax_rk_tensor **out_plan,
+                      int64_t *estimate) {
+    AX_REQUIRE(out_plan != nullptr || estimate != nullptr, "ax_rk_tensor_create: null plan pointer");
+    AX_REQUIRE(l_host && (estimate || (r_host && w_host)),
+               "ax_rk_tensor_create: null grid or l values");
+    AX_REQUIRE(nv >= 1 && ng >= 1 && ng <= 16, "ax_rk_tensor_create: bad grid nv=%d ng=%d", nv, ng);
+    AX_REQUIRE(n_orb >= 1 && kmax >= 0 && kmax <= 64, "ax_rk_tensor_create: bad n_orb/kmax");
+    AX_REQUIRE(precision == AX_PREC_FP64 || precision == AX_PREC_SPLIT,
+               "ax_rk_tensor_create: unknown precision %d", precision);
+    for (int a = 0; a < n_orb; ++a)
+        AX_REQUIRE(l_host[a] >= 0, "ax_rk_tensor_create: negative l for orbital %d", a);
+    const int64_t n = (int64_t)nv * ng;
+    const int64_t npad = (n + ax::kSplitKTile - 1) / ax::kSplitKTile * ax::kSplitKTile;
+    const int64_t tiles = npad / ax::kSplitKTile;
+
+    // pairs a <= b in lexicographic order
+    std::vector<int32_t> pa, pb;
+    for (int a = 0; a < n_orb; ++a)
+        for (int b = a; b < n_orb; ++b) {
+            pa.push_back(a);
+            pb.push_back(b);
+        }
+    const int n_pairs = static_cast<int>(pa.size());
+
+    // valid pairs per rank, Y^k slots, scan items
+    std::vector<int32_t> rows_all, it_row, it_k, it_out;
+    std::vector<int64_t> row_off(kmax + 1), y_off(kmax + 1), out_off(kmax + 1);
+    std::vector<int> m_k(kmax + 1, 0);
+    int64_t yrows = 0, outsz = 0;
+    for (int k = 0; k <= kmax; ++k) {
+        row_off[k] = static_cast<int64_t>(rows_all.size());
+        y_off[k] = yrows;
+        int m = 0;
+        for (int p = 0; p < n_pairs; ++p) {
+            const int la = l_host[pa[p]], lb = l_host[pb[p]];
+            const int lo = la > lb ? la - lb : lb - la;
+            if (k < lo || k > la + lb || ((la + lb + k) & 1)) continue;
+            rows_all.push_back(p);
+            it_row.push_back(p);
+            it_k.push_back(k);
+            it_out.push_back(static_cast<int32_t>(yrows + m));
+            ++m;
+        }
+        m_k[k] = m;
+        out_off[k] = outsz;
+        outsz += (int64_t)m * (m + 1) / 2;
+        yrows += m;
+    }
+    AX_REQUIRE(yrows < 0x7fffffff, "ax_rk_tensor_create: too many Y^k rows");
+    const int64_t yr = yrows > 0 ? yrows : 1;
+
+    // contraction segments (one per rank) and the tile work list
+    std::vector<ax::RkWork> work;
+    std::vector<int> seg_k;
+    for (int k = 0; k <= kmax; ++k) {
+        if (m_k[k] == 0) continue;
+        ax::upper_tiles(m_k[k], static_cast<int>(seg_k.size()), work);
+        seg_k.push_back(k);
+    }
+    const int n_work = static_cast<int>(work.size());
+    const int n_split = choose_split(n_work, n);
+    const int split_cap = std::min(16, 2 * n_split);  // partials sized for tuning up to here
+
+    // one device block for everything the plan owns
+    Layout L;
+    const int64_t o_r = L.add(n * 8), o_w = L.add(n * 8);
+    const int64_t o_rk = L.add(n * (kmax + 1) * 8), o_rkm = L.add(n * (kmax + 1) * 8);
+    const int64_t o_mo = L.add((int64_t)ng * ng * 8), o_mi = L.add((int64_t)ng * ng * 8);
+    const int64_t o_pa = L.add(n_pairs * 4), o_pb = L.add(n_pairs * 4);
+    const int64_t o_rows = L.add((int64_t)rows_all.size() * 4);
+    const int64_t o_irow = L.add((int64_t)it_row.size() * 4);
+    const int64_t o_ik = L.add((int64_t)it_k.size() * 4);
+    const int64_t o_iout = L.add((int64_t)it_out.size() * 4);
+    const int64_t o_segs = L.add((int64_t)seg_k.size() * sizeof(ax::RkSeg));
+    const int64_t o_work = L.add((int64_t)n_work * sizeof(ax::RkWork));
+    const int64_t head = L.size;  // everything above is uploaded from the host
+    // split precision on 8-node cells: the scan writes Y^k slices only
+    const bool y_fp64 = !(precision == AX_PREC_SPLIT && ax::scan_slices_fused(ng, nv));
+    const int64_t o_D = L.add((int64_t)n_pairs * n * 8);
+    const int64_t o_Y = y_fp64 ? L.add(yr * n * 8) : -1;
+    const int64_t o_part = L.add(ax::rk_gemm_partial_doubles(n_work, split_cap) * 8);
+    int64_t o_Ds = 0, o_Ys = 0, o_De = 0, o_Ye = 0;
+    if (precision == AX_PREC_SPLIT) {
+        o_Ds = L.add((int64_t)ax::kSplitSlices * n_pairs * npad);
+        o_Ys = L.add((int64_t)ax::kSplitSlices * yr * npad);
+        o_De = L.add((int64_t)n_pairs * tiles * 4);
+        o_Ye = L.add(yr * tiles * 4);
+    }
+    if (estimate) {
+        *estimate = static_cast<int64_t>(ax::pool_class(static_cast<size_t>(L.size)));
+        return AX_OK;
+    }
+
+    ax_rk_tensor *plan = new ax_rk_tensor();
+    plan->nv = nv;
+    plan->ng = ng;
+    plan->n_orb = n_orb;
+    plan->kmax = kmax;
+    plan->precision = precision;
+    plan->n = n;
+    plan->npad = npad;
+    plan->n_pairs = n_pairs;
+    plan->n_yrows = static_cast<int>(yrows);
+    plan->n_items = static_cast<int>(it_row.size());
+    plan->n_work = n_work;
+    plan->n_split = n_split;
+    plan->split_cap = split_cap;
+    plan->m_k = m_k;
+    plan->out_off = out_off;
+    plan->out_size = outsz;
+
+    void *blk = nullptr;
+    hipError_t he = ax::pool_alloc(&blk, static_cast<size_t>(L.size));
+    if (he != hipSuccess) {
+        delete plan;
+        ax::set_error("device allocation of %lld bytes failed: %s", (long long)L.size,
+                      hipGetErrorString(he));
+        return he == hipErrorOutOfMemory ? AX_ERR_MEMORY : AX_ERR_HIP;
+    }
+    plan->allocs.push_back(blk);
+    plan->footprint = static_cast<int64_t>(ax::pool_class(static_cast<size_t>(L.size)));
+    char *base = static_cast<char *>(blk);
+    auto dptr = [&](int64_t off) { return static_cast<void *>(base + off); };
+
+    // host image of the uploaded head of the block
+    std::vector<char> img(static_cast<size_t>(head), 0);
+    auto put = [&](int64_t off, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(img.data() + off, src, bytes);
+    };
+    put(o_r, r_host, n * 8);
+    put(o_w, w_host, n * 8);
+    {
+        std::vector<double> mo(ng * ng), mi(ng * ng);
+        ax_status st = ax_yk_partial_matrices(ng, mo.data(), mi.data());
+        if (st != AX_OK) {
+            free_plan(plan);
+            delete plan;
+            return st;
+        }
+        put(o_mo, mo.data(), mo.size() * 8);
+        put(o_mi, mi.data(), mi.size() * 8);
+    }
+    put(o_pa, pa.data(), pa.size() * 4);
+    put(o_pb, pb.data(), pb.size() * 4);
+    put(o_rows, rows_all.data(), rows_all.size() * 4);
+    put(o_irow, it_row.data(), it_row.size() * 4);
+    put(o_ik, it_k.data(), it_k.size() * 4);
+    put(o_iout, it_out.data(), it_out.size() * 4);
+
+    double *D = static_cast<double *>(dptr(o_D));
+    double *Y = y_fp64 ? static_cast<double *>(dptr(o_Y)) : nullptr;
+    int8_t *Ds = precision == AX_PREC_SPLIT ? static_cast<int8_t *>(dptr(o_Ds)) : nullptr;
+    int8_t *Ys = precision == AX_PREC_SPLIT ? static_cast<int8_t *>(dptr(o_Ys)) : nullptr;
+    int32_t *De = precision == AX_PREC_SPLIT ? static_cast<int32_t *>(dptr(o_De)) : nullptr;
+    int32_t *Ye = precision == AX_PREC_SPLIT ? static_cast<int32_t *>(dptr(o_Ye)) : nullptr;
+    const int32_t *rows_dev = static_cast<const int32_t *>(dptr(o_rows));
+    {
+        std::vector<ax::RkSeg> segs;
+        for (int k : seg_k) {
+            ax::RkSeg sg{};
+            sg.D = D;
+            sg.ld_d = n;
+            sg.rows = rows_dev + row_off[k];
+            sg.Y = Y ? Y + y_off[k] * n : nullptr;
+            sg.ld_y = n;
+            sg.n = n;
+            sg.out_off = out_off[k];
+            sg.m = m_k[k];
+            if (precision == AX_PREC_SPLIT) {
+                sg.Ds = Ds;
+                sg.ds_ld = npad;
+                sg.ds_plane = (int64_t)n_pairs * npad;
+                sg.De = De;
+                sg.de_ld = tiles;
+                sg.Ys = Ys + y_off[k] * npad;
+                sg.ys_ld = npad;
+                sg.ys_plane = yr * npad;
+                sg.Ye = Ye + y_off[k] * tiles;
+                sg.ye_ld = tiles;
+            }
+            segs.push_back(sg);
+        }
+        put(o_segs, segs.data(), segs.size() * sizeof(ax::RkSeg));
+        put(o_work, work.data(), work.size() * sizeof(ax::RkWork));
+    }
+    ax::PlanHost &h = plan->host;
+    h.n = n;
+    h.npad = npad;
+    h.nv = nv;
+    h.ng = ng;
+    h.kmax = kmax;
+    h.n_pairs = n_pairs;
+    h.n_items = plan->n_items;
+    h.n_yrows = plan->n_yrows;
+    h.n_work = n_work;
+    h.n_split = n_split;
+    h.precision = precision;
+    h.order = ax::kOrderLinear;
+    h.mo = static_cast<const double *>(dptr(o_mo));
+    h.mi = static_cast<const double *>(dptr(o_mi));
+    h.w = static_cast<const double *>(dptr(o_w));
+    h.rk = static_cast<const double *>(dptr(o_rk));
+    h.rkm = static_cast<const double *>(dptr(o_rkm));
+    h.pa = static_cast<const int32_t *>(dptr(o_pa));
+    h.pb = static_cast<const int32_t *>(dptr(o_pb));
+    h.item_row = static_cast<const int32_t *>(dptr(o_irow));
+    h.item_k = static_cast<const int32_t *>(dptr(o_ik));
+    h.item_out = static_cast<const int32_t *>(dptr(o_iout));
+    h.D = D;
+    h.Y = Y;
+    h.Ds = Ds;
+    h.Ys = Ys;
+    h.De = De;
+    h.Ye = Ye;
+    h.segs = static_cast<const ax::RkSeg *>(dptr(o_segs));
+    h.work = static_cast<const ax::RkWork *>(dptr(o_work));
+    h.partial = static_cast<double *>(dptr(o_part));
+
+    // one upload, r-power tables on the device, wait for this stream only
+    hipStream_t ss = ax::setup_stream();
+    ax_status st = AX_OK;
+    hipError_t e = hipMemcpyAsync(blk, img.data(), img.size(), hipMemcpyHostToDevice, ss);
+    if (e == hipSuccess)
+        st = ax_grid_rpow(static_cast<const double *>(dptr(o_r)), n, kmax,
+                          static_cast<double *>(dptr(o_rk)), static_cast<double *>(dptr(o_rkm)),
+                          ss);
+    if (e == hipSuccess && st == AX_OK) e = hipStreamSynchronize(ss);
+    if (e == hipSuccess && st == AX_OK) e = hipEventCreateWithFlags(&plan->done, hipEventDisableTiming);
+    if (e != hipSuccess || st != AX_OK) {
+        if (e != hipSuccess) ax::set_error("R^k tensor set-up failed: %s", hipGetErrorString(e));
+        free_plan(plan);
+        delete plan;
+        return st != AX_OK ? st : AX_ERR_HIP;
+    }
+    *out_plan = plan;
+    return AX_OK;
 }
 
 }  // namespace
@@ -88,146 +355,25 @@ ax_status ax_rk_tensor_create(const double *r_host, const double *w_host, int nv
                               int n_orb, const int32_t *l_host, int kmax, int precision,
                               ax_rk_tensor **out_plan) {
     AX_REQUIRE(out_plan != nullptr, "ax_rk_tensor_create: null plan pointer");
-    AX_REQUIRE(nv >= 1 && ng >= 1 && ng <= 16, "ax_rk_tensor_create: bad grid nv=%d ng=%d", nv, ng);
-    AX_REQUIRE(n_orb >= 1 && kmax >= 0 && kmax <= 64, "ax_rk_tensor_create: bad n_orb/kmax");
-    AX_REQUIRE(precision == AX_PREC_FP64 || precision == AX_PREC_SPLIT,
-               "ax_rk_tensor_create: unknown precision %d", precision);
-    for (int a = 0; a < n_orb; ++a)
-        AX_REQUIRE(l_host[a] >= 0, "ax_rk_tensor_create: negative l for orbital %d", a);
-    ax_rk_tensor *plan = new ax_rk_tensor();
-    plan->nv = nv;
-    plan->ng = ng;
-    plan->n_orb = n_orb;
-    plan->kmax = kmax;
-    plan->precision = precision;
-    plan->n = (int64_t)nv * ng;
-    plan->npad = (plan->n + ax::kSplitKTile - 1) / ax::kSplitKTile * ax::kSplitKTile;
-    const int64_t n = plan->n;
+    return plan_create(r_host, w_host, nv, ng, n_orb, l_host, kmax, precision, out_plan, nullptr);
+}
 
-    // pairs a <= b in lexicographic order
-    std::vector<int32_t> pa, pb;
-    for (int a = 0; a < n_orb; ++a)
-        for (int b = a; b < n_orb; ++b) {
-            pa.push_back(a);
-            pb.push_back(b);
-        }
-    plan->n_pairs = static_cast<int>(pa.size());
-
-    // valid pairs per rank, Y^k slots, scan items
-    std::vector<int32_t> rows_all, it_row, it_k, it_out;
-    std::vector<int64_t> row_off(kmax + 1), y_off(kmax + 1);
-    plan->m_k.assign(kmax + 1, 0);
-    plan->out_off.assign(kmax + 1, 0);
-    int64_t yrows = 0, outsz = 0;
-    for (int k = 0; k <= kmax; ++k) {
-        row_off[k] = static_cast<int64_t>(rows_all.size());
-        y_off[k] = yrows;
-        int m = 0;
-        for (int p = 0; p < plan->n_pairs; ++p) {
-            const int la = l_host[pa[p]], lb = l_host[pb[p]];
-            const int lo = la > lb ? la - lb : lb - la;
-            if (k < lo || k > la + lb || ((la + lb + k) & 1)) continue;
-            rows_all.push_back(p);
-            it_row.push_back(p);
-            it_k.push_back(k);
-            it_out.push_back(static_cast<int32_t>(yrows + m));
-            ++m;
-        }
-        plan->m_k[k] = m;
-        plan->out_off[k] = outsz;
-        outsz += (int64_t)m * (m + 1) / 2;
-        yrows += m;
-    }
-    plan->out_size = outsz;
-    plan->n_yrows = static_cast<int>(yrows);
-    plan->n_items = static_cast<int>(it_row.size());
-
-    // grid tables
-    AX_TRY(dev_alloc(plan, &plan->r, n));
-    AX_TRY(dev_alloc(plan, &plan->w, n));
-    AX_TRY(dev_alloc(plan, &plan->rk, n * (kmax + 1)));
-    AX_TRY(dev_alloc(plan, &plan->rkm, n * (kmax + 1)));
-    AX_TRY(dev_alloc(plan, &plan->mo, ng * ng));
-    AX_TRY(dev_alloc(plan, &plan->mi, ng * ng));
-    AX_TRY(upload(plan->r, r_host, n));
-    AX_TRY(upload(plan->w, w_host, n));
-    {
-        std::vector<double> mo(ng * ng), mi(ng * ng);
-        AX_TRY(ax_yk_partial_matrices(ng, mo.data(), mi.data()));
-        AX_TRY(upload(plan->mo, mo.data(), mo.size()));
-        AX_TRY(upload(plan->mi, mi.data(), mi.size()));
-    }
-    AX_TRY(ax_grid_rpow(plan->r, n, kmax, plan->rk, plan->rkm, nullptr));
-
-    AX_TRY(dev_alloc(plan, &plan->pa, pa.size()));
-    AX_TRY(dev_alloc(plan, &plan->pb, pb.size()));
-    AX_TRY(upload(plan->pa, pa.data(), pa.size()));
-    AX_TRY(upload(plan->pb, pb.data(), pb.size()));
-    AX_TRY(dev_alloc(plan, &plan->rows_all, rows_all.size()));
-    AX_TRY(upload(plan->rows_all, rows_all.data(), rows_all.size()));
-    AX_TRY(dev_alloc(plan, &plan->item_row, it_row.size()));
-    AX_TRY(dev_alloc(plan, &plan->item_k, it_k.size()));
-    AX_TRY(dev_alloc(plan, &plan->item_out, it_out.size()));
-    AX_TRY(upload(plan->item_row, it_row.data(), it_row.size()));
-    AX_TRY(upload(plan->item_k, it_k.data(), it_k.size()));
-    AX_TRY(upload(plan->item_out, it_out.data(), it_out.size()));
-    AX_TRY(dev_alloc(plan, &plan->D, (size_t)plan->n_pairs * n));
-    AX_TRY(dev_alloc(plan, &plan->Y, (size_t)(yrows > 0 ? yrows : 1) * n));
-
-    if (precision == AX_PREC_SPLIT) {
-        const int64_t tiles = plan->npad / ax::kSplitKTile;
-        AX_TRY(dev_alloc(plan, &plan->Ds, (size_t)ax::kSplitSlices * plan->n_pairs * plan->npad));
-        AX_TRY(dev_alloc(plan, &plan->Ys, (size_t)ax::kSplitSlices * (yrows > 0 ? yrows : 1) * plan->npad));
-        AX_TRY(dev_alloc(plan, &plan->De, (size_t)plan->n_pairs * tiles));
-        AX_TRY(dev_alloc(plan, &plan->Ye, (size_t)(yrows > 0 ? yrows : 1) * tiles));
-    }
-
-    // contraction segments (one per rank) and the tile work list
-    std::vector<ax::RkSeg> segs;
-    std::vector<ax::RkWork> work;
-    const int64_t tiles = plan->npad / ax::kSplitKTile;
-    for (int k = 0; k <= kmax; ++k) {
-        if (plan->m_k[k] == 0) continue;
-        ax::RkSeg s{};
-        s.D = plan->D;
-        s.ld_d = n;
-        s.rows = plan->rows_all + row_off[k];
-        s.Y = plan->Y + y_off[k] * n;
-        s.ld_y = n;
-        s.n = n;
-        s.out_off = plan->out_off[k];
-        s.m = plan->m_k[k];
-        if (precision == AX_PREC_SPLIT) {
-            s.Ds = plan->Ds;
-            s.ds_ld = plan->npad;
-            s.ds_plane = (int64_t)plan->n_pairs * plan->npad;
-            s.De = plan->De;
-            s.de_ld = tiles;
-            s.Ys = plan->Ys + y_off[k] * plan->npad;
-            s.ys_ld = plan->npad;
-            s.ys_plane = (int64_t)(yrows > 0 ? yrows : 1) * plan->npad;
-            s.Ye = plan->Ye + y_off[k] * tiles;
-            s.ye_ld = tiles;
-        }
-        ax::upper_tiles(s.m, static_cast<int>(segs.size()), work);
-        segs.push_back(s);
-    }
-    plan->n_work = static_cast<int>(work.size());
-    plan->n_split = choose_split(plan->n_work, n);
-    AX_TRY(dev_alloc(plan, &plan->segs, segs.size()));
-    AX_TRY(dev_alloc(plan, &plan->work, work.size()));
-    AX_TRY(upload(plan->segs, segs.data(), segs.size()));
-    AX_TRY(upload(plan->work, work.data(), work.size()));
-    AX_TRY(dev_alloc(plan, &plan->partial,
-                     (size_t)ax::rk_gemm_partial_doubles(plan->n_work, plan->n_split)));
-    AX_CHECK_HIP(hipDeviceSynchronize());
-    *out_plan = plan;
-    return AX_OK;
+int64_t ax_rk_tensor_estimate(int nv, int ng, int n_orb, const int32_t *l_host, int kmax,
+                              int precision) {
+    int64_t bytes = -1;
+    if (plan_create(nullptr, nullptr, nv, ng, n_orb, l_host, kmax, precision, nullptr, &bytes) !=
+        AX_OK)
+        return -1;
+    return bytes;
 }
 
 ax_status ax_rk_tensor_destroy(ax_rk_tensor *plan) {
     if (!plan) return AX_OK;
-    (void)hipDeviceSynchronize();
+    // the blocks return to the cache: wait until no launch can still use them
+    if (plan->captured)
+        (void)hipDeviceSynchronize();
+    else if (plan->done)
+        (void)hipEventSynchronize(plan->done);
     free_plan(plan);
     delete plan;
     return AX_OK;
@@ -250,71 +396,160 @@ int64_t ax_rk_tensor_footprint(const ax_rk_tensor *plan) { return plan ? plan->f
 ax_status ax_rk_tensor_run_stage(ax_rk_tensor *plan, int stage, const double *P,
                                  const double *Q, double *out, void *stream) {
     AX_REQUIRE(plan != nullptr, "ax_rk_tensor_run_stage: null plan");
+    AX_REQUIRE(stage != AX_STAGE_DENSITY || P != nullptr, "ax_rk_tensor_run_stage: null orbitals");
     hipStream_t s = ax::as_stream(stream);
-    const int64_t n = plan->n;
-    switch (stage) {
-        case AX_STAGE_DENSITY:
-            AX_REQUIRE(P != nullptr, "ax_rk_tensor_run_stage: null orbitals");
-            return ax_pair_density(P, Q, n, n, plan->n_pairs, plan->pa, plan->pb, plan->w, plan->D,
-                                   n, stream);
-        case AX_STAGE_SCAN:
-            return ax_yk_batch(plan->D, n, nullptr, plan->rk, plan->rkm, n, plan->nv, plan->ng,
-                               plan->kmax, plan->mo, plan->mi, plan->n_items, plan->item_row,
-                               plan->item_k, plan->item_out, plan->Y, n, stream);
-        case AX_STAGE_SLICE: {
-            if (plan->precision != AX_PREC_SPLIT) return AX_OK;
-            const int64_t tiles = plan->npad / ax::kSplitKTile;
-            ax_status st = ax::slice_rows_launch(plan->D, n, nullptr, plan->n_pairs, n, plan->npad,
-                                                 plan->Ds, plan->npad,
-                                                 (int64_t)plan->n_pairs * plan->npad, plan->De,
-                                                 tiles, s);
-            if (st != AX_OK) return st;
-            return ax::slice_rows_launch(
-                plan->Y, n, nullptr, plan->n_yrows, n, plan->npad, plan->Ys, plan->npad,
-                (int64_t)(plan->n_yrows > 0 ? plan->n_yrows : 1) * plan->npad, plan->Ye, tiles, s);
-        }
-        case AX_STAGE_CONTRACT:
-            return ax::rk_gemm_contract_launch(plan->segs, plan->work, plan->n_work, plan->n_split,
-                                               plan->precision, plan->partial, s);
-        case AX_STAGE_REDUCE:
-            AX_REQUIRE(out != nullptr, "ax_rk_tensor_run_stage: null output");
-            return ax::rk_gemm_reduce_launch(plan->segs, plan->work, plan->n_work, plan->n_split,
-                                             plan->partial, out, s);
-        default:
-            ax::set_error("ax_rk_tensor_run_stage: unknown stage %d", stage);
-            return AX_ERR_VALUE;
-    }
+    ax_status st = run_stage_batch(&plan, 1, stage, &P, &Q, &out, s);
+    if (st != AX_OK) return st;
+    return mark_done(&plan, 1, s);
 }
 
 ax_status ax_rk_tensor_run(ax_rk_tensor *plan, const double *P, const double *Q, double *out,
                            void *stream) {
     AX_REQUIRE(plan && P && out, "ax_rk_tensor_run: null argument");
-    for (int stage = AX_STAGE_DENSITY; stage <= AX_STAGE_REDUCE; ++stage) {
-        ax_status st = ax_rk_tensor_run_stage(plan, stage, P, Q, out, stream);
+    return ax_rk_tensor_run_batch(&plan, 1, &P, &Q, &out, stream);
+}
+
+ax_status ax_rk_tensor_run_batch(ax_rk_tensor *const *plans, int n_plans, const double *const *P,
+                                 const double *const *Q, double *const *out, void *stream) {
+    AX_REQUIRE(n_plans >= 0, "ax_rk_tensor_run_batch: negative plan count");
+    if (n_plans == 0) return AX_OK;
+    AX_REQUIRE(plans && P && out, "ax_rk_tensor_run_batch: null argument");
+    for (int i = 0; i < n_plans; ++i) {
+        AX_REQUIRE(plans[i] && P[i] && out[i], "ax_rk_tensor_run_batch: null entry %d", i);
+        AX_REQUIRE(plans[i]->precision == plans[0]->precision,
+                   "ax_rk_tensor_run_batch: plans of one batch need one precision mode");
+    }
+    hipStream_t s = ax::as_stream(stream);
+    for (int b0 = 0; b0 < n_plans; b0 += ax::kMaxBatch) {
+        const int nb = std::min(ax::kMaxBatch, n_plans - b0);
+        const double *q[ax::kMaxBatch];
+        for (int i = 0; i < nb; ++i) q[i] = Q ? Q[b0 + i] : nullptr;
+        for (int stage = AX_STAGE_DENSITY; stage <= AX_STAGE_REDUCE; ++stage) {
+            ax_status st = run_stage_batch(plans + b0, nb, stage, P + b0, q, out + b0, s);
+            if (st != AX_OK) return st;
+        }
+        ax_status st = mark_done(plans + b0, nb, s);
         if (st != AX_OK) return st;
     }
     return AX_OK;
 }
 
 // ---------------------------------------------------------------------------
+// Contraction schedule: block order and K split, set directly or chosen by
+// timing every candidate on the plan's own data.  The split fixes the
+// summation order of the reduction, so a plan's results are deterministic
+// for a given schedule (and the same alone and in a batch).
+// ---------------------------------------------------------------------------
+ax_status ax_rk_tensor_set_schedule(ax_rk_tensor *plan, int order, int n_split) {
+    AX_REQUIRE(plan != nullptr, "ax_rk_tensor_set_schedule: null plan");
+    AX_REQUIRE(order >= 0 && order < ax::kOrderCount, "ax_rk_tensor_set_schedule: bad order %d",
+               order);
+    AX_REQUIRE(n_split >= 1 && n_split <= plan->split_cap,
+               "ax_rk_tensor_set_schedule: split %d outside [1, %d]", n_split, plan->split_cap);
+    if (plan->done) AX_CHECK_HIP(hipEventSynchronize(plan->done));
+    plan->host.order = order;
+    plan->host.n_split = n_split;
+    plan->n_split = n_split;
+    return AX_OK;
+}
+
+ax_status ax_rk_tensor_get_schedule(const ax_rk_tensor *plan, int *order_host,
+                                    int *n_split_host) {
+    AX_REQUIRE(plan && order_host && n_split_host, "ax_rk_tensor_get_schedule: null argument");
+    *order_host = plan->host.order;
+    *n_split_host = plan->host.n_split;
+    return AX_OK;
+}
+
+ax_status ax_rk_tensor_tune(ax_rk_tensor *plan, const double *P, const double *Q, double *out,
+                            int reps, void *stream, float *best_ms_host) {
+    AX_REQUIRE(plan && P && out, "ax_rk_tensor_tune: null argument");
+    AX_REQUIRE(reps >= 1 && reps <= 100, "ax_rk_tensor_tune: reps must be in [1, 100]");
+    hipStream_t s = ax::as_stream(stream);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    AX_CHECK_HIP(hipStreamIsCapturing(s, &cs));
+    AX_REQUIRE(cs == hipStreamCaptureStatusNone, "ax_rk_tensor_tune: stream is capturing");
+    ax_status st = ax_rk_tensor_run(plan, P, Q, out, stream);
+    if (st != AX_OK) return st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    AX_CHECK_HIP(hipEventCreate(&e0));
+    AX_CHECK_HIP(hipEventCreate(&e1));
+    const int save_order = plan->host.order, save_split = plan->host.n_split;
+    float best = -1.0f;
+    int best_order = save_order, best_split = save_split;
+    for (int split = 1; split <= plan->split_cap && st == AX_OK; split *= 2) {
+        for (int order = 0; order < ax::kOrderCount && st == AX_OK; ++order) {
+            plan->host.order = order;
+            plan->host.n_split = split;
+            for (int r = 0; r <= reps && st == AX_OK; ++r) {  // r = 0 warms up
+                if (r == 1 && hipEventRecord(e0, s) != hipSuccess) st = AX_ERR_HIP;
+                if (st == AX_OK)
+                    st = run_stage_batch(&plan, 1, AX_STAGE_CONTRACT, &P, &Q, &out, s);
+                if (st == AX_OK) st = run_stage_batch(&plan, 1, AX_STAGE_REDUCE, &P, &Q, &out, s);
+            }
+            float ms = 0.0f;
+            if (st == AX_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                                hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+                ax::set_error("ax_rk_tensor_tune: event timing failed");
+                st = AX_ERR_HIP;
+            }
+            if (st == AX_OK && (best < 0.0f || ms < best)) {
+                best = ms;
+                best_order = order;
+                best_split = split;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    plan->host.order = st == AX_OK ? best_order : save_order;
+    plan->host.n_split = st == AX_OK ? best_split : save_split;
+    plan->n_split = plan->host.n_split;
+    if (st != AX_OK) return st;
+    if (best_ms_host) *best_ms_host = best / reps;
+    // leave `out` holding the result of the chosen schedule
+    st = run_stage_batch(&plan, 1, AX_STAGE_CONTRACT, &P, &Q, &out, s);
+    if (st == AX_OK) st = run_stage_batch(&plan, 1, AX_STAGE_REDUCE, &P, &Q, &out, s);
+    if (st == AX_OK) st = mark_done(&plan, 1, s);
+    return st;
+}
+
+// ---------------------------------------------------------------------------
 // Single block: packed upper triangle of D[rows] Y^T.
 // ---------------------------------------------------------------------------
+namespace {
+
+struct GemmWorkspace {
+    int64_t o_seg, o_work, o_part, o_Ds, o_Ys, o_De, o_Ye, size;
+};
+
+GemmWorkspace gemm_workspace(int m, int64_t n, int precision, int n_work, int n_split) {
+    Layout L;
+    GemmWorkspace g{};
+    g.o_seg = L.add(sizeof(ax::RkSeg));
+    g.o_work = L.add((int64_t)n_work * sizeof(ax::RkWork));
+    g.o_part = L.add(ax::rk_gemm_partial_doubles(n_work, n_split) * 8);
+    if (precision == AX_PREC_SPLIT) {
+        const int64_t npad = (n + ax::kSplitKTile - 1) / ax::kSplitKTile * ax::kSplitKTile;
+        const int64_t tiles = npad / ax::kSplitKTile;
+        g.o_Ds = L.add((int64_t)ax::kSplitSlices * m * npad);
+        g.o_Ys = L.add((int64_t)ax::kSplitSlices * m * npad);
+        g.o_De = L.add((int64_t)m * tiles * 4);
+        g.o_Ye = L.add((int64_t)m * tiles * 4);
+    }
+    g.size = L.size;
+    return g;
+}
+
+}  // namespace
+
 int64_t ax_rk_gemm_workspace_bytes(int m, int64_t n, int precision) {
     if (m <= 0 || n <= 0) return 0;
     std::vector<ax::RkWork> work;
     ax::upper_tiles(m, 0, work);
     const int nw = static_cast<int>(work.size());
-    const int split = choose_split(nw, n);
-    int64_t bytes = 256 + (int64_t)sizeof(ax::RkSeg) + (int64_t)nw * sizeof(ax::RkWork);
-    bytes = (bytes + 255) / 256 * 256;
-    bytes += ax::rk_gemm_partial_doubles(nw, split) * 8;
-    if (precision == AX_PREC_SPLIT) {
-        const int64_t npad = (n + ax::kSplitKTile - 1) / ax::kSplitKTile * ax::kSplitKTile;
-        const int64_t tiles = npad / ax::kSplitKTile;
-        bytes += 2 * ((int64_t)ax::kSplitSlices * m * npad + 256);
-        bytes += 2 * ((int64_t)m * tiles * 4 + 256);
-    }
-    return bytes;
+    return gemm_workspace(m, n, precision, nw, choose_split(nw, n)).size;
 }
 
 ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const double *Y,
@@ -324,19 +559,15 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
     AX_REQUIRE(precision == AX_PREC_FP64 || precision == AX_PREC_SPLIT,
                "ax_rk_gemm: unknown precision %d", precision);
     if (m == 0) return AX_OK;
-    AX_REQUIRE(workspace != nullptr, "ax_rk_gemm: workspace required");
+    AX_REQUIRE(workspace != nullptr && out != nullptr, "ax_rk_gemm: workspace and output required");
     hipStream_t s = ax::as_stream(stream);
     std::vector<ax::RkWork> work;
     ax::upper_tiles(m, 0, work);
     const int nw = static_cast<int>(work.size());
     const int split = choose_split(nw, n);
+    const GemmWorkspace g = gemm_workspace(m, n, precision, nw, split);
     char *ws = static_cast<char *>(workspace);
-    ax::RkSeg *segs_dev = reinterpret_cast<ax::RkSeg *>(ws);
-    ax::RkWork *work_dev = reinterpret_cast<ax::RkWork *>(ws + 256);
-    int64_t off = 256 + (int64_t)nw * sizeof(ax::RkWork);
-    off = (off + 255) / 256 * 256;
-    double *partial = reinterpret_cast<double *>(ws + off);
-    off += ax::rk_gemm_partial_doubles(nw, split) * 8;
+
     ax::RkSeg sg{};
     sg.D = D;
     sg.ld_d = ld_d;
@@ -349,16 +580,10 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
     if (precision == AX_PREC_SPLIT) {
         const int64_t npad = (n + ax::kSplitKTile - 1) / ax::kSplitKTile * ax::kSplitKTile;
         const int64_t tiles = npad / ax::kSplitKTile;
-        int8_t *Ds = reinterpret_cast<int8_t *>(ws + off);
-        off += (int64_t)ax::kSplitSlices * m * npad + 256;
-        off = (off + 255) / 256 * 256;
-        int8_t *Ys = reinterpret_cast<int8_t *>(ws + off);
-        off += (int64_t)ax::kSplitSlices * m * npad + 256;
-        off = (off + 255) / 256 * 256;
-        int32_t *De = reinterpret_cast<int32_t *>(ws + off);
-        off += (int64_t)m * tiles * 4 + 256;
-        off = (off + 255) / 256 * 256;
-        int32_t *Ye = reinterpret_cast<int32_t *>(ws + off);
+        int8_t *Ds = reinterpret_cast<int8_t *>(ws + g.o_Ds);
+        int8_t *Ys = reinterpret_cast<int8_t *>(ws + g.o_Ys);
+        int32_t *De = reinterpret_cast<int32_t *>(ws + g.o_De);
+        int32_t *Ye = reinterpret_cast<int32_t *>(ws + g.o_Ye);
         // D rows are gathered while slicing, so the sliced operand is identity-indexed
         ax_status st = ax::slice_rows_launch(D, ld_d, rows, m, n, npad, Ds, npad, (int64_t)m * npad,
                                              De, tiles, s);
@@ -378,10 +603,21 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
         sg.Ye = Ye;
         sg.ye_ld = tiles;
     }
-    AX_CHECK_HIP(hipMemcpyAsync(segs_dev, &sg, sizeof(sg), hipMemcpyHostToDevice, s));
-    AX_CHECK_HIP(hipMemcpyAsync(work_dev, work.data(), work.size() * sizeof(ax::RkWork),
+    AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_seg, &sg, sizeof(sg), hipMemcpyHostToDevice, s));
+    AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_work, work.data(), work.size() * sizeof(ax::RkWork),
                                 hipMemcpyHostToDevice, s));
-    ax_status st = ax::rk_gemm_launch(segs_dev, work_dev, nw, split, precision, partial, out, s);
+    ax::PlanHost h{};
+    h.n = n;
+    h.n_work = nw;
+    h.n_split = split;
+    h.precision = precision;
+    h.segs = reinterpret_cast<const ax::RkSeg *>(ws + g.o_seg);
+    h.work = reinterpret_cast<const ax::RkWork *>(ws + g.o_work);
+    h.partial = reinterpret_cast<double *>(ws + g.o_part);
+    const ax::PlanHost *hp = &h;
+    ax_status st = ax::contract_batch_launch(&hp, 1, s);
+    if (st != AX_OK) return st;
+    st = ax::reduce_batch_launch(&hp, 1, &out, s);
     if (st != AX_OK) return st;
     // the host-side descriptors above must outlive the asynchronous copies
     AX_CHECK_HIP(hipStreamSynchronize(s));

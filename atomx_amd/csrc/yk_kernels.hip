@@ -8,6 +8,7 @@
 // (floor(64/ng) cells per wave) so that every global load of a wave is one
 // contiguous run of doubles.
 #include "ax_common.h"
+#include "rk_gemm.h"
 
 namespace {
 
@@ -51,6 +52,42 @@ __global__ void pair_density_kernel(const double *__restrict__ P, const double *
             if (Q) v += Qa[x] * Qb[x];
             if (w) v = w[x] * v;
             o[x] = v;
+        }
+    }
+}
+
+// Batched weighted pair densities D_p = w (P_a P_b + Q_a Q_b) of several plans:
+// blockIdx.y walks the concatenated pair list, blockIdx.x the grid points.
+// For split-precision plans the same pass writes the int8 slices of D: every
+// wave holds one 64-aligned run of x, i.e. one K-tile, so the loop runs to
+// the padded length (wave-uniform) with zeros beyond n.
+__global__ __launch_bounds__(256) void pair_density_batch_kernel(
+    const ax::BatchOf<ax::DensArg> A) {
+    const int64_t total = A.end[A.n - 1];
+    for (int64_t gp = blockIdx.y; gp < total; gp += gridDim.y) {
+        int64_t p;
+        const int i = ax::batch_find(A, gp, p);
+        const ax::DensArg &g = A.a[i];
+        const int64_t n = g.n, ld = g.ld;
+        const int64_t a = g.pa[p], b = g.pb[p];
+        const double *Pa = g.P + a * ld;
+        const double *Pb = g.P + b * ld;
+        const double *Qa = g.Q ? g.Q + a * ld : nullptr;
+        const double *Qb = g.Q ? g.Q + b * ld : nullptr;
+        double *o = g.D + p * n;
+        const int64_t lim = g.Ds ? g.npad : n;
+        for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < lim;
+             x += (int64_t)gridDim.x * blockDim.x) {
+            double v = 0.0;
+            if (x < n) {
+                v = Pa[x] * Pb[x];
+                if (Qa) v += Qa[x] * Qb[x];
+                v = g.w[x] * v;
+                o[x] = v;
+            }
+            if (g.Ds)
+                ax::slice_value(v, x, g.Ds + p * g.npad, g.ds_plane,
+                                g.De + p * (g.npad / ax::kSplitKTile) + x / ax::kSplitKTile);
         }
     }
 }
@@ -220,6 +257,154 @@ constexpr int kScanThreads = 512;
 constexpr int kScanWaves = kScanThreads / 64;
 constexpr int kMaxRounds = 4;  // nv <= kScanThreads * kMaxRounds
 
+// LDS-resident scan of one (density row, k) item by a whole work-group:
+// g_out/g_in of the row in LDS (cell pitch NG+1 against bank conflicts),
+// per-cell sums, wave-shuffle prefix/suffix scans over cells, then one lane
+// per (cell, node) applies the partial matrices and writes Y coalesced.
+template <int NG>
+struct ScanLds {
+    double (*wt_o)[kScanWaves];
+    double (*wt_i)[kScanWaves];
+    double mo_r[NG], mi_r[NG];
+};
+
+// SLICE (NG = 8 only): instead of the FP64 row, write its split-precision
+// slices (planes sdst[s*splane + x], exponents edst[tile]); the eight cells of
+// one wave are exactly one 64-node K-tile.
+template <int NG, bool SLICE = false>
+__device__ inline void scan_item(double *__restrict__ sm, ScanLds<NG> &L,
+                                 const double *__restrict__ src, const double *__restrict__ w,
+                                 const double *__restrict__ rkp, const double *__restrict__ rkmp,
+                                 int64_t n, int nv, double *__restrict__ dst,
+                                 int8_t *__restrict__ sdst = nullptr, int64_t splane = 0,
+                                 int32_t *__restrict__ edst = nullptr) {
+    static_assert(!SLICE || NG * (64 / NG) == 64, "in-scan slicing needs whole K-tiles per wave");
+    constexpr int P = NG + 1;
+    double *gos = sm;                          // [nv * P] g_out
+    double *gis = sm + (int64_t)nv * P;        // [nv * P] g_in
+    double *bo = sm + (int64_t)2 * nv * P;     // [nv] outward base per cell
+    double *bi = bo + nv;                      // [nv] inward base per cell
+    auto wt_o = L.wt_o;
+    auto wt_i = L.wt_i;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int cpw = 64 / NG;
+    const int ciw = lane / NG, node = lane - (lane / NG) * NG;
+    const bool lane_used = lane < cpw * NG;
+    const double *mo_r = L.mo_r;
+    const double *mi_r = L.mi_r;
+    const int rounds = (nv + kScanThreads - 1) / kScanThreads;
+    __syncthreads();  // the previous item is done with the LDS
+    for (int64_t x = tid; x < n; x += kScanThreads) {
+        double d = src[x];
+        if (w) d = w[x] * d;
+        const int64_t px = x + x / NG;
+        gos[px] = d * rkp[x];
+        gis[px] = d * rkmp[x];
+    }
+    __syncthreads();
+    double so[kMaxRounds], si[kMaxRounds], ex_o[kMaxRounds], ex_i[kMaxRounds];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        so[r] = si[r] = ex_o[r] = ex_i[r] = 0.0;
+        if (r < rounds) {
+            const int c = r * kScanThreads + tid;
+            if (c < nv) {
+                double sa = 0.0, sb = 0.0;
+#pragma unroll
+                for (int j = 0; j < NG; ++j) {
+                    sa += gos[(int64_t)c * P + j];
+                    sb += gis[(int64_t)c * P + j];
+                }
+                so[r] = sa;
+                si[r] = sb;
+            }
+            double pa = so[r], sb2 = si[r];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const double va = __shfl_up(pa, off, 64);
+                const double vb = __shfl_down(sb2, off, 64);
+                if (lane >= off) pa += va;
+                if (lane + off < 64) sb2 += vb;
+            }
+            if (lane == 63) wt_o[r][wave] = pa;
+            if (lane == 0) wt_i[r][wave] = sb2;
+            const double ea = __shfl_up(pa, 1, 64);
+            const double eb = __shfl_down(sb2, 1, 64);
+            ex_o[r] = lane == 0 ? 0.0 : ea;
+            ex_i[r] = lane == 63 ? 0.0 : eb;
+        }
+    }
+    __syncthreads();
+    double carry = 0.0;
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        if (r < rounds) {
+            double before = carry;
+            for (int v = 0; v < wave; ++v) before += wt_o[r][v];
+            for (int v = 0; v < kScanWaves; ++v) carry += wt_o[r][v];
+            const int c = r * kScanThreads + tid;
+            if (c < nv) bo[c] = ex_o[r] + before;
+        }
+    }
+    carry = 0.0;
+#pragma unroll
+    for (int r = kMaxRounds - 1; r >= 0; --r) {
+        if (r < rounds) {
+            double after = carry;
+            for (int v = kScanWaves - 1; v > wave; --v) after += wt_i[r][v];
+            for (int v = 0; v < kScanWaves; ++v) carry += wt_i[r][v];
+            const int c = r * kScanThreads + tid;
+            if (c < nv) bi[c] = ex_i[r] + after;
+        }
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < nv; c0 += cpw * kScanWaves) {
+        const int c = c0 + wave * cpw + ciw;
+        if constexpr (SLICE) {
+            if (c0 + wave * cpw < nv) {  // wave-uniform: the wave's tile exists
+                double v = 0.0;
+                if (c < nv) {
+                    const double *a = gos + (int64_t)c * P;
+                    const double *b = gis + (int64_t)c * P;
+                    double po = 0.0, pi = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NG; ++j) {
+                        po += mo_r[j] * a[j];
+                        pi += mi_r[j] * b[j];
+                    }
+                    const int64_t x = (int64_t)c * NG + node;
+                    v = rkmp[x] * (bo[c] + po) + rkp[x] * (bi[c] + pi);
+                }
+                const int64_t x = (int64_t)c * NG + node;
+                ax::slice_value(v, x, sdst, splane, edst + x / ax::kSplitKTile);
+            }
+        } else if (lane_used && c < nv) {
+            const double *a = gos + (int64_t)c * P;
+            const double *b = gis + (int64_t)c * P;
+            double po = 0.0, pi = 0.0;
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                po += mo_r[j] * a[j];
+                pi += mi_r[j] * b[j];
+            }
+            const int64_t x = (int64_t)c * NG + node;
+            dst[x] = rkmp[x] * (bo[c] + po) + rkp[x] * (bi[c] + pi);
+        }
+    }
+}
+
+template <int NG>
+__device__ inline void scan_load_partials(ScanLds<NG> &L, const double *__restrict__ mo,
+                                          const double *__restrict__ mi) {
+    const int lane = threadIdx.x & 63;
+    const int node = lane - (lane / NG) * NG;
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+        L.mo_r[j] = mo[node * NG + j];
+        L.mi_r[j] = mi[node * NG + j];
+    }
+}
+
 template <int NG>
 __global__ __launch_bounds__(kScanThreads) void yk_scan_lds_kernel(
     const double *__restrict__ rho, int64_t ld_rho, const double *__restrict__ w,
@@ -228,114 +413,51 @@ __global__ __launch_bounds__(kScanThreads) void yk_scan_lds_kernel(
     const int32_t *__restrict__ row, const int32_t *__restrict__ kk,
     const int32_t *__restrict__ out_row, double *__restrict__ y, int64_t ld_y) {
     extern __shared__ double sm[];
-    constexpr int P = NG + 1;
-    double *gos = sm;                          // [nv * P] g_out
-    double *gis = sm + (int64_t)nv * P;        // [nv * P] g_in
-    double *bo = sm + (int64_t)2 * nv * P;     // [nv] outward base per cell
-    double *bi = bo + nv;                      // [nv] inward base per cell
     __shared__ double wt_o[kMaxRounds][kScanWaves], wt_i[kMaxRounds][kScanWaves];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // phase-3 lane role: cell_in_wave, node; rows of the partial matrices
-    constexpr int cpw = 64 / NG;
-    const int ciw = lane / NG, node = lane - (lane / NG) * NG;
-    const bool lane_used = lane < cpw * NG;
-    double mo_r[NG], mi_r[NG];
-#pragma unroll
-    for (int j = 0; j < NG; ++j) {
-        mo_r[j] = mo[node * NG + j];
-        mi_r[j] = mi[node * NG + j];
-    }
-    const int rounds = (nv + kScanThreads - 1) / kScanThreads;
+    ScanLds<NG> L;
+    L.wt_o = wt_o;
+    L.wt_i = wt_i;
+    scan_load_partials<NG>(L, mo, mi);
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const int k = kk[it];
         double *dst = y + (int64_t)out_row[it] * ld_y;
         if (k < 0 || k > kmax) {
-            for (int64_t x = tid; x < n; x += kScanThreads) dst[x] = __builtin_nan("");
+            for (int64_t x = threadIdx.x; x < n; x += kScanThreads) dst[x] = __builtin_nan("");
             continue;
         }
-        const double *src = rho + (int64_t)row[it] * ld_rho;
-        const double *rkp = rk_tab + (int64_t)k * n;
-        const double *rkmp = rkm_tab + (int64_t)k * n;
-        __syncthreads();  // the previous item is done with the LDS
-        for (int64_t x = tid; x < n; x += kScanThreads) {
-            double d = src[x];
-            if (w) d = w[x] * d;
-            const int64_t px = x + x / NG;
-            gos[px] = d * rkp[x];
-            gis[px] = d * rkmp[x];
-        }
-        __syncthreads();
-        double so[kMaxRounds], si[kMaxRounds], ex_o[kMaxRounds], ex_i[kMaxRounds];
-#pragma unroll
-        for (int r = 0; r < kMaxRounds; ++r) {
-            so[r] = si[r] = ex_o[r] = ex_i[r] = 0.0;
-            if (r < rounds) {
-                const int c = r * kScanThreads + tid;
-                if (c < nv) {
-                    double sa = 0.0, sb = 0.0;
-#pragma unroll
-                    for (int j = 0; j < NG; ++j) {
-                        sa += gos[(int64_t)c * P + j];
-                        sb += gis[(int64_t)c * P + j];
-                    }
-                    so[r] = sa;
-                    si[r] = sb;
-                }
-                double pa = so[r], sb2 = si[r];
-#pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    const double va = __shfl_up(pa, off, 64);
-                    const double vb = __shfl_down(sb2, off, 64);
-                    if (lane >= off) pa += va;
-                    if (lane + off < 64) sb2 += vb;
-                }
-                if (lane == 63) wt_o[r][wave] = pa;
-                if (lane == 0) wt_i[r][wave] = sb2;
-                const double ea = __shfl_up(pa, 1, 64);
-                const double eb = __shfl_down(sb2, 1, 64);
-                ex_o[r] = lane == 0 ? 0.0 : ea;
-                ex_i[r] = lane == 63 ? 0.0 : eb;
-            }
-        }
-        __syncthreads();
-        double carry = 0.0;
-#pragma unroll
-        for (int r = 0; r < kMaxRounds; ++r) {
-            if (r < rounds) {
-                double before = carry;
-                for (int v = 0; v < wave; ++v) before += wt_o[r][v];
-                for (int v = 0; v < kScanWaves; ++v) carry += wt_o[r][v];
-                const int c = r * kScanThreads + tid;
-                if (c < nv) bo[c] = ex_o[r] + before;
-            }
-        }
-        carry = 0.0;
-#pragma unroll
-        for (int r = kMaxRounds - 1; r >= 0; --r) {
-            if (r < rounds) {
-                double after = carry;
-                for (int v = kScanWaves - 1; v > wave; --v) after += wt_i[r][v];
-                for (int v = 0; v < kScanWaves; ++v) carry += wt_i[r][v];
-                const int c = r * kScanThreads + tid;
-                if (c < nv) bi[c] = ex_i[r] + after;
-            }
-        }
-        __syncthreads();
-        for (int c0 = 0; c0 < nv; c0 += cpw * kScanWaves) {
-            const int c = c0 + wave * cpw + ciw;
-            if (lane_used && c < nv) {
-                const double *a = gos + (int64_t)c * P;
-                const double *b = gis + (int64_t)c * P;
-                double po = 0.0, pi = 0.0;
-#pragma unroll
-                for (int j = 0; j < NG; ++j) {
-                    po += mo_r[j] * a[j];
-                    pi += mi_r[j] * b[j];
-                }
-                const int64_t x = (int64_t)c * NG + node;
-                dst[x] = rkmp[x] * (bo[c] + po) + rkp[x] * (bi[c] + pi);
-            }
-        }
+        scan_item<NG>(sm, L, rho + (int64_t)row[it] * ld_rho, w, rk_tab + (int64_t)k * n,
+                      rkm_tab + (int64_t)k * n, n, nv, dst);
+    }
+}
+
+// Batched scan over the items of several plans (same NG; the partial
+// matrices depend only on NG).  Plan densities are already weighted.
+template <int NG, bool SLICE>
+__global__ __launch_bounds__(kScanThreads) void yk_scan_batch_kernel(
+    const ax::BatchOf<ax::ScanArg> A, const double *__restrict__ mo,
+    const double *__restrict__ mi) {
+    extern __shared__ double sm[];
+    __shared__ double wt_o[kMaxRounds][kScanWaves], wt_i[kMaxRounds][kScanWaves];
+    ScanLds<NG> L;
+    L.wt_o = wt_o;
+    L.wt_i = wt_i;
+    scan_load_partials<NG>(L, mo, mi);
+    const int64_t total = A.end[A.n - 1];
+    for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+        int64_t local;
+        const int p = ax::batch_find(A, it, local);
+        const ax::ScanArg &g = A.a[p];
+        const int k = g.item_k[local];
+        const int64_t n = g.n;
+        const int64_t orow = g.item_out[local];
+        const double *src = g.D + (int64_t)g.item_row[local] * n;
+        if constexpr (SLICE)
+            scan_item<NG, true>(sm, L, src, nullptr, g.rk + (int64_t)k * n,
+                                g.rkm + (int64_t)k * n, n, g.nv, nullptr, g.Ys + orow * g.npad,
+                                g.ys_plane, g.Ye + orow * (g.npad / ax::kSplitKTile));
+        else
+            scan_item<NG>(sm, L, src, nullptr, g.rk + (int64_t)k * n, g.rkm + (int64_t)k * n, n,
+                          g.nv, g.Y + orow * n);
     }
 }
 
@@ -405,6 +527,36 @@ __global__ void bspline_eval_kernel(const double *__restrict__ coeffs, int64_t l
 // at most ks cells, so the node-wise cumulative integrals are constant
 // outside that support; the column is assembled over all cells.
 // ---------------------------------------------------------------------------
+// <B1_gi | w | B2_gj> (numerics/bspline/integrals.py:18-56), one thread per
+// matrix element, gathering the cells that hold both B-splines in ascending
+// order (deterministic, no atomics).
+__global__ void assemble_integral_kernel(int nv, int ng, int ks1, int ks2, int mm,
+                                         const double *__restrict__ b1,
+                                         const double *__restrict__ b2,
+                                         const double *__restrict__ wt, double *__restrict__ mat) {
+    const int64_t total = (int64_t)mm * mm;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int gi = static_cast<int>(t / mm), gj = static_cast<int>(t - (int64_t)gi * mm);
+        int lo = gi - ks1 + 1;
+        if (gj - ks2 + 1 > lo) lo = gj - ks2 + 1;
+        if (lo < 0) lo = 0;
+        int hi = gi < gj ? gi : gj;
+        if (hi > nv - 1) hi = nv - 1;
+        double acc = 0.0;
+        for (int iv = lo; iv <= hi; ++iv) {
+            const int i = gi - iv, j = gj - iv;
+            double s = 0.0;
+            for (int m = 0; m < ng; ++m) {
+                const int64_t c = (int64_t)iv * ng + m;
+                s += wt[c] * b1[c * ks1 + i] * b2[c * ks2 + j];
+            }
+            acc += s;
+        }
+        mat[t] = acc;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void exchange_yk_kernel(
     const double *__restrict__ fb, const double *__restrict__ bsp,
     const double *__restrict__ r, const double *__restrict__ w, int nv, int ng, int ks, int ns,
@@ -569,6 +721,133 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double *__
     if (threadIdx.x == 0) out[0] = sa + sb;
 }
 
+template <int NG, bool SLICE>
+void launch_scan_batch(size_t lds, int grid, hipStream_t s, const ax::BatchOf<ax::ScanArg> &A,
+                       const double *mo, const double *mi) {
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(yk_scan_batch_kernel<NG, SLICE>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipLaunchKernelGGL((yk_scan_batch_kernel<NG, SLICE>), dim3(grid), dim3(kScanThreads), lds, s, A,
+                       mo, mi);
+}
+
+int lds_capacity() {
+    static int cap = -1;
+    if (cap < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+            cap = v;
+        else
+            cap = 65536;
+    }
+    return cap;
+}
+
+size_t scan_lds_bytes(int nv, int ng) {
+    return ((size_t)2 * nv * (ng + 1) + 2 * (size_t)nv) * sizeof(double);
+}
+
+// Whether the LDS-resident scan serves a grid of nv cells of ng nodes.
+bool scan_lds_fits(int nv, int ng) {
+    if (ng != 7 && ng != 8 && ng != 9) return false;
+    const size_t static_lds = 2 * kMaxRounds * kScanWaves * sizeof(double);
+    return nv <= kScanThreads * kMaxRounds &&
+           scan_lds_bytes(nv, ng) + static_lds <= (size_t)lds_capacity();
+}
+
+}  // namespace
+
+namespace ax {
+
+bool scan_slices_fused(int ng, int nv) { return ng == 8 && scan_lds_fits(nv, ng); }
+
+ax_status density_batch_launch(const PlanHost *const *plans, int n, const double *const *P,
+                               const double *const *Q, hipStream_t stream) {
+    if (n <= 0) return AX_OK;
+    if (n > kMaxBatch) {
+        set_error("density_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
+        return AX_ERR_VALUE;
+    }
+    BatchOf<DensArg> A{};
+    A.n = n;
+    int64_t total = 0, nmax = 1;
+    for (int i = 0; i < n; ++i) {
+        const PlanHost *h = plans[i];
+        if (!P[i]) {
+            set_error("density_batch_launch: null orbital samples for plan %d", i);
+            return AX_ERR_VALUE;
+        }
+        total += h->n_pairs;
+        A.end[i] = total;
+        const bool sl = h->precision == AX_PREC_SPLIT;
+        A.a[i] = DensArg{P[i], Q ? Q[i] : nullptr, h->n, h->n, h->pa, h->pb, h->w, h->D,
+                         sl ? h->Ds : nullptr, sl ? h->De : nullptr, h->npad,
+                         (int64_t)h->n_pairs * h->npad};
+        const int64_t len = sl ? h->npad : h->n;
+        if (len > nmax) nmax = len;
+    }
+    if (total == 0) return AX_OK;
+    dim3 grid(grid_for(nmax, 256, 64), total < 65535 ? static_cast<unsigned>(total) : 65535u);
+    hipLaunchKernelGGL(pair_density_batch_kernel, grid, dim3(256), 0, stream, A);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream) {
+    if (n <= 0) return AX_OK;
+    if (n > kMaxBatch) {
+        set_error("scan_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
+        return AX_ERR_VALUE;
+    }
+    // one LDS-scan launch per (n_gauss, in-scan slicing); grids the LDS scan
+    // does not serve go through the generic kernel one plan at a time
+    for (int ng : {7, 8, 9}) {
+        for (int sl = 0; sl < (ng == 8 ? 2 : 1); ++sl) {
+            BatchOf<ScanArg> A{};
+            int64_t total = 0;
+            int nv_max = 0;
+            const double *mo = nullptr, *mi = nullptr;
+            for (int i = 0; i < n; ++i) {
+                const PlanHost *h = plans[i];
+                if (h->ng != ng || !scan_lds_fits(h->nv, h->ng) || h->n_items == 0) continue;
+                const bool fused = h->precision == AX_PREC_SPLIT && h->Y == nullptr;
+                if (fused != (sl == 1)) continue;
+                total += h->n_items;
+                A.end[A.n] = total;
+                const int64_t yr = h->n_yrows > 0 ? h->n_yrows : 1;
+                A.a[A.n] = ScanArg{h->D, h->Y, h->rk, h->rkm, h->item_row, h->item_k,
+                                   h->item_out, h->n, h->nv, h->kmax, h->Ys, h->Ye, h->npad,
+                                   yr * h->npad};
+                ++A.n;
+                if (h->nv > nv_max) nv_max = h->nv;
+                mo = h->mo;
+                mi = h->mi;
+            }
+            if (A.n == 0) continue;
+            const int grid = total < 8192 ? static_cast<int>(total) : 8192;
+            const size_t lds = scan_lds_bytes(nv_max, ng);
+            if (ng == 7) launch_scan_batch<7, false>(lds, grid, stream, A, mo, mi);
+            if (ng == 8 && sl == 0) launch_scan_batch<8, false>(lds, grid, stream, A, mo, mi);
+            if (ng == 8 && sl == 1) launch_scan_batch<8, true>(lds, grid, stream, A, mo, mi);
+            if (ng == 9) launch_scan_batch<9, false>(lds, grid, stream, A, mo, mi);
+            AX_CHECK_LAUNCH();
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        const PlanHost *h = plans[i];
+        if (scan_lds_fits(h->nv, h->ng) || h->n_items == 0) continue;
+        ax_status st = ax_yk_batch(h->D, h->n, nullptr, h->rk, h->rkm, h->n, h->nv, h->ng,
+                                   h->kmax, h->mo, h->mi, h->n_items, h->item_row, h->item_k,
+                                   h->item_out, h->Y, h->n, stream);
+        if (st != AX_OK) return st;
+    }
+    return AX_OK;
+}
+
+}  // namespace ax
+
+namespace {
 }  // namespace
 
 extern "C" {
@@ -607,33 +886,21 @@ ax_status ax_yk_batch(const double *rho, int64_t ld_rho, const double *w, const 
     AX_REQUIRE((size_t)2 * nv * sizeof(double) <= 48 * 1024,
                "ax_yk_batch: nv=%d exceeds the LDS cell-sum capacity", nv);
     if (n_items <= 0) return AX_OK;
-    if (ng == 7 || ng == 8 || ng == 9) {
-        static int lds_cap = -1;
-        if (lds_cap < 0) {
-            int dev = 0, v = 0;
-            if (hipGetDevice(&dev) == hipSuccess &&
-                hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
-                lds_cap = v;
-            else
-                lds_cap = 65536;
-        }
-        const size_t lds = ((size_t)2 * nv * (ng + 1) + 2 * (size_t)nv) * sizeof(double);
-        const size_t static_lds = 2 * kMaxRounds * kScanWaves * sizeof(double);
-        if (nv <= kScanThreads * kMaxRounds && lds + static_lds <= (size_t)lds_cap) {
-            const int grid = n_items < 8192 ? n_items : 8192;
-            hipStream_t s = ax::as_stream(stream);
-            if (ng == 7)
-                launch_scan_lds<7>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi,
-                                   n_items, row, kk, out_row, y, ld_y);
-            else if (ng == 8)
-                launch_scan_lds<8>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi,
-                                   n_items, row, kk, out_row, y, ld_y);
-            else
-                launch_scan_lds<9>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi,
-                                   n_items, row, kk, out_row, y, ld_y);
-            AX_CHECK_LAUNCH();
-            return AX_OK;
-        }
+    if (scan_lds_fits(nv, ng)) {
+        const size_t lds = scan_lds_bytes(nv, ng);
+        const int grid = n_items < 8192 ? n_items : 8192;
+        hipStream_t s = ax::as_stream(stream);
+        if (ng == 7)
+            launch_scan_lds<7>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi, n_items,
+                               row, kk, out_row, y, ld_y);
+        else if (ng == 8)
+            launch_scan_lds<8>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi, n_items,
+                               row, kk, out_row, y, ld_y);
+        else
+            launch_scan_lds<9>(lds, grid, s, rho, ld_rho, w, rk, rkm, n, nv, kmax, mo, mi, n_items,
+                               row, kk, out_row, y, ld_y);
+        AX_CHECK_LAUNCH();
+        return AX_OK;
     }
     const size_t shmem = (size_t)2 * nv * sizeof(double);
     const int grid = n_items < 4096 ? n_items : 4096;
@@ -665,6 +932,19 @@ ax_status ax_bspline_eval(const double *coeffs, int64_t ld_c, int n_orb, const d
     dim3 grid(ax::grid_for((int64_t)nv * ng, 256, 64), n_orb < 65535 ? n_orb : 65535);
     hipLaunchKernelGGL(bspline_eval_kernel, grid, dim3(256), 0, ax::as_stream(stream), coeffs,
                        ld_c, n_orb, bsp, nv, ng, ks, ns, out, ld_out);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_assemble_integral(int nv, int ng, int ks1, int ks2, int mm, const double *bsp1,
+                               const double *bsp2, const double *weights, double *mat,
+                               void *stream) {
+    AX_REQUIRE(nv > 0 && ng > 0 && ks1 > 0 && ks2 > 0 && mm > 0,
+               "ax_assemble_integral: bad sizes (nv=%d ng=%d ks=%d,%d mm=%d)", nv, ng, ks1, ks2, mm);
+    AX_REQUIRE(bsp1 && bsp2 && weights && mat, "ax_assemble_integral: null argument");
+    const int64_t total = (int64_t)mm * mm;
+    hipLaunchKernelGGL(assemble_integral_kernel, dim3(ax::grid_for(total, 256, 8192)), dim3(256), 0,
+                       ax::as_stream(stream), nv, ng, ks1, ks2, mm, bsp1, bsp2, weights, mat);
     AX_CHECK_LAUNCH();
     return AX_OK;
 }

@@ -61,10 +61,7 @@ class SlaterTensor:
     def run(self, P, Q, out=None, stream=None):
         """P, Q device tensors (n_orb, n) FP64 (or host arrays); returns the
         packed output (device tensor of ``size`` values)."""
-        P = dv.f64(P)
-        Q = dv.f64(Q)
-        if P.shape != (self.n_orb, self.n) or Q.shape != (self.n_orb, self.n):
-            raise ValueError(f"orbitals must be ({self.n_orb}, {self.n})")
+        P, Q = self.check_inputs(P, Q)
         if out is None:
             out = dv.empty((self.size,))
         call("ax_rk_tensor_run", self._h, dv.ptr(P), dv.ptr(Q), dv.ptr(out),
@@ -72,6 +69,35 @@ class SlaterTensor:
         return out
 
     STAGES = ("density", "scan", "slice", "contract", "reduce")
+    ORDERS = ("tile-major", "xcd-contiguous", "xcd-contiguous-split-fastest")
+
+    def schedule(self):
+        """(block order name, K split) of the contraction."""
+        o, s = ctypes.c_int(), ctypes.c_int()
+        call("ax_rk_tensor_get_schedule", self._h, ctypes.byref(o), ctypes.byref(s))
+        return self.ORDERS[o.value], s.value
+
+    def set_schedule(self, order, n_split):
+        call("ax_rk_tensor_set_schedule", self._h, self.ORDERS.index(order), int(n_split))
+
+    def tune(self, P, Q, out=None, reps=3, stream=None):
+        """Times every contraction schedule on these inputs, keeps the fastest
+        and returns (order, split, ms per contraction + reduction); ``out``
+        then holds the tuned result."""
+        P, Q = self.check_inputs(P, Q)
+        if out is None:
+            out = dv.empty((self.size,))
+        ms = ctypes.c_float()
+        call("ax_rk_tensor_tune", self._h, dv.ptr(P), dv.ptr(Q), dv.ptr(out), int(reps),
+             dv.stream_handle(stream), ctypes.byref(ms))
+        return self.schedule() + (float(ms.value),)
+
+    def check_inputs(self, P, Q):
+        P = dv.f64(P)
+        Q = dv.f64(Q) if Q is not None else None
+        if P.shape != (self.n_orb, self.n) or (Q is not None and Q.shape != P.shape):
+            raise ValueError(f"orbitals must be ({self.n_orb}, {self.n})")
+        return P, Q
 
     def run_stage(self, stage, P=None, Q=None, out=None, stream=None):
         """Launches one stage of ``run`` (per-kernel timing); P/Q/out must be
@@ -102,3 +128,24 @@ class SlaterTensor:
     @staticmethod
     def pairs(n_orb):
         return pair_list(n_orb)
+
+
+def run_batch(plans, Ps, Qs, outs=None, stream=None):
+    """Runs several plans (different grids, orbital counts and k ranges, one
+    precision mode) as one set of five launches.  Returns the list of packed
+    outputs; each equals bitwise what ``plan.run`` alone returns."""
+    if not plans:
+        return []
+    if len(Ps) != len(plans) or len(Qs) != len(plans):
+        raise ValueError("one (P, Q) per plan")
+    if outs is None:
+        outs = [dv.empty((p.size,)) for p in plans]
+    ins = [p.check_inputs(P, Q) for p, P, Q in zip(plans, Ps, Qs)]
+    nb = len(plans)
+    arr = ctypes.c_void_p * nb
+    h = arr(*[p._h.value for p in plans])
+    pp = arr(*[dv.ptr(P) for P, _ in ins])
+    qq = arr(*[dv.ptr(Q) if Q is not None else None for _, Q in ins])
+    oo = arr(*[dv.ptr(o) for o in outs])
+    call("ax_rk_tensor_run_batch", h, nb, pp, qq, oo, dv.stream_handle(stream))
+    return outs

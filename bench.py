@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--n-orb", type=int, default=40)
     ap.add_argument("--kmax", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB device-to-device once "
+                         "(a kernel of known traffic for calibrating HBM byte counters)")
     return ap.parse_args()
 
 
@@ -138,18 +141,14 @@ def main():
     plans = {}
     for prec in ("fp64", "split"):
         plan = SlaterTensor(gr, grw, ls, args.kmax, precision=prec)
-        out = plan.run(Pd, Qd)
+        out = dv.empty((plan.size,))
+        order, split, tuned_ms = plan.tune(Pd, Qd, out)  # schedule autotuner (untimed set-up)
         torch.cuda.synchronize()
         el, sc = sample_errors(plan, dv.host(out), P, Q, gr, grw)
         plans[prec] = (plan, out)
-        modes[prec] = {"max_rel_err": el, "max_err_over_abs_integrand": sc}
-
-    if args.precision == "auto":
-        prec = "split" if (modes["split"]["max_rel_err"] <= 1e-11
-                           and modes["split"]["max_err_over_abs_integrand"] <= 1e-11) else "fp64"
-    else:
-        prec = args.precision
-    plan, out = plans[prec]
+        modes[prec] = {"max_rel_err": el, "max_err_over_abs_integrand": sc,
+                       "schedule": {"order": order, "k_split": split,
+                                    "contract_reduce_ms": tuned_ms}}
 
     # whole-step timing, and both modes for the record
     for name, (pl, o) in plans.items():
@@ -158,6 +157,15 @@ def main():
         ms = time_steps(lambda: pl.run(Pd, Qd, out=o), args.steps, stream)
         modes[name]["ms_per_step"] = ms
         modes[name]["rk_per_s"] = n_rk / (ms * 1e-3)
+
+    # the headline is the faster mode among those meeting the accuracy bar
+    ok = [m for m in ("split", "fp64") if modes[m]["max_rel_err"] <= 1e-11
+          and modes[m]["max_err_over_abs_integrand"] <= 1e-11]
+    if args.precision == "auto":
+        prec = max(ok or ["fp64"], key=lambda m: modes[m]["rk_per_s"])
+    else:
+        prec = args.precision
+    plan, out = plans[prec]
 
     # per-stage timing (HIP events on the launch stream)
     stages = {}
@@ -201,6 +209,13 @@ def main():
     print(json.dumps(line), flush=True)
     for pl, _ in plans.values():
         pl.close()
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy calibration: {args.copy_mib} MiB read + {args.copy_mib} MiB written", flush=True)
 
 
 if __name__ == "__main__":

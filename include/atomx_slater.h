@@ -96,6 +96,17 @@ ax_status ax_bspline_eval(const double *coeffs, int64_t ld_c, int n_orb, const d
                           int nv, int ng, int ks, int ns, double *out, int64_t ld_out,
                           void *stream);
 
+/* numerics/bspline/integrals.py:18-56 _assemble_integral:
+ * mat[gi][gj] = sum_iv sum_m weights[iv][m] bsp1[iv][m][gi-iv] bsp2[iv][m][gj-iv]
+ * over the cells iv that hold both B-splines (iv ascending, as the
+ * reference accumulates), for gi, gj < mm; mat (mm, mm) is overwritten and
+ * holds zeros outside the band.  bsp1 (nv, ng, ks1), bsp2 (nv, ng, ks2),
+ * weights (nv, ng).  With weights = grw * Y^0[rho] this is the direct
+ * potential of methods/dirac_fock.py:954-965. */
+ax_status ax_assemble_integral(int nv, int ng, int ks1, int ks2, int mm, const double *bsp1,
+                               const double *bsp2, const double *weights, double *mat,
+                               void *stream);
+
 /* numerics/bspline/slater_yk.py:35-122 _exchange_matrix_kernel: K (ns, ns)
  * for orbital samples fb (nv*ng), node-wise cumulative Y^k scans. */
 ax_status ax_exchange_matrix_yk(const double *fb, const double *bsp, const double *r,
@@ -139,6 +150,10 @@ ax_status ax_rk_tensor_create(const double *r_host, const double *w_host, int nv
                               int n_orb, const int32_t *l_host, int kmax, int precision,
                               ax_rk_tensor **plan);
 ax_status ax_rk_tensor_destroy(ax_rk_tensor *plan);
+/* Device bytes ax_rk_tensor_create would hold for this shape (admission by
+ * HBM footprint before anything is allocated); -1 on invalid arguments. */
+int64_t ax_rk_tensor_estimate(int nv, int ng, int n_orb, const int32_t *l_host, int kmax,
+                              int precision);
 int64_t ax_rk_tensor_size(const ax_rk_tensor *plan);
 /* number of valid pairs of rank k (rows of that rank's packed triangle) */
 int ax_rk_tensor_pairs(const ax_rk_tensor *plan, int k);
@@ -148,6 +163,16 @@ int64_t ax_rk_tensor_offset(const ax_rk_tensor *plan, int k);
 int64_t ax_rk_tensor_footprint(const ax_rk_tensor *plan);
 ax_status ax_rk_tensor_run(ax_rk_tensor *plan, const double *P, const double *Q, double *out,
                            void *stream);
+/* Several plans (any grids, orbital counts and k ranges; one precision
+ * mode) as one set of launches: density, Y^k scan (one per n_gauss value),
+ * slicing, contraction and reduction each run once for the whole batch, in
+ * chunks of 32 plans.  P, Q and out are host arrays of n_plans device
+ * pointers (Q may be NULL, or hold NULL entries, for no small component).
+ * Each plan's results are bitwise those of its own ax_rk_tensor_run: tiles,
+ * K splits and summation order are fixed per plan. */
+ax_status ax_rk_tensor_run_batch(ax_rk_tensor *const *plans, int n_plans,
+                                 const double *const *P, const double *const *Q,
+                                 double *const *out, void *stream);
 /* One stage of run(), for per-kernel timing: the stages in order are
  * density (needs P, Q), Y^k scan, slicing (split precision only), the
  * contraction (split partials) and the fixed-order reduction (needs out).
@@ -161,6 +186,31 @@ enum {
 };
 ax_status ax_rk_tensor_run_stage(ax_rk_tensor *plan, int stage, const double *P,
                                  const double *Q, double *out, void *stream);
+
+/* Contraction schedule of a plan: block order (0 = tile-major, 1 = tiles
+ * dealt to the 8 XCDs in contiguous ranges, 2 = as 1 with the K split
+ * fastest) and K split (1 .. 2x the default, power of two).  The split fixes
+ * the reduction order, so results are deterministic per schedule.
+ * tune() runs the plan once on P, Q, then times every (order, split) with
+ * `reps` timed repetitions of the contraction and reduction, keeps the
+ * fastest, and leaves its result in `out`; best_ms_host (may be NULL)
+ * receives the chosen schedule's time per contraction + reduction.
+ * No counterpart in the reference (its contraction is one numpy dot per
+ * integral).  tune() synchronises `stream` and must not be captured. */
+ax_status ax_rk_tensor_set_schedule(ax_rk_tensor *plan, int order, int n_split);
+ax_status ax_rk_tensor_get_schedule(const ax_rk_tensor *plan, int *order_host,
+                                    int *n_split_host);
+ax_status ax_rk_tensor_tune(ax_rk_tensor *plan, const double *P, const double *Q, double *out,
+                            int reps, void *stream, float *best_ms_host);
+
+/* ---- device workspace cache -------------------------------------------------
+ * Plan workspaces come from a caching allocator (size-classed free lists per
+ * device), so creating and destroying plans per request costs no hipMalloc
+ * once the cache is warm.  No counterpart in the reference (its workspaces
+ * are numpy arrays).  cached_bytes: device bytes held in free lists;
+ * trim: waits for the device and returns every cached block to HIP. */
+int64_t ax_pool_cached_bytes(void);
+ax_status ax_pool_trim(void);
 
 /* ---- (C) DBSR cell algorithm ------------------------------------------------
  * Bank layout: bankT[ip][jp][i][j] = R^k(B_i B_{i+ip}; B_j B_{j+jp}), i.e. the

@@ -1,0 +1,168 @@
+"""GPU: batched launches over several R^k plans and HIP-graph replay.
+
+The batch runs every plan's tiles, K splits and summation order unchanged,
+so each plan's output must be bitwise identical alone and in any batch
+(different grids, n_gauss values, orbital counts and k ranges; more plans than
+one launch holds; a grid too large for the LDS-resident scan).  Values are
+checked against the CPU oracle at 1e-12 (FP64) relative to sum_x |D||Y|.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# (nv, ng, n_orb, kmax, seed); nv = 2100 exceeds the LDS scan's capacity
+SHAPES = [(60, 8, 7, 4, 1), (90, 7, 5, 6, 2), (40, 8, 12, 2, 3), (75, 9, 4, 3, 4),
+          (2100, 8, 3, 2, 5)]
+
+
+def _problem(nv, ng, n_orb, seed):
+    from atomx_amd.workloads import synthetic_grid, synthetic_orbitals
+    gr, grw = synthetic_grid(nv=nv, ng=ng, r_min=1e-5, r_max=60.0)
+    P, Q, ls = synthetic_orbitals(gr, grw, n_orb=n_orb, seed=seed)
+    return gr, grw, P, Q, ls
+
+
+@pytest.fixture(scope="module")
+def problems():
+    return [_problem(nv, ng, n_orb, seed) + (kmax,) for nv, ng, n_orb, kmax, seed in SHAPES]
+
+
+def _oracle_scaled_err(plan, out, gr, grw, P, Q, kmax):
+    """max |GPU - oracle| / sum_x |w rho_p| |Y^k[rho_q]| over all values of the plan."""
+    a, b = plan.pairs(P.shape[0])
+    rho = P[a] * P[b] + Q[a] * Q[b]
+    nv, ng = gr.shape
+    w = grw.ravel()
+    mo, mi = O.yk_partial_matrices(ng)
+    worst = 0.0
+    for k in range(kmax + 1):
+        if plan.m[k] == 0:
+            continue
+        p, q = plan.quartets(k)
+        ref = O.rk_quartets(rho, gr, grw, nv, ng, p, q, np.full(len(p), k))
+        got = out[plan.offsets[k]:plan.offsets[k] + len(p)]
+        ys = {j: np.abs(O.yk_flat(rho[j], gr.ravel(), w, w.size, k, nv, ng, mo, mi))
+              for j in set(q.tolist())}
+        scale = np.array([np.sum(np.abs(w * rho[i]) * ys[j]) for i, j in zip(p, q)])
+        worst = max(worst, float(np.max(np.abs(got - ref) / scale)))
+    return worst
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_batch_bitwise_equals_alone(problems, precision):
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor, run_batch
+    plans, ins, alone = [], [], []
+    for gr, grw, P, Q, ls, kmax in problems:
+        pl = SlaterTensor(gr, grw, ls, kmax, precision=precision)
+        Pd, Qd = dv.f64(P), dv.f64(Q)
+        plans.append(pl)
+        ins.append((Pd, Qd))
+        alone.append(dv.host(pl.run(Pd, Qd)))
+    outs = run_batch(plans, [a for a, _ in ins], [b for _, b in ins])
+    for i, (o, ref) in enumerate(zip(outs, alone)):
+        got = dv.host(o)
+        print(f"{precision} plan {i} {SHAPES[i]}: {got.size} values")
+        assert got.shape == ref.shape
+        assert np.array_equal(got, ref), f"plan {i} differs in the batch"
+    # the mix in another order and with a repeated plan
+    order = [3, 0, 2, 1, 0]
+    outs2 = run_batch([plans[i] for i in order], [ins[i][0] for i in order],
+                      [ins[i][1] for i in order])
+    for j, i in enumerate(order):
+        assert np.array_equal(dv.host(outs2[j]), alone[i])
+    if precision == "fp64":
+        gr, grw, P, Q, ls, kmax = problems[1]
+        assert _oracle_scaled_err(plans[1], alone[1], gr, grw, P, Q, kmax) < 1e-13
+    for pl in plans:
+        pl.close()
+
+
+def test_batch_more_plans_than_one_launch():
+    """35 plans (two launch chunks of at most 32)."""
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor, run_batch
+    plans, Ps, Qs, alone = [], [], [], []
+    for i in range(35):
+        gr, grw, P, Q, ls = _problem(12 + i, 8, 2 + i % 4, 100 + i)
+        pl = SlaterTensor(gr, grw, ls, 1 + i % 3, precision="fp64")
+        plans.append(pl)
+        Ps.append(dv.f64(P))
+        Qs.append(dv.f64(Q))
+        alone.append(dv.host(pl.run(Ps[-1], Qs[-1])))
+    outs = run_batch(plans, Ps, Qs)
+    for o, ref in zip(outs, alone):
+        assert np.array_equal(dv.host(o), ref)
+    for pl in plans:
+        pl.close()
+
+
+def test_graph_replay_matches_eager(problems):
+    """A batch captured into a HIP graph and replayed three times, with eager
+    work between the replays, reproduces the eager result each time."""
+    import torch
+
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor, run_batch
+    sel = problems[:3]
+    plans = [SlaterTensor(gr, grw, ls, kmax, precision="split")
+             for gr, grw, P, Q, ls, kmax in sel]
+    Ps = [dv.f64(p[2]) for p in sel]
+    Qs = [dv.f64(p[3]) for p in sel]
+    eager = [dv.host(o) for o in run_batch(plans, Ps, Qs)]
+    outs = [dv.empty((pl.size,)) for pl in plans]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        run_batch(plans, Ps, Qs, outs)  # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        run_batch(plans, Ps, Qs, outs)
+    for rep in range(3):
+        for o in outs:
+            o.fill_(0.0)
+        g.replay()
+        torch.cuda.synchronize()
+        for i, (o, ref) in enumerate(zip(outs, eager)):
+            assert np.array_equal(dv.host(o), ref), f"replay {rep} plan {i}"
+        # eager work between replays
+        run_batch(plans[:1], Ps[:1], Qs[:1])
+        torch.cuda.synchronize()
+    del g
+    torch.cuda.synchronize()
+    for pl in plans:
+        pl.close()
+
+
+def test_schedule_tuner(problems):
+    """The tuner keeps a valid schedule, leaves that schedule's result in out
+    (a later run reproduces it bitwise), and every schedule agrees with the
+    default one to FP64 rounding."""
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    gr, grw, P, Q, ls, kmax = problems[0]
+    for precision in ("fp64", "split"):
+        plan = SlaterTensor(gr, grw, ls, kmax, precision=precision)
+        Pd, Qd = dv.f64(P), dv.f64(Q)
+        ref = dv.host(plan.run(Pd, Qd))
+        out = dv.empty((plan.size,))
+        order, split, ms = plan.tune(Pd, Qd, out, reps=2)
+        print(f"{precision}: tuned order={order} split={split} {ms * 1e3:.1f} us")
+        assert order in SlaterTensor.ORDERS and split >= 1 and ms > 0
+        tuned = dv.host(out)
+        assert np.array_equal(dv.host(plan.run(Pd, Qd)), tuned)
+        scale = np.max(np.abs(ref))
+        assert np.max(np.abs(tuned - ref)) <= 1e-13 * scale
+        for o in SlaterTensor.ORDERS:
+            plan.set_schedule(o, 1)
+            got = dv.host(plan.run(Pd, Qd))
+            assert np.max(np.abs(got - ref)) <= 1e-13 * scale
+        with pytest.raises(ValueError):
+            plan.set_schedule("tile-major", 64)
+        plan.close()
