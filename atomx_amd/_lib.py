@@ -46,7 +46,11 @@ SIGNATURES = {
                            _vp, _vp, _vp, _vp, _i64, _vp]),
     "ax_rk_dot": (_int, [_vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _vp, _vp]),
     "ax_bspline_eval": (_int, [_vp, _i64, _int, _vp, _int, _int, _int, _int, _vp, _i64, _vp]),
-    "ax_assemble_integral": (_int, [_int, _int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp]),
+    "ax_assemble_integral": (_int, [_int, _int, _int, _int, _int, _vp, _vp, _vp, _int, _vp,
+                                    _vp]),
+    "ax_symmetrize": (_int, [_vp, _int, _int, _vp]),
+    "ax_exchange_matrix_yk_batch": (_int, [_vp, _i64, _vp, _vp, _vp, _vp, _int, _int, _int,
+                                           _int, _int, _int, _vp, _vp, _vp, _vp]),
     "ax_exchange_matrix_yk": (_int, [_vp, _vp, _vp, _vp, _int, _int, _int, _int, _int, _vp,
                                      _vp]),
     "ax_slater_rk_direct": (_int, [_vp, _vp, _vp, _vp, _int, _int, _int, _int, _vp, _vp, _vp,

@@ -190,8 +190,8 @@ class ConvolCache:
         if g is None:
             ks = self.order(c)
             g = dv.empty((ks, self.ns))
-            call("ax_cell_global_moments", dv.ptr(self._moments(c, k)), self.nv, ks, self.ns,
-                 dv.ptr(g), _s())
+            mom = self._moments(c, k)
+            call("ax_cell_global_moments", dv.ptr(mom), self.nv, ks, self.ns, dv.ptr(g), _s())
             self._cache[key] = g
         return g
 
@@ -435,7 +435,8 @@ def compute_direct_potential_cell(orb_P, orb_Q, bsp_data, k, cache=None):
             v = cache.case1(int(k), a + b + a + b, dens[b], force="bank")
             banded = v if banded is None else axpy(banded, v)
         full = dv.zeros((1, ns, ns))
-        call("ax_banded_to_full_add", dv.ptr(full), ns * ns, dv.ptr(banded.contiguous()),
+        banded = banded.contiguous()
+        call("ax_banded_to_full_add", dv.ptr(full), ns * ns, dv.ptr(banded),
              ns * cache.order(a), 1, ns, cache.order(a), 1.0, _s())
         blocks.append(dv.host(full)[0])
     return tuple(blocks)

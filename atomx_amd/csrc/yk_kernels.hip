@@ -527,16 +527,19 @@ __global__ void bspline_eval_kernel(const double *__restrict__ coeffs, int64_t l
 // at most ks cells, so the node-wise cumulative integrals are constant
 // outside that support; the column is assembled over all cells.
 // ---------------------------------------------------------------------------
-// <B1_gi | w | B2_gj> (numerics/bspline/integrals.py:18-56), one thread per
-// matrix element, gathering the cells that hold both B-splines in ascending
-// order (deterministic, no atomics).
-__global__ void assemble_integral_kernel(int nv, int ng, int ks1, int ks2, int mm,
+// <B1_gi | w | B2_gj> (numerics/bspline/integrals.py:18-56) for n_w weight
+// rows, one thread per matrix element, gathering the cells that hold both
+// B-splines in ascending order (deterministic, no atomics).
+__global__ void assemble_integral_kernel(int nv, int ng, int ks1, int ks2, int mm, int n_w,
                                          const double *__restrict__ b1,
                                          const double *__restrict__ b2,
-                                         const double *__restrict__ wt, double *__restrict__ mat) {
-    const int64_t total = (int64_t)mm * mm;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
+                                         const double *__restrict__ wts,
+                                         double *__restrict__ mats) {
+    const int64_t per = (int64_t)mm * mm, total = per * n_w;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t wi = u / per, t = u - wi * per;
+        const double *wt = wts + wi * (int64_t)nv * ng;
         const int gi = static_cast<int>(t / mm), gj = static_cast<int>(t - (int64_t)gi * mm);
         int lo = gi - ks1 + 1;
         if (gj - ks2 + 1 > lo) lo = gj - ks2 + 1;
@@ -553,50 +556,135 @@ __global__ void assemble_integral_kernel(int nv, int ng, int ks1, int ks2, int m
             }
             acc += s;
         }
-        mat[t] = acc;
+        mats[u] = acc;
     }
 }
 
-__global__ __launch_bounds__(kThreads) void exchange_yk_kernel(
-    const double *__restrict__ fb, const double *__restrict__ bsp,
-    const double *__restrict__ r, const double *__restrict__ w, int nv, int ng, int ks, int ns,
-    const double *__restrict__ rk, const double *__restrict__ rkm, double *__restrict__ K) {
-    extern __shared__ double sm[];
-    const int64_t n = (int64_t)nv * ng;
-    for (int nidx = blockIdx.x; nidx < ns; nidx += gridDim.x) {
+// M = (M + M^T) / 2 in place for n_mat square matrices (one thread per i < j).
+__global__ void symmetrize_kernel(double *__restrict__ mats, int mm, int n_mat) {
+    const int64_t per = (int64_t)mm * mm, total = per * n_mat;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = u % per;
+        const int i = static_cast<int>(t / mm), j = static_cast<int>(t - (int64_t)i * mm);
+        if (j <= i) continue;
+        double *M = mats + (u - t);
+        const double v = 0.5 * (M[(int64_t)i * mm + j] + M[(int64_t)j * mm + i]);
+        M[(int64_t)i * mm + j] = v;
+        M[(int64_t)j * mm + i] = v;
+    }
+}
+
+// Fast form of the exchange matrix, in two passes per (orbital, rank) item.
+// Y^k of the density f B_n (supported on the cells of B_n) is, outside that
+// support, r^-(k+1) T_out[n] above it and r^k T_in[n] below it.  So when the
+// supports of B_m and B_n are disjoint, K[m][n] = A^-[m] T_out[n] (m above)
+// or A^+[m] T_in[n] (m below) with the moments A^-/A^+ of w f B_m r^-(k+1) /
+// w f B_m r^k; only the band |m - n| < ks needs the in-support profiles of
+// the scan, evaluated exactly as the reference does (iv ascending, then the
+// Gauss nodes).
+//
+// Pass 1, one thread per (item, n): in-support inclusive prefix / suffix
+// (reference order), the totals and the two moments of B_n.
+__global__ void exchange_profiles_kernel(const double *__restrict__ fb, int64_t ld_fb,
+                                         const double *__restrict__ bsp,
+                                         const double *__restrict__ w,
+                                         const double *__restrict__ rk_tab,
+                                         const double *__restrict__ rkm_tab, int kmax_tab,
+                                         int nv, int ng, int ks, int ns, int n_items,
+                                         const int32_t *__restrict__ item_b,
+                                         const int32_t *__restrict__ item_k,
+                                         double *__restrict__ prof) {
+    const int L = ks * ng;
+    const int64_t n_all = (int64_t)nv * ng;
+    const int64_t total = (int64_t)n_items * ns;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int it = static_cast<int>(u / ns), nidx = static_cast<int>(u - (int64_t)it * ns);
+        const int64_t kb = item_b ? item_b[it] : it, kk = item_k ? item_k[it] : kmax_tab;
+        const double *f = fb + kb * ld_fb;
+        const double *rk = rk_tab + kk * n_all;
+        const double *rkm = rkm_tab + kk * n_all;
+        double *pr = prof + u * (2 * L + 4);  // [L] out-prefix, [L] in-suffix, Tout, Tin, A-, A+
         int lo = nidx - ks + 1;
         if (lo < 0) lo = 0;
         int hi = nidx + 1;
         if (hi > nv) hi = nv;
-        const int64_t x_lo = (int64_t)lo * ng, x_hi = (int64_t)hi * ng;
-        const int len = hi > lo ? static_cast<int>(x_hi - x_lo) : 0;
-        double *iout = sm;        // [len] inclusive prefix inside the support
-        double *iin = sm + len;   // [len] inclusive suffix inside the support
-        __shared__ double tot_out_s, tot_in_s;
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int t = 0; t < len; ++t) {
-                const int64_t x = x_lo + t;
-                const int iv = static_cast<int>(x / ng);
-                const double rho = fb[x] * bsp[x * ks + (nidx - iv)];
-                s += w[x] * rho * rk[x];
-                iout[t] = s;
-            }
-            tot_out_s = s;
-            s = 0.0;
-            for (int t = len - 1; t >= 0; --t) {
-                const int64_t x = x_lo + t;
-                const int iv = static_cast<int>(x / ng);
-                const double rho = fb[x] * bsp[x * ks + (nidx - iv)];
-                s += w[x] * rho * rkm[x];
-                iin[t] = s;
-            }
-            tot_in_s = s;
+        const int64_t x_lo = (int64_t)lo * ng;
+        const int len = hi > lo ? (hi - lo) * ng : 0;
+        double s = 0.0, am = 0.0, ap = 0.0;
+        for (int t = 0; t < len; ++t) {
+            const int64_t x = x_lo + t;
+            const int iv = static_cast<int>(x / ng);
+            const double rho = f[x] * bsp[x * ks + (nidx - iv)];
+            s += w[x] * rho * rk[x];
+            pr[t] = s;
         }
-        __syncthreads();
-        const double tot_out = tot_out_s, tot_in = tot_in_s;
-        // K[m, nidx] = sum_{iv+s=m} sum_g (w fb)(x) B_s(x) Y(x), iv ascending
-        for (int m = threadIdx.x; m < ns; m += blockDim.x) {
+        pr[2 * L] = s;
+        s = 0.0;
+        for (int t = len - 1; t >= 0; --t) {
+            const int64_t x = x_lo + t;
+            const int iv = static_cast<int>(x / ng);
+            const double rho = f[x] * bsp[x * ks + (nidx - iv)];
+            s += w[x] * rho * rkm[x];
+            pr[L + t] = s;
+        }
+        pr[2 * L + 1] = s;
+        // moments of B_n as the left factor: cells iv ascending, nodes inside
+        for (int sidx = ks - 1; sidx >= 0; --sidx) {
+            const int iv = nidx - sidx;
+            if (iv < 0 || iv >= nv) continue;
+            double a1 = 0.0, a2 = 0.0;
+            for (int g = 0; g < ng; ++g) {
+                const int64_t x = (int64_t)iv * ng + g;
+                const double v = (w[x] * f[x]) * bsp[x * ks + sidx];
+                a1 += v * rkm[x];
+                a2 += v * rk[x];
+            }
+            am += a1;
+            ap += a2;
+        }
+        pr[2 * L + 2] = am;
+        pr[2 * L + 3] = ap;
+    }
+}
+
+// Pass 2, one thread per (item, m, n).
+__global__ void exchange_assemble_kernel(const double *__restrict__ fb, int64_t ld_fb,
+                                         const double *__restrict__ bsp,
+                                         const double *__restrict__ w,
+                                         const double *__restrict__ rk_tab,
+                                         const double *__restrict__ rkm_tab, int kmax_tab,
+                                         int nv, int ng, int ks, int ns, int n_items,
+                                         const int32_t *__restrict__ item_b,
+                                         const int32_t *__restrict__ item_k,
+                                         const double *__restrict__ prof,
+                                         double *__restrict__ K) {
+    const int L = ks * ng;
+    const int64_t n_all = (int64_t)nv * ng;
+    const int64_t per = (int64_t)ns * ns, total = per * n_items;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int it = static_cast<int>(u / per);
+        const int64_t e = u - (int64_t)it * per;
+        const int m = static_cast<int>(e / ns), nidx = static_cast<int>(e - (int64_t)m * ns);
+        const double *pn = prof + ((int64_t)it * ns + nidx) * (2 * L + 4);
+        double val;
+        if (m - nidx >= ks) {
+            val = prof[((int64_t)it * ns + m) * (2 * L + 4) + 2 * L + 2] * pn[2 * L];
+        } else if (nidx - m >= ks) {
+            val = prof[((int64_t)it * ns + m) * (2 * L + 4) + 2 * L + 3] * pn[2 * L + 1];
+        } else {
+            const int64_t kb = item_b ? item_b[it] : it, kk = item_k ? item_k[it] : kmax_tab;
+            const double *f = fb + kb * ld_fb;
+            const double *rk = rk_tab + kk * n_all;
+            const double *rkm = rkm_tab + kk * n_all;
+            int lo = nidx - ks + 1;
+            if (lo < 0) lo = 0;
+            int hi = nidx + 1;
+            if (hi > nv) hi = nv;
+            const int64_t x_lo = (int64_t)lo * ng, x_hi = (int64_t)hi * ng;
+            const double tot_out = pn[2 * L], tot_in = pn[2 * L + 1];
             double acc_m = 0.0;
             for (int sidx = ks - 1; sidx >= 0; --sidx) {
                 const int iv = m - sidx;
@@ -612,19 +700,20 @@ __global__ __launch_bounds__(kThreads) void exchange_yk_kernel(
                         io = tot_out;
                         ii = 0.0;
                     } else {
-                        io = iout[x - x_lo];
-                        ii = iin[x - x_lo];
+                        io = pn[x - x_lo];
+                        ii = pn[L + (x - x_lo)];
                     }
                     const double yk = rkm[x] * io + rk[x] * ii;
-                    acc += (w[x] * fb[x]) * bsp[x * ks + sidx] * yk;
+                    acc += (w[x] * f[x]) * bsp[x * ks + sidx] * yk;
                 }
                 acc_m += acc;
             }
-            K[(int64_t)m * ns + nidx] = acc_m;
+            val = acc_m;
         }
-        __syncthreads();
+        K[(int64_t)it * per + (int64_t)m * ns + nidx] = val;
     }
 }
+
 
 // ---------------------------------------------------------------------------
 // "direct" O(N^2) double quadrature with the coincident-cell cusp split
@@ -937,14 +1026,26 @@ ax_status ax_bspline_eval(const double *coeffs, int64_t ld_c, int n_orb, const d
 }
 
 ax_status ax_assemble_integral(int nv, int ng, int ks1, int ks2, int mm, const double *bsp1,
-                               const double *bsp2, const double *weights, double *mat,
+                               const double *bsp2, const double *weights, int n_w, double *mat,
                                void *stream) {
-    AX_REQUIRE(nv > 0 && ng > 0 && ks1 > 0 && ks2 > 0 && mm > 0,
+    AX_REQUIRE(nv > 0 && ng > 0 && ks1 > 0 && ks2 > 0 && mm > 0 && n_w >= 0,
                "ax_assemble_integral: bad sizes (nv=%d ng=%d ks=%d,%d mm=%d)", nv, ng, ks1, ks2, mm);
+    if (n_w == 0) return AX_OK;
     AX_REQUIRE(bsp1 && bsp2 && weights && mat, "ax_assemble_integral: null argument");
-    const int64_t total = (int64_t)mm * mm;
-    hipLaunchKernelGGL(assemble_integral_kernel, dim3(ax::grid_for(total, 256, 8192)), dim3(256), 0,
-                       ax::as_stream(stream), nv, ng, ks1, ks2, mm, bsp1, bsp2, weights, mat);
+    const int64_t total = (int64_t)mm * mm * n_w;
+    hipLaunchKernelGGL(assemble_integral_kernel, dim3(ax::grid_for(total, 256, 16384)), dim3(256),
+                       0, ax::as_stream(stream), nv, ng, ks1, ks2, mm, n_w, bsp1, bsp2, weights,
+                       mat);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_symmetrize(double *mat, int mm, int n_mat, void *stream) {
+    AX_REQUIRE(mm > 0 && n_mat >= 0, "ax_symmetrize: bad sizes");
+    if (n_mat == 0) return AX_OK;
+    const int64_t total = (int64_t)mm * mm * n_mat;
+    hipLaunchKernelGGL(symmetrize_kernel, dim3(ax::grid_for(total, 256, 16384)), dim3(256), 0,
+                       ax::as_stream(stream), mat, mm, n_mat);
     AX_CHECK_LAUNCH();
     return AX_OK;
 }
@@ -953,22 +1054,47 @@ ax_status ax_exchange_matrix_yk(const double *fb, const double *bsp, const doubl
                                 const double *w, int nv, int ng, int ks, int ns, int k,
                                 double *K, void *stream) {
     AX_REQUIRE(nv > 0 && ng > 0 && ks > 0 && ns > 0 && k >= 0, "ax_exchange_matrix_yk: bad sizes");
-    AX_REQUIRE((size_t)2 * ks * ng * sizeof(double) <= 32 * 1024,
-               "ax_exchange_matrix_yk: ks*n_gauss too large");
     const int64_t n = (int64_t)nv * ng;
-    // r^j and r^-(j+1) for j = 0..k; the kernel reads the rank-k rows
+    hipStream_t s = ax::as_stream(stream);
+    // r^j and r^-(j+1) for j = 0..k; the item reads the rank-k rows
     double *tab = nullptr;
-    AX_CHECK_HIP(hipMallocAsync(&tab, sizeof(double) * 2 * n * (k + 1), ax::as_stream(stream)));
+    AX_CHECK_HIP(hipMallocAsync(&tab, sizeof(double) * 2 * n * (k + 1), s));
     double *rk_all = tab, *rkm_all = tab + n * (k + 1);
-    hipLaunchKernelGGL(rpow_kernel, dim3(ax::grid_for(n * (k + 1), 256, 4096)), dim3(256), 0,
-                       ax::as_stream(stream), r, n, k, rk_all, rkm_all);
+    hipLaunchKernelGGL(rpow_kernel, dim3(ax::grid_for(n * (k + 1), 256, 4096)), dim3(256), 0, s, r,
+                       n, k, rk_all, rkm_all);
     AX_CHECK_LAUNCH();
-    const size_t shmem = (size_t)2 * ks * ng * sizeof(double);
-    hipLaunchKernelGGL(exchange_yk_kernel, dim3(ns < 1024 ? ns : 1024), dim3(kThreads), shmem,
-                       ax::as_stream(stream), fb, bsp, r, w, nv, ng, ks, ns, rk_all + n * k,
-                       rkm_all + n * k, K);
+    ax_status st = ax_exchange_matrix_yk_batch(fb, n, bsp, w, rk_all, rkm_all, k, nv, ng, ks, ns, 1,
+                                               nullptr, nullptr, K, stream);
+    AX_CHECK_HIP(hipFreeAsync(tab, s));
+    return st;
+}
+
+ax_status ax_exchange_matrix_yk_batch(const double *fb, int64_t ld_fb, const double *bsp,
+                                      const double *w, const double *rk_tab,
+                                      const double *rkm_tab, int kmax_tab, int nv, int ng, int ks,
+                                      int ns, int n_items, const int32_t *item_b,
+                                      const int32_t *item_k, double *K, void *stream) {
+    AX_REQUIRE(nv > 0 && ng > 0 && ks > 0 && ns > 0 && kmax_tab >= 0 && n_items >= 0 &&
+                   ld_fb >= (int64_t)nv * ng,
+               "ax_exchange_matrix_yk_batch: bad sizes");
+    AX_REQUIRE(ns <= nv + ks - 1, "ax_exchange_matrix_yk_batch: ns=%d exceeds nv+ks-1", ns);
+    if (n_items == 0) return AX_OK;
+    hipStream_t s = ax::as_stream(stream);
+    const int L = ks * ng;
+    double *prof = nullptr;
+    const size_t bytes = sizeof(double) * (size_t)n_items * ns * (2 * L + 4);
+    AX_CHECK_HIP(hipMallocAsync(&prof, bytes, s));
+    const int64_t t1 = (int64_t)n_items * ns;
+    hipLaunchKernelGGL(exchange_profiles_kernel, dim3(ax::grid_for(t1, 64, 65535)), dim3(64), 0, s,
+                       fb, ld_fb, bsp, w, rk_tab, rkm_tab, kmax_tab, nv, ng, ks, ns, n_items,
+                       item_b, item_k, prof);
     AX_CHECK_LAUNCH();
-    AX_CHECK_HIP(hipFreeAsync(tab, ax::as_stream(stream)));
+    const int64_t t2 = (int64_t)n_items * ns * ns;
+    hipLaunchKernelGGL(exchange_assemble_kernel, dim3(ax::grid_for(t2, 256, 65535)), dim3(256), 0,
+                       s, fb, ld_fb, bsp, w, rk_tab, rkm_tab, kmax_tab, nv, ng, ks, ns, n_items,
+                       item_b, item_k, prof, K);
+    AX_CHECK_LAUNCH();
+    AX_CHECK_HIP(hipFreeAsync(prof, s));
     return AX_OK;
 }
 

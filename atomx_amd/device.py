@@ -28,13 +28,17 @@ def device():
 
 
 def to_dev(a, dtype=None):
-    """Host array (or tensor) -> contiguous device tensor."""
+    """Host array (or tensor) -> contiguous device tensor.
+
+    Host data is staged through pinned memory and copied asynchronously on
+    the current stream, so an upload does not wait for work already queued
+    there (torch keeps the staging block until the copy has run)."""
     t = torch()
     if isinstance(a, t.Tensor):
         out = a.to(device=device(), dtype=dtype or a.dtype)
         return out.contiguous()
     arr = np.ascontiguousarray(a, dtype=dtype if dtype is not None else None)
-    return t.from_numpy(arr).to(device())
+    return t.from_numpy(arr).pin_memory().to(device(), non_blocking=True)
 
 
 def f64(a):

@@ -1,0 +1,603 @@
+"""Request server for Slater-integral and Fock-build requests on one GPU.
+
+Continuous batching.  A scheduler thread owns the device.  Requests wait in a
+FIFO queue; the head is admitted as soon as an in-flight slot is free and its
+HBM footprint (estimated before anything is allocated) fits the budget beside
+everything in flight and resident.  Admitted Slater-integral requests of one
+precision mode are launched together (one batched launch set for up to 16
+plans, ``ax_rk_tensor_run_batch``) on one of a few streams, so one batch runs
+while the next is admitted and launched; each Fock-build request takes a
+stream slot of its own.  Results return through futures; latency is measured
+from ``submit`` to the moment the result is ready on the host.
+
+Residency.  A request that names a ``state`` keeps its plan (grid tables, pair
+densities, Y^k workspaces) and its input and output buffers on the device
+after it completes, so the next request of that state (the next SCF
+iteration of the same atom) uploads only its orbitals.  Idle resident states
+are evicted least recently used when admission needs their memory.
+
+Determinism.  A request's plan, tiles, K splits and summation order do not
+depend on what else is in flight, so its result is bitwise the same alone and
+in any mix.
+
+The scheduler is independent of the device: it drives an executor object
+(``GpuExecutor`` here) through footprint / reserve / launch / done / finish /
+evict calls, which is what the CPU tests exercise with a stand-in executor.
+"""
+
+from __future__ import annotations
+
+import collections
+import itertools
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+from typing import Any, Hashable, Optional, Sequence
+
+import numpy as np
+
+__all__ = ["SlaterRequest", "FockRequest", "SlaterResult", "FockResult", "Scheduler",
+           "GpuExecutor", "SlaterServer", "percentile"]
+
+
+# ---------------------------------------------------------------------------
+# Requests and results
+# ---------------------------------------------------------------------------
+@dataclass
+class SlaterRequest:
+    """All R^k(ab;cd), k = 0..kmax, over the valid pair-pairs of the orbitals
+    sampled on a Gauss grid (the tensor of ``atomx_amd.tensor``)."""
+    gr: np.ndarray                 # (nv, ng) nodes
+    grw: np.ndarray                # (nv, ng) weights
+    P: np.ndarray                  # (n_orb, nv*ng) large-component samples
+    Q: Optional[np.ndarray]        # small-component samples, or None
+    ls: Sequence[int]
+    kmax: int
+    precision: str = "split"
+    state: Optional[Hashable] = None
+
+    kind = "slater"
+
+
+@dataclass
+class FockRequest:
+    """Y^k-form Fock pieces of every target of one atom (``FockYk.build``)."""
+    bsp_data: dict
+    P: np.ndarray                  # (n_orb, ns) coefficients
+    Q: np.ndarray
+    occ: Sequence[float]
+    exchange_coef: np.ndarray      # (n_orb, n_orb, K): target, source, rank
+    targets: Optional[Sequence[int]] = None
+    state: Optional[Hashable] = None
+
+    kind = "fock"
+
+
+@dataclass
+class SlaterResult:
+    values: np.ndarray             # packed upper triangles, rank by rank
+    offsets: list
+    m: list
+    latency_s: float = 0.0
+
+    def matrix(self, k):
+        m = self.m[k]
+        vals = self.values[self.offsets[k]:self.offsets[k] + m * (m + 1) // 2]
+        M = np.zeros((m, m))
+        i, j = np.triu_indices(m)
+        M[i, j] = vals
+        M[j, i] = vals
+        return M
+
+
+@dataclass
+class FockResult:
+    blocks: dict                   # VJ_PP, VJ_QQ, K_PP, K_QQ: (n_targets, ns, ns)
+    latency_s: float = 0.0
+
+
+def percentile(values, q):
+    """Nearest-rank percentile (q in [0, 100]) of a non-empty sequence."""
+    v = np.sort(np.asarray(values, dtype=float))
+    if v.size == 0:
+        return float("nan")
+    rank = int(np.ceil(q / 100.0 * v.size))
+    return float(v[min(max(rank, 1), v.size) - 1])
+
+
+# ---------------------------------------------------------------------------
+# Scheduler (device independent)
+# ---------------------------------------------------------------------------
+@dataclass
+class _Ticket:
+    req: Any
+    future: Future
+    t_submit: float
+    seq: int
+    nbytes: int = 0
+
+
+@dataclass
+class _Job:
+    handle: Any
+    tickets: list
+    nbytes: int
+    t_launch: float = field(default_factory=time.perf_counter)
+
+
+class Scheduler:
+    """FIFO admission by footprint, batched launches, in-flight slots.
+
+    executor interface:
+      ready(req) -> bool         False while the request's resident state is in flight
+      footprint(req) -> int      device bytes the request holds while in flight
+      reserve(req)               pin the request's resident state (if any)
+      resident_bytes() -> int    bytes held by idle resident states
+      evict(nbytes) -> int       drop idle resident states (LRU), bytes freed
+      batch_key(req)             requests with equal keys may share a launch
+      launch(reqs) -> handle     start the work of a batch (asynchronous)
+      done(handle) -> bool
+      finish(handle) -> list     results (or exceptions) in request order
+    """
+
+    def __init__(self, executor, budget_bytes, max_inflight=4, max_batch=16, poll_s=2e-4):
+        self.ex = executor
+        self.budget = int(budget_bytes)
+        self.max_inflight = int(max_inflight)
+        self.max_batch = int(max_batch)
+        self.poll_s = float(poll_s)
+        self._cv = threading.Condition()
+        self._arrivals = collections.deque()
+        self._pending = collections.deque()
+        self._inflight = []
+        self._inflight_bytes = 0
+        self._seq = itertools.count()
+        self._stop = False
+        self.latencies = []
+        self.completed = 0
+        self.failed = 0
+        self.peak_bytes = 0
+        self.peak_inflight = 0
+        self.launches = 0
+        self.error = None
+        self._thread = threading.Thread(target=self._run, name="slater-scheduler", daemon=True)
+        self._thread.start()
+
+    # -- client side -----------------------------------------------------------
+    def submit(self, req) -> Future:
+        fut = Future()
+        t = _Ticket(req, fut, time.perf_counter(), next(self._seq))
+        with self._cv:
+            if self._stop:
+                raise RuntimeError("scheduler is closed")
+            self._arrivals.append(t)
+            self._cv.notify()
+        return fut
+
+    def close(self, wait=True):
+        with self._cv:
+            self._stop = True
+            self._cv.notify()
+        if wait:
+            self._thread.join()
+
+    def reset_stats(self):
+        self.latencies = []
+        self.completed = self.failed = self.launches = 0
+        self.peak_bytes = self.peak_inflight = 0
+
+    def stats(self):
+        lat = self.latencies
+        return {"completed": self.completed, "failed": self.failed,
+                "p50_ms": percentile(lat, 50) * 1e3 if lat else None,
+                "p99_ms": percentile(lat, 99) * 1e3 if lat else None,
+                "peak_bytes": self.peak_bytes, "peak_inflight": self.peak_inflight,
+                "launches": self.launches, "budget_bytes": self.budget}
+
+    # -- scheduler thread --------------------------------------------------------
+    def _run(self):
+        try:
+            while True:
+                with self._cv:
+                    while self._arrivals:
+                        self._pending.append(self._arrivals.popleft())
+                    if self._stop and not self._pending and not self._inflight:
+                        return
+                progressed = self._complete()
+                progressed |= self._admit_and_launch()
+                if not progressed:
+                    with self._cv:
+                        if not self._arrivals:
+                            busy = bool(self._inflight or self._pending)
+                            if busy or not self._stop:
+                                self._cv.wait(self.poll_s if busy else 0.05)
+        except BaseException as exc:  # fail every outstanding request loudly
+            self.error = exc
+            with self._cv:
+                rest = list(self._arrivals) + list(self._pending)
+                self._arrivals.clear()
+                self._pending.clear()
+            for job in self._inflight:
+                rest.extend(job.tickets)
+            self._inflight = []
+            for t in rest:
+                if not t.future.done():
+                    t.future.set_exception(exc)
+            raise
+
+    def _complete(self):
+        progressed = False
+        still = []
+        for job in self._inflight:
+            if not self.ex.done(job.handle):
+                still.append(job)
+                continue
+            progressed = True
+            results = self.ex.finish(job.handle)
+            now = time.perf_counter()
+            self._inflight_bytes -= job.nbytes
+            for t, res in zip(job.tickets, results):
+                if isinstance(res, BaseException):
+                    self.failed += 1
+                    t.future.set_exception(res)
+                    continue
+                lat = now - t.t_submit
+                if hasattr(res, "latency_s"):
+                    res.latency_s = lat
+                self.latencies.append(lat)
+                self.completed += 1
+                t.future.set_result(res)
+        self._inflight = still
+        return progressed
+
+    def _admit_and_launch(self):
+        if len(self._inflight) >= self.max_inflight:
+            return False
+        admitted = []
+        nbytes = 0
+        while self._pending:
+            t = self._pending[0]
+            if not self.ex.ready(t.req):
+                break  # its resident state is still in flight (FIFO: no overtaking)
+            need = self.ex.footprint(t.req)
+            if need > self.budget:
+                self._pending.popleft()
+                self.failed += 1
+                t.future.set_exception(MemoryError(
+                    f"request needs {need} device bytes, budget is {self.budget}"))
+                continue
+            used = self._inflight_bytes + nbytes + self.ex.resident_bytes()
+            if used + need > self.budget:
+                self.ex.evict(used + need - self.budget)
+                used = self._inflight_bytes + nbytes + self.ex.resident_bytes()
+                if used + need > self.budget:
+                    break  # wait for in-flight work to free memory (FIFO: no overtaking)
+            if admitted and (len(admitted) >= self.max_batch
+                             or self.ex.batch_key(t.req) != self.ex.batch_key(admitted[0].req)):
+                break
+            self._pending.popleft()
+            self.ex.reserve(t.req)
+            t.nbytes = need
+            nbytes += need
+            admitted.append(t)
+        if not admitted:
+            return False
+        handle = self.ex.launch([t.req for t in admitted])
+        self._inflight.append(_Job(handle, admitted, nbytes))
+        self._inflight_bytes += nbytes
+        self.launches += 1
+        self.peak_inflight = max(self.peak_inflight, len(self._inflight))
+        self.peak_bytes = max(self.peak_bytes, self._inflight_bytes + self.ex.resident_bytes())
+        return True
+
+
+# ---------------------------------------------------------------------------
+# GPU executor
+# ---------------------------------------------------------------------------
+def _grid_key(gr, grw):
+    import hashlib
+    h = hashlib.blake2b(digest_size=16)
+    for a in (gr, grw):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        h.update(np.asarray(a.shape, dtype=np.int64).tobytes())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+class _State:
+    """Device objects of one resident state (or of one transient request)."""
+
+    def __init__(self, key, shape_key, nbytes):
+        self.key = key
+        self.shape_key = shape_key
+        self.nbytes = nbytes
+        self.busy = False
+        self.plan = None
+        self.fock = None
+        self.P = self.Q = self.out = None
+        self.host_out = None
+
+    def close(self):
+        if self.plan is not None:
+            self.plan.close()
+        self.plan = self.fock = None
+        self.P = self.Q = self.out = self.host_out = None
+
+
+class GpuExecutor:
+    """Runs Slater-integral and Fock-build requests on the current GPU."""
+
+    def __init__(self, n_streams=4):
+        import torch
+
+        from . import device as dv
+        from ._lib import PRECISIONS, load
+        dv.require_gpu()
+        self._torch = torch
+        self._dv = dv
+        self._lib = load()
+        self._prec = PRECISIONS
+        self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        self._free_streams = list(range(n_streams))
+        self.states = collections.OrderedDict()  # resident, LRU order
+        self.plans_created = 0
+        self._fock_cache = {}
+
+    # -- footprints ----------------------------------------------------------------
+    def _info(self, req):
+        """(shape key, device bytes of a fresh state), computed once per request."""
+        info = getattr(req, "_ax_info", None)
+        if info is None:
+            if req.kind == "slater":
+                info = (self._slater_shape(req), self._slater_bytes(req))
+            else:
+                info = (None, self._fock_bytes(req))
+            req._ax_info = info
+        return info
+
+    def _slater_shape(self, req):
+        """Identity of a resident state's plan (None for a one-off request)."""
+        if req.state is None:
+            return None
+        nv, ng = np.shape(req.gr)
+        ls = tuple(int(x) for x in req.ls)
+        return ("slater", _grid_key(req.gr, req.grw), nv, ng, ls, int(req.kmax), req.precision,
+                req.Q is not None)
+
+    def _slater_bytes(self, req):
+        import ctypes
+        nv, ng = np.shape(req.gr)
+        ls = np.ascontiguousarray(req.ls, dtype=np.int32)
+        plan = int(self._lib.ax_rk_tensor_estimate(
+            nv, ng, ls.size, ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), int(req.kmax),
+            self._prec[req.precision]))
+        if plan < 0:
+            raise ValueError("invalid Slater-integral request")
+        n_rk = _count_rk(ls, int(req.kmax))
+        return plan + 8 * (2 * ls.size * nv * ng + n_rk)
+
+    def _fock_bytes(self, req):
+        from .fock_yk import FockYk
+        n_orb = np.shape(req.P)[0]
+        nt = n_orb if req.targets is None else len(req.targets)
+        bd = req.bsp_data
+        ns, nv, ng = int(bd["ns"]), int(bd["nv"]), np.shape(bd["gr"])[1]
+        return FockYk.footprint(n_orb, nt, ns, nv * ng, nv, ng, int(bd["ks"]))
+
+    def _key(self, req):
+        return (req.kind, req.state) if req.state is not None else None
+
+    def footprint(self, req):
+        key = self._key(req)
+        if key is not None and key in self.states:
+            return self.states[key].nbytes
+        return self._info(req)[1]
+
+    def ready(self, req):
+        key = self._key(req)
+        return key is None or key not in self.states or not self.states[key].busy
+
+    def reserve(self, req):
+        key = self._key(req)
+        if key is None:
+            return
+        if key not in self.states:  # placeholder until the launch builds it
+            shape, nbytes = self._info(req)
+            self.states[key] = _State(key, shape, nbytes)
+        self.states[key].busy = True
+        self.states.move_to_end(key)
+
+    def resident_bytes(self):
+        return sum(s.nbytes for s in self.states.values() if not s.busy)
+
+    def evict(self, nbytes):
+        freed = 0
+        for key in list(self.states):
+            if freed >= nbytes:
+                break
+            st = self.states[key]
+            if st.busy:
+                continue
+            freed += st.nbytes
+            st.close()
+            del self.states[key]
+        return freed
+
+    def batch_key(self, req):
+        return ("slater", req.precision) if req.kind == "slater" else ("fock", id(req))
+
+    # -- launches --------------------------------------------------------------------
+    def _state_for(self, req):
+        key = self._key(req)
+        shape, nbytes = self._info(req)
+        if key is not None and key in self.states:
+            st = self.states[key]
+            if st.shape_key != shape:
+                raise ValueError(f"state {req.state!r} was created with another shape")
+            st.busy = True
+            return st
+        st = _State(key, shape, nbytes)
+        st.busy = True
+        if key is not None:
+            self.states[key] = st
+        return st
+
+    def launch(self, reqs):
+        torch, dv = self._torch, self._dv
+        sid = self._free_streams.pop(0)
+        stream = self.streams[sid]
+        job = {"sid": sid, "reqs": reqs, "states": [], "errors": [None] * len(reqs),
+               "keep": [], "event": torch.cuda.Event()}
+        with torch.cuda.stream(stream):
+            if reqs[0].kind == "slater":
+                self._launch_slater(reqs, job, stream)
+            else:
+                self._launch_fock(reqs[0], job, stream)
+            job["event"].record(stream)
+        return job
+
+    def _upload(self, host, dev, keep):
+        torch = self._torch
+        src = torch.from_numpy(np.ascontiguousarray(host, dtype=np.float64)).pin_memory()
+        keep.append(src)
+        dev.copy_(src, non_blocking=True)
+
+    def _launch_slater(self, reqs, job, stream):
+        from .tensor import SlaterTensor, run_batch
+        torch, dv = self._torch, self._dv
+        plans, Ps, Qs, outs = [], [], [], []
+        for i, req in enumerate(reqs):
+            try:
+                st = self._state_for(req)
+                if st.plan is None:
+                    st.plan = SlaterTensor(req.gr, req.grw, req.ls, req.kmax, req.precision)
+                    self.plans_created += 1
+                    n_orb, n = st.plan.n_orb, st.plan.n
+                    st.P = dv.empty((n_orb, n))
+                    st.Q = dv.empty((n_orb, n)) if req.Q is not None else None
+                    st.out = dv.empty((st.plan.size,))
+                if st.host_out is None or st.key is None:
+                    st.host_out = torch.empty(st.plan.size, dtype=torch.float64, pin_memory=True)
+                if np.shape(req.P) != (st.plan.n_orb, st.plan.n):
+                    raise ValueError(f"orbital samples must be ({st.plan.n_orb}, {st.plan.n})")
+                self._upload(req.P, st.P, job["keep"])
+                if req.Q is not None:
+                    self._upload(req.Q, st.Q, job["keep"])
+                job["states"].append(st)
+                plans.append(st.plan)
+                Ps.append(st.P)
+                Qs.append(st.Q)
+                outs.append(st.out)
+            except Exception as exc:  # the request fails alone
+                job["errors"][i] = exc
+                job["states"].append(None)
+        if plans:
+            run_batch(plans, Ps, Qs, outs, stream=stream)
+            for st in job["states"]:
+                if st is not None:
+                    st.host_out.copy_(st.out, non_blocking=True)
+
+    def _launch_fock(self, req, job, stream):
+        from .fock_yk import FockYk
+        torch = self._torch
+        try:
+            st = self._state_for(req)
+            if st.fock is None:
+                key = _grid_key(req.bsp_data["gr"], req.bsp_data["grw"]) + str(
+                    req.bsp_data["bsp_p"].shape)
+                fk = self._fock_cache.get(key)
+                if fk is None:
+                    fk = FockYk(req.bsp_data)
+                    self._fock_cache[key] = fk
+                    if len(self._fock_cache) > 8:
+                        self._fock_cache.pop(next(iter(self._fock_cache)))
+                st.fock = fk
+            blocks = st.fock.build(req.P, req.Q, req.occ, req.exchange_coef, req.targets)
+            host = {}
+            for name, t in blocks.items():
+                h = torch.empty(t.shape, dtype=torch.float64, pin_memory=True)
+                h.copy_(t, non_blocking=True)
+                host[name] = h
+            job["keep"].append(blocks)
+            job["host"] = host
+            job["states"].append(st)
+        except Exception as exc:
+            job["errors"][0] = exc
+            job["states"].append(None)
+
+    def done(self, job):
+        return job["event"].query()
+
+    def finish(self, job):
+        job["event"].synchronize()
+        results = []
+        for req, st, err in zip(job["reqs"], job["states"], job["errors"]):
+            if err is not None:
+                results.append(err)
+                if st is not None:
+                    self._release(st)
+                continue
+            if req.kind == "slater":
+                vals = st.host_out.numpy()
+                if st.key is not None:  # a resident state reuses its staging buffer
+                    vals = vals.copy()
+                results.append(SlaterResult(vals, list(st.plan.offsets), list(st.plan.m)))
+            else:  # the pinned result buffers are handed over (the arrays keep them alive)
+                results.append(FockResult({k: v.numpy() for k, v in job["host"].items()}))
+            self._release(st)
+        self._free_streams.append(job["sid"])
+        return results
+
+    def _release(self, st):
+        st.busy = False
+        if st.key is None:
+            st.close()
+
+    def close(self):
+        self._torch.cuda.synchronize()
+        for st in self.states.values():
+            st.close()
+        self.states.clear()
+        self._fock_cache.clear()
+
+
+def _count_rk(ls, kmax):
+    from .workloads import count_rk
+    return int(count_rk(np.asarray(ls), int(kmax)))
+
+
+class SlaterServer:
+    """Scheduler + GPU executor.  ``budget_bytes`` defaults to 80 % of the
+    device memory free at start-up."""
+
+    def __init__(self, budget_bytes=None, max_inflight=4, max_batch=16):
+        import torch
+        self.executor = GpuExecutor(n_streams=max_inflight)
+        if budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info()
+            budget_bytes = int(0.8 * free)
+        self.scheduler = Scheduler(self.executor, budget_bytes, max_inflight=max_inflight,
+                                   max_batch=max_batch)
+
+    def submit(self, req) -> Future:
+        return self.scheduler.submit(req)
+
+    def reset_stats(self):
+        self.scheduler.reset_stats()
+
+    def stats(self):
+        s = self.scheduler.stats()
+        s["plans_created"] = self.executor.plans_created
+        s["resident_states"] = len(self.executor.states)
+        return s
+
+    def close(self):
+        self.scheduler.close()
+        self.executor.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

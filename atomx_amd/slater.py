@@ -28,8 +28,9 @@ __all__ = ["DEFAULT_METHOD", "AVAILABLE_METHODS", "slater_rk", "slater_rk_poisso
 
 def _dot_rows(D, Y, p, q):
     out = dv.empty((len(p),))
+    pd, qd = dv.i32(p), dv.i32(q)  # held until the launch is queued
     call("ax_rk_dot", dv.ptr(D), dv.ld(D), dv.ptr(Y), dv.ld(Y), D.shape[1], len(p),
-         dv.ptr(dv.i32(p)), dv.ptr(dv.i32(q)), dv.ptr(out), dv.stream_handle())
+         dv.ptr(pd), dv.ptr(qd), dv.ptr(out), dv.stream_handle())
     return out
 
 

@@ -73,16 +73,11 @@ def synthetic_orbitals(gr, grw, n_orb=40, seed=0):
 def valid_pairs(ls, k):
     """Pair indices (a <= b, lexicographic numbering) valid at rank k:
     |l_a - l_b| <= k <= l_a + l_b and l_a + l_b + k even."""
-    out = []
-    n = len(ls)
-    p = 0
-    for a in range(n):
-        for b in range(a, n):
-            la, lb = int(ls[a]), int(ls[b])
-            if abs(la - lb) <= k <= la + lb and (la + lb + k) % 2 == 0:
-                out.append(p)
-            p += 1
-    return np.asarray(out, dtype=np.int64)
+    ls = np.asarray(ls, dtype=np.int64)
+    a, b = np.triu_indices(ls.size)
+    la, lb = ls[a], ls[b]
+    ok = (np.abs(la - lb) <= k) & (k <= la + lb) & ((la + lb + k) % 2 == 0)
+    return np.nonzero(ok)[0].astype(np.int64)
 
 
 def pair_list(n_orb):
@@ -92,8 +87,81 @@ def pair_list(n_orb):
 
 def count_rk(ls, kmax):
     """Number of R^k values of the all-quartet workload (pair-pairs once)."""
+    ls = np.asarray(ls, dtype=np.int64)
+    a, b = np.triu_indices(ls.size)
+    la, lb = ls[a], ls[b]
+    lo, hi, par = np.abs(la - lb), la + lb, (la + lb) % 2
     total = 0
     for k in range(kmax + 1):
-        m = len(valid_pairs(ls, k))
+        m = int(np.count_nonzero((lo <= k) & (k <= hi) & (par == k % 2)))
         total += m * (m + 1) // 2
     return total
+
+
+# ---------------------------------------------------------------------------
+# Serving workload: a Poisson stream of Slater-integral and Fock-build requests
+# ---------------------------------------------------------------------------
+def _log_uniform(rng, lo, hi):
+    return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+
+_BSP_CACHE = {}
+
+
+def serving_bspline_data(nv, ks=8, ng=8):
+    """B-spline workspace on the exponential grid of ``nv`` intervals (cached)."""
+    from .bspline import prepare_bspline_data
+    key = (nv, ks, ng)
+    if key not in _BSP_CACHE:
+        ns = nv + ks - 1
+        t = make_exponential_knots(ns, ks, 1e-5, 60.0)
+        _BSP_CACHE[key] = prepare_bspline_data(t, ns, ks, n_gauss=ng)
+    return _BSP_CACHE[key]
+
+
+def exchange_coefficients(ls, kmax):
+    """Synthetic exchange weights c[t, b, k] = 1/(2k+1) where the triangle and
+    parity rules of (l_t, l_b, k) hold, k <= kmax."""
+    n = len(ls)
+    c = np.zeros((n, n, kmax + 1))
+    for t in range(n):
+        for b in range(n):
+            lt, lb = int(ls[t]), int(ls[b])
+            for k in range(abs(lt - lb), min(lt + lb, kmax) + 1):
+                if (lt + lb + k) % 2 == 0:
+                    c[t, b, k] = 1.0 / (2 * k + 1)
+    return c
+
+
+def serving_stream(n_requests, rate=200.0, seed=0, fock_fraction=0.2, precision="split"):
+    """[(arrival_s, request)]: seeded exponential inter-arrival times of mean
+    1/rate; per request n_orb ~ logU[5, 60], N ~ logU[1000, 6000] (nv =
+    round(N/8), 8 Gauss points), k_max ~ logU[2, 10] (rounded); a fraction
+    ``fock_fraction`` are Fock-build requests (random decaying B-spline
+    coefficients of order 8 on the same kind of grid), the rest all-R^k
+    requests on synthetic orbitals."""
+    from .serve import FockRequest, SlaterRequest
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n_requests):
+        t += float(rng.exponential(1.0 / rate))
+        n_orb = int(round(_log_uniform(rng, 5, 60)))
+        nv = int(round(_log_uniform(rng, 1000, 6000) / 8))
+        kmax = int(round(_log_uniform(rng, 2, 10)))
+        sub = int(rng.integers(0, 2 ** 31 - 1))
+        if rng.uniform() < fock_fraction:
+            bd = serving_bspline_data(nv)
+            r2 = np.random.default_rng(sub)
+            ns = bd["ns"]
+            decay = np.exp(-np.arange(ns) / max(ns / 8.0, 1.0))
+            P = r2.normal(size=(n_orb, ns)) * decay
+            Q = 0.01 * r2.normal(size=(n_orb, ns)) * decay
+            ls = [p[0] for p in draw_orbital_params(n_orb, sub)]
+            req = FockRequest(bd, P, Q, np.full(n_orb, 2.0), exchange_coefficients(ls, kmax))
+        else:
+            gr, grw = synthetic_grid(nv=nv, ng=8)
+            P, Q, ls = synthetic_orbitals(gr, grw, n_orb=n_orb, seed=sub)
+            req = SlaterRequest(gr, grw, P, Q, ls, kmax, precision=precision)
+        out.append((t, req))
+    return out

@@ -96,22 +96,39 @@ ax_status ax_bspline_eval(const double *coeffs, int64_t ld_c, int n_orb, const d
                           int nv, int ng, int ks, int ns, double *out, int64_t ld_out,
                           void *stream);
 
-/* numerics/bspline/integrals.py:18-56 _assemble_integral:
- * mat[gi][gj] = sum_iv sum_m weights[iv][m] bsp1[iv][m][gi-iv] bsp2[iv][m][gj-iv]
+/* numerics/bspline/integrals.py:18-56 _assemble_integral, for n_w weight
+ * rows at once:
+ * mat[w][gi][gj] = sum_iv sum_m weights[w][iv][m] bsp1[iv][m][gi-iv] bsp2[iv][m][gj-iv]
  * over the cells iv that hold both B-splines (iv ascending, as the
- * reference accumulates), for gi, gj < mm; mat (mm, mm) is overwritten and
- * holds zeros outside the band.  bsp1 (nv, ng, ks1), bsp2 (nv, ng, ks2),
- * weights (nv, ng).  With weights = grw * Y^0[rho] this is the direct
+ * reference accumulates), for gi, gj < mm; mat (n_w, mm, mm) is overwritten
+ * and holds zeros outside the band.  bsp1 (nv, ng, ks1), bsp2 (nv, ng, ks2),
+ * weights (n_w, nv, ng).  With weights = grw * Y^0[rho] this is the direct
  * potential of methods/dirac_fock.py:954-965. */
 ax_status ax_assemble_integral(int nv, int ng, int ks1, int ks2, int mm, const double *bsp1,
-                               const double *bsp2, const double *weights, double *mat,
+                               const double *bsp2, const double *weights, int n_w, double *mat,
                                void *stream);
+/* methods/dirac_fock.py:957,962: M = (M + M^T)/2 in place, n_mat (mm, mm). */
+ax_status ax_symmetrize(double *mat, int mm, int n_mat, void *stream);
 
 /* numerics/bspline/slater_yk.py:35-122 _exchange_matrix_kernel: K (ns, ns)
  * for orbital samples fb (nv*ng), node-wise cumulative Y^k scans. */
 ax_status ax_exchange_matrix_yk(const double *fb, const double *bsp, const double *r,
                                 const double *w, int nv, int ng, int ks, int ns, int k,
                                 double *K, void *stream);
+
+/* ax_exchange_matrix_yk for n_items (orbital, rank) pairs at once:
+ * K[t] (ns, ns) from row item_b[t] of fb (rows of ld_fb samples) and rank
+ * item_k[t] <= kmax_tab of the ax_grid_rpow tables rk_tab / rkm_tab
+ * (kmax_tab+1, nv*ng).  item_b, item_k are device arrays (NULL item_b: row t;
+ * NULL item_k: rank kmax_tab).  The caller checks item_k[t] <= kmax_tab.
+ * Entries whose B-spline supports are disjoint use the factorised form
+ * K[m][n] = A^-(m) T_out(n) or A^+(m) T_in(n) of the same integral; the band
+ * |m - n| < ks follows the reference's per-node sum. */
+ax_status ax_exchange_matrix_yk_batch(const double *fb, int64_t ld_fb, const double *bsp,
+                                      const double *w, const double *rk_tab,
+                                      const double *rkm_tab, int kmax_tab, int nv, int ng, int ks,
+                                      int ns, int n_items, const int32_t *item_b,
+                                      const int32_t *item_k, double *K, void *stream);
 
 /* numerics/bspline/slater.py:51-115 slater_rk_direct: O(N^2) double
  * quadrature, disjoint cells by the product rule, coincident cells split

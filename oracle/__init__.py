@@ -185,6 +185,44 @@ def exchange_matrix_kernel(fb, bsp, gr, grw, nv, ng, ks, ns, k):
     return K
 
 
+def assemble_integral(nv, ks1, ks2, mm, bsp1, bsp2, weights):
+    """numerics/bspline/integrals.py:18-56: <B1_i | w | B2_j>, (mm, mm)."""
+    bsp1, bsp2, weights = _c(bsp1), _c(bsp2), _c(weights)
+    ng = bsp1.shape[1]
+    mat = np.empty((mm, mm))
+    lib().orc_assemble_integral(ctypes.c_int(nv), ctypes.c_int(ng), ctypes.c_int(ks1),
+                                ctypes.c_int(ks2), ctypes.c_int(mm), _p(bsp1), _p(bsp2),
+                                _p(weights), _p(mat))
+    return mat
+
+
+def direct_potential_yk(P_stack, Q_stack, occ, bsp_data, target=None):
+    """methods/dirac_fock.py:929-965 (_build_direct_potential_yk): Y^0 of the
+    occupation-weighted density (minus one electron of ``target``), assembled
+    in the P and Q bases and symmetrised.  Returns (V_J_PP, V_J_QQ)."""
+    bd = bsp_data
+    nv, ns = bd["nv"], bd["ns"]
+    gr, grw = bd["gr"], bd["grw"]
+    ng = gr.shape[1]
+    ksp, nsp = bd.get("ksp", bd["ks"]), bd.get("nsp", ns)
+    ksq, nsq = bd.get("ksq", bd["ks"]), bd.get("nsq", ns)
+    bsp_p, bsp_q = bd["bsp_p"], bd.get("bsp_q", bd["bsp_p"])
+    rho = np.zeros((nv, ng))
+    for P, Q, q in zip(P_stack, Q_stack, occ):
+        fP = eval_orbital_on_grid(P, bsp_p, nv, ksp, nsp)
+        fQ = eval_orbital_on_grid(Q, bsp_q, nv, ksq, nsq)
+        rho += q * (fP ** 2 + fQ ** 2)
+    if target is not None:
+        fP = eval_orbital_on_grid(P_stack[target], bsp_p, nv, ksp, nsp)
+        fQ = eval_orbital_on_grid(Q_stack[target], bsp_q, nv, ksq, nsq)
+        rho -= fP ** 2 + fQ ** 2
+    yk = compute_yk(rho, gr, grw, nv, ng, 0)
+    wts = grw * yk
+    vpp = assemble_integral(nv, ksp, ksp, ns, bsp_p, bsp_p, wts)
+    vqq = assemble_integral(nv, ksq, ksq, ns, bsp_q, bsp_q, wts)
+    return 0.5 * (vpp + vpp.T), 0.5 * (vqq + vqq.T)
+
+
 def eval_orbital_on_grid(coeffs, bsp, nv, ks, ns):
     """numerics/bspline/evaluate.py:71-97."""
     gi = np.arange(nv)[:, None] + np.arange(ks)[None, :]

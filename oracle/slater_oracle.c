@@ -642,6 +642,27 @@ void orc_banded_to_full_add(double *V, const double *vb, int ns, int ks, double 
         }
 }
 
+/* numerics/bspline/integrals.py:18-56 (_assemble_integral): <B1_i|w|B2_j>
+ * on an (mm, mm) matrix, accumulated cell by cell in the reference's order. */
+void orc_assemble_integral(int nv, int ng, int ks1, int ks2, int mm, const double *bsp1,
+                           const double *bsp2, const double *weights, double *mat) {
+    memset(mat, 0, sizeof(double) * (size_t)mm * mm);
+    for (int iv = 0; iv < nv; ++iv)
+        for (int i = 0; i < ks1; ++i) {
+            const int gi = iv + i;
+            if (gi >= mm) continue;
+            for (int j = 0; j < ks2; ++j) {
+                const int gj = iv + j;
+                if (gj >= mm) continue;
+                double s = 0.0;
+                for (int m = 0; m < ng; ++m)
+                    s += weights[IDX2(iv, m, ng)] * bsp1[((long)iv * ng + m) * ks1 + i] *
+                         bsp2[((long)iv * ng + m) * ks2 + j];
+                mat[IDX2(gi, gj, mm)] += s;
+            }
+        }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Semiseparable (scan) form of the cell algorithm                          */
 /* ---------------------------------------------------------------------- */

@@ -192,3 +192,62 @@ def test_angular_coefficients_closed_values():
             assert s == pytest.approx(1.0, rel=1e-14)
     assert O.exchange_k_range(-1, -2) == [1]
     assert O.exchange_k_range(-2, -2) == [0, 2]
+
+
+# ---------------------------------------------------------------------------
+# _assemble_integral (integrals.py:18-56) and the Y^0 direct potential
+# (dirac_fock.py:929-965)
+# ---------------------------------------------------------------------------
+def _full_bspline_matrix(bd, bsp, ks, ns):
+    """Dense (nv*ng, ns) table of B_i at the Gauss nodes."""
+    nv, ng = bd["nv"], bd["gr"].shape[1]
+    Bm = np.zeros((nv * ng, ns))
+    for iv in range(nv):
+        for i in range(ks):
+            if iv + i < ns:
+                Bm[iv * ng:(iv + 1) * ng, iv + i] = bsp[iv, :, i]
+    return Bm
+
+
+def test_assemble_integral_closed_forms():
+    bd, t = physics_workspace(Z=2, ks=8, r_max=30.0)
+    nv, ns, ks = bd["nv"], bd["ns"], bd["ks"]
+    gr, grw, bsp = bd["gr"], bd["grw"], bd["bsp_p"]
+    S = O.assemble_integral(nv, ks, ks, ns, bsp, bsp, grw)
+    # partition of unity: sum_ij S_ij = r_max; row sums = int B_i = (t_{i+k}-t_i)/k
+    assert abs(S.sum() - t[-1]) < 1e-12 * t[-1]
+    row = (t[ks:ks + ns] - t[:ns]) / ks
+    assert np.max(np.abs(S.sum(axis=1) - row)) < 1e-13 * t[-1]
+    # r^3 weight: sum_ij = r_max^4 / 4 (the Gauss rule is exact for r^3)
+    M3 = O.assemble_integral(nv, ks, ks, ns, bsp, bsp, grw * gr ** 3)
+    assert abs(M3.sum() - t[-1] ** 4 / 4) < 1e-12 * t[-1] ** 4
+    # against a dense B^T diag(w) B with an arbitrary weight (index check)
+    wts = grw * np.cos(gr) / (1.0 + gr)
+    Bm = _full_bspline_matrix(bd, bsp, ks, ns)
+    ref = Bm.T @ (wts.ravel()[:, None] * Bm)
+    got = O.assemble_integral(nv, ks, ks, ns, bsp, bsp, wts)
+    assert np.max(np.abs(got - ref)) < 1e-13 * np.max(np.abs(ref))
+    # truncated output (mm < ns) keeps the leading block
+    got_m = O.assemble_integral(nv, ks, ks, ns - 3, bsp, bsp, wts)
+    assert np.array_equal(got_m, got[:ns - 3, :ns - 3])
+
+
+def test_direct_potential_yk_hydrogenic():
+    """V_J of the projected 1s density against <B_i| Y0_exact |B_j> with the
+    closed form Y^0(r) = 1/r - (Z + 1/r) e^{-2Zr}; minus the target's own
+    electron the potential of one occupied orbital vanishes."""
+    Z = 1.0
+    bd, t = physics_workspace(Z=1)
+    coeffs, _ = project_1s(bd, t, Z)
+    ns, ks = bd["ns"], bd["ks"]
+    gr, grw, bsp = bd["gr"], bd["grw"], bd["bsp_p"]
+    P = coeffs[None, :]
+    Qz = np.zeros_like(P)
+    vpp, vqq = O.direct_potential_yk(P, Qz, [1.0], bd)
+    y0 = 1.0 / gr - (Z + 1.0 / gr) * np.exp(-2.0 * Z * gr)
+    Bm = _full_bspline_matrix(bd, bsp, ks, ns)
+    ref = Bm.T @ ((grw * y0).ravel()[:, None] * Bm)
+    assert np.max(np.abs(vpp - ref)) < 1e-9 * np.max(np.abs(ref))
+    assert np.array_equal(vpp, vpp.T) and np.array_equal(vqq, vpp)
+    v0, _ = O.direct_potential_yk(P, Qz, [1.0], bd, target=0)
+    assert np.max(np.abs(v0)) < 1e-14 * np.max(np.abs(ref))
