@@ -123,6 +123,24 @@ def cpu_baseline(P, Q, gr, grw, ls, kmax, threads, min_seconds=6.0):
     return reps * qp.size / dt, dt, reps
 
 
+TRAFFIC_TABLE = "profiles/c2_hbm_traffic.txt"
+
+
+def hbm_traffic(args, kernel, k_split):
+    """HBM bytes per launch of the contraction kernel at this K split, from
+    the calibrated FETCH_SIZE + WRITE_SIZE table measured on the default
+    workload (None for other workloads or splits)."""
+    default = (args.nv, args.ng, args.n_orb, args.kmax, args.seed) == (400, 8, 40, 8, 0)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_TABLE)
+    if not default or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 6 and f[0] == "traffic:" and f[1] == kernel and int(f[2]) == int(k_split):
+            return int(f[5])
+    return None
+
+
 def serving_leg(args):
     """Poisson stream of Slater-integral and Fock-build requests through the
     server (split precision, continuous batching): p50/p99 latency and R^k/s;
@@ -270,8 +288,12 @@ def main():
         from atomx_amd._lib import load
         peak = INT8_MFMA_PEAK / load().ax_split_products()
     achieved = flops / (contract_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": f"rk_gemm_{prec}_kernel", "achieved": achieved,
-                "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+    kernel = "rk_gemm_f64_kernel" if prec == "fp64" else "rk_gemm_split_kernel"
+    k_split = modes[prec]["schedule"]["k_split"]
+    roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved,
+                "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": hbm_traffic(args, kernel, k_split),
+                "traffic_source": TRAFFIC_TABLE, "k_split": k_split,
                 "algorithmic_flops_per_launch": flops, "avg_launch_ms": contract_ms}
 
     ms = modes[prec]["ms_per_step"]
