@@ -59,12 +59,15 @@ SIGNATURES = {
     "ax_rk_gemm": (_int, [_vp, _i64, _vp, _vp, _i64, _int, _i64, _int, _vp, _vp, _vp]),
     "ax_rk_tensor_create": (_int, [_dp, _dp, _int, _int, _int, _ip, _int, _int,
                                    ctypes.POINTER(_vp)]),
+    "ax_rk_tensor_create_pairs": (_int, [_dp, _dp, _int, _int, _int, _int, _ip, _ip, _int,
+                                         _int, ctypes.POINTER(_vp)]),
     "ax_rk_tensor_destroy": (_int, [_vp]),
     "ax_rk_tensor_estimate": (_i64, [_int, _int, _int, _ip, _int, _int]),
     "ax_rk_tensor_size": (_i64, [_vp]),
     "ax_rk_tensor_pairs": (_int, [_vp, _int]),
     "ax_rk_tensor_offset": (_i64, [_vp, _int]),
     "ax_rk_tensor_footprint": (_i64, [_vp]),
+    "ax_rk_tensor_slices": (_int, [_vp]),
     "ax_rk_tensor_run": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "ax_rk_tensor_run_stage": (_int, [_vp, _int, _vp, _vp, _vp, _vp]),
     "ax_rk_tensor_run_batch": (_int, [ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp),

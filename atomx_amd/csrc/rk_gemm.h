@@ -39,7 +39,8 @@ struct RkWork {
     int pad;
 };
 
-constexpr int kSplitSlices = 7;   // int8 slices per operand (7 bits each)
+constexpr int kSplitSlices = 7;      // int8 slices per operand (7 bits each)
+constexpr int kSplitSlicesFine = 8;  // for grids whose K-tiles span a wide radius range
 constexpr int kSplitKTile = 64;   // K-tile that shares one exponent
 constexpr int kMaxBatch = 16;     // plans per batched launch
 
@@ -55,6 +56,7 @@ struct DensArg {             // weighted pair densities (+ their slices)
     int8_t *Ds;              // split precision: slice planes of D, else null
     int32_t *De;
     int64_t npad, ds_plane;
+    int slices, pad;
 };
 struct ScanArg {             // Y^k items of one plan
     const double *D;
@@ -66,12 +68,14 @@ struct ScanArg {             // Y^k items of one plan
     int8_t *Ys;              // slices written by the scan (n_gauss = 8, split)
     int32_t *Ye;
     int64_t npad, ys_plane;
+    int slices, pad;
 };
 struct SliceArg {            // rows sliced by a separate pass
     const double *src;
     int64_t n, npad, plane;
     int8_t *dst;
     int32_t *e;
+    int slices, pad;
 };
 struct GemmArg {             // contraction tiles and split reduction
     const RkSeg *segs;
@@ -140,19 +144,18 @@ __device__ inline int batch_find(const BatchOf<T> &B, int64_t it, int64_t &local
 
 // Split-precision slicing of one value per lane of a wave whose 64 lanes
 // hold one 64-wide K-tile (lane l = element t*64 + l): power-of-two exponent
-// e with max|v| < 2^e over the tile, then kSplitSlices signed 7-bit slices
+// e with max|v| < 2^e over the tile, then `slices` signed 7-bit slices
 // of v*2^-e by truncation, plane s at dst_row[s*plane + x]; lane 0 writes e.
 // Every lane of the wave must call it (it reduces across the wave).
 __device__ inline void slice_value(double v, int64_t x, int8_t *__restrict__ dst_row,
-                                   int64_t plane, int32_t *__restrict__ e_slot) {
+                                   int64_t plane, int32_t *__restrict__ e_slot, int slices) {
     double mx = fabs(v);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
     const int ex = mx > 0.0 ? ilogb(mx) + 1 : 0;
     double rho = ldexp(v, 7 - ex);  // v * 2^-e * 2^7, |rho| < 128
     int8_t *d = dst_row + x;
-#pragma unroll
-    for (int s = 0; s < kSplitSlices; ++s) {
+    for (int s = 0; s < slices; ++s) {
         const double c = trunc(rho);
         d[(int64_t)s * plane] = static_cast<int8_t>(c);
         rho = (rho - c) * 128.0;
@@ -163,9 +166,9 @@ __device__ inline void slice_value(double v, int64_t x, int8_t *__restrict__ dst
 // One wave slices K-tile t of a row of n values (zero beyond n).
 __device__ inline void slice_tile(const double *__restrict__ src_row, int64_t n, int64_t t,
                                   int8_t *__restrict__ dst_row, int64_t dst_plane,
-                                  int32_t *__restrict__ e_slot) {
+                                  int32_t *__restrict__ e_slot, int slices) {
     const int64_t x = t * kSplitKTile + (threadIdx.x & 63);
-    slice_value(x < n ? src_row[x] : 0.0, x, dst_row, dst_plane, e_slot);
+    slice_value(x < n ? src_row[x] : 0.0, x, dst_row, dst_plane, e_slot, slices);
 }
 
 int gemm_tile();
@@ -177,6 +180,7 @@ struct PlanHost {
     int64_t n, npad;
     int nv, ng, kmax, n_pairs, n_items, n_yrows, n_work, n_split, precision;
     int order;              // contraction block order (kOrder*)
+    int slices;             // split precision: int8 slices per operand
     const double *mo, *mi;  // partial matrices of ng (device)
     const double *w, *rk, *rkm;
     const int32_t *pa, *pb, *item_row, *item_k, *item_out;
@@ -198,7 +202,8 @@ ax_status slice_batch_launch(const PlanHost *const *plans, int n, hipStream_t st
 ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream);
 ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const *out,
                               hipStream_t stream);
-ax_status rk_gemm_split_launch(const BatchOf<GemmArg> &A, dim3 grid, hipStream_t stream);
+ax_status rk_gemm_split_launch(const BatchOf<GemmArg> &A, int slices, dim3 grid,
+                               hipStream_t stream);
 
 // Slices n_rows rows of src (row r at src + rows[r]*ld, or r*ld) into
 // kSplitSlices int8 planes dst[s*plane + r*dst_ld + x] (x < npad, zero

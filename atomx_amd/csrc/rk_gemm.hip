@@ -200,28 +200,45 @@ ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t
         set_error("contract_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
         return AX_ERR_VALUE;
     }
-    const int precision = plans[0]->precision;
-    for (int i = 1; i < n; ++i)
-        if (plans[i]->precision != precision) {
-            set_error("a batched launch needs one precision mode for all plans");
+    for (int i = 0; i < n; ++i) {
+        const PlanHost *h = plans[i];
+        const bool ok = h->precision == AX_PREC_FP64 ||
+                        (h->precision == AX_PREC_SPLIT &&
+                         (h->slices == kSplitSlices || h->slices == kSplitSlicesFine));
+        if (!ok) {
+            set_error("contract_batch_launch: plan %d has precision %d with %d slices", i,
+                      h->precision, h->slices);
             return AX_ERR_VALUE;
         }
-    BatchOf<GemmArg> A{};
-    const int64_t total = fill_work(A, plans, n, nullptr);
-    if (total == 0) return AX_OK;
-    if (total * A.max_split > 0x7fffffff) {
-        set_error("contract_batch_launch: %lld tiles exceed one launch", (long long)total);
-        return AX_ERR_VALUE;
     }
-    const dim3 grid(static_cast<unsigned>(total * A.max_split));
-    if (precision == AX_PREC_FP64) {
-        hipLaunchKernelGGL(rk_gemm_f64_kernel, grid, dim3(256), 0, stream, A);
-        AX_CHECK_LAUNCH();
-        return AX_OK;
+    // one launch per contraction kind (FP64, split with 7 or 8 slices)
+    const int kinds[3][2] = {{AX_PREC_FP64, 0},
+                             {AX_PREC_SPLIT, kSplitSlices},
+                             {AX_PREC_SPLIT, kSplitSlicesFine}};
+    for (const auto &kd : kinds) {
+        const PlanHost *sub[kMaxBatch];
+        int ns = 0;
+        for (int i = 0; i < n; ++i)
+            if (plans[i]->precision == kd[0] && (kd[0] == AX_PREC_FP64 || plans[i]->slices == kd[1]))
+                sub[ns++] = plans[i];
+        if (ns == 0) continue;
+        BatchOf<GemmArg> A{};
+        const int64_t total = fill_work(A, sub, ns, nullptr);
+        if (total == 0) continue;
+        if (total * A.max_split > 0x7fffffff) {
+            set_error("contract_batch_launch: %lld tiles exceed one launch", (long long)total);
+            return AX_ERR_VALUE;
+        }
+        const dim3 grid(static_cast<unsigned>(total * A.max_split));
+        if (kd[0] == AX_PREC_FP64) {
+            hipLaunchKernelGGL(rk_gemm_f64_kernel, grid, dim3(256), 0, stream, A);
+            AX_CHECK_LAUNCH();
+        } else {
+            ax_status st = rk_gemm_split_launch(A, kd[1], grid, stream);
+            if (st != AX_OK) return st;
+        }
     }
-    if (precision == AX_PREC_SPLIT) return rk_gemm_split_launch(A, grid, stream);
-    set_error("unknown precision mode %d", precision);
-    return AX_ERR_VALUE;
+    return AX_OK;
 }
 
 ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const *out,

@@ -20,7 +20,6 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int S = ax::kSplitSlices;
 constexpr int KT = ax::kSplitKTile;
 constexpr int BM = 64;
 
@@ -39,7 +38,7 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restric
         const int64_t t = item - (int64_t)r * n_tiles;
         const int64_t rr = rows ? (int64_t)rows[r] : (int64_t)r;
         ax::slice_tile(src + rr * ld, n, t, dst + (int64_t)r * dst_ld, dst_plane,
-                   e + (int64_t)r * e_ld + t);
+                       e + (int64_t)r * e_ld + t, ax::kSplitSlices);
     }
 }
 
@@ -56,7 +55,8 @@ __global__ __launch_bounds__(256) void slice_batch_kernel(const ax::BatchOf<ax::
         const int64_t n_tiles = g.npad / KT;
         const int64_t r = local / n_tiles;
         const int64_t t = local - r * n_tiles;
-        ax::slice_tile(g.src + r * g.n, g.n, t, g.dst + r * g.npad, g.plane, g.e + r * n_tiles + t);
+        ax::slice_tile(g.src + r * g.n, g.n, t, g.dst + r * g.npad, g.plane, g.e + r * n_tiles + t,
+                       g.slices);
     }
 }
 
@@ -69,6 +69,7 @@ constexpr int PLANEB = BM * ROWB;
 
 // Global -> register staging: thread t moves S chunks of 16 bytes, chunk
 // (s, row = t/4, q = t%4) of the K-tile.
+template <int S>
 __device__ inline void stage_load(const int8_t *__restrict__ base, int64_t plane, int64_t ld,
                                   const int32_t *__restrict__ rows, int row0, int m, int64_t t,
                                   v4i (&reg)[S]) {
@@ -86,6 +87,7 @@ __device__ inline void stage_load(const int8_t *__restrict__ base, int64_t plane
     }
 }
 
+template <int S>
 __device__ inline void stage_store(int8_t *__restrict__ lds, const v4i (&reg)[S]) {
     const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
 #pragma unroll
@@ -93,9 +95,11 @@ __device__ inline void stage_store(int8_t *__restrict__ lds, const v4i (&reg)[S]
         *reinterpret_cast<v4i *>(lds + s2 * PLANEB + r * ROWB + q * 16) = reg[s2];
 }
 
+template <int S>
 __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax::GemmArg> A) {
-    __shared__ __attribute__((aligned(16))) int8_t As[S * PLANEB];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[S * PLANEB];
+    extern __shared__ __attribute__((aligned(16))) int8_t lds_split[];
+    int8_t *As = lds_split;                // [S * PLANEB]
+    int8_t *Bs = lds_split + S * PLANEB;   // [S * PLANEB]
     __shared__ int eAs[BM], eBs[BM];
 
     // one 1-D grid of tiles x max_split blocks in the launch's block order
@@ -142,22 +146,22 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax
     v4i ra[S], rb[S];
     int rexp = 0;
     if (tb < te) {
-        stage_load(sg.Ds, sg.ds_plane, sg.ds_ld, sg.rows, row0, sg.m, tb, ra);
-        stage_load(sg.Ys, sg.ys_plane, sg.ys_ld, nullptr, col0, sg.m, tb, rb);
+        stage_load<S>(sg.Ds, sg.ds_plane, sg.ds_ld, sg.rows, row0, sg.m, tb, ra);
+        stage_load<S>(sg.Ys, sg.ys_plane, sg.ys_ld, nullptr, col0, sg.m, tb, rb);
         if (tid < BM) rexp = eok ? sg.De[erow * sg.de_ld + tb] : 0;
         else if (tid < 2 * BM) rexp = eok ? sg.Ye[erow * sg.ye_ld + tb] : 0;
     }
     const v4i zero = {0, 0, 0, 0};
     for (int64_t t = tb; t < te; ++t) {
         __syncthreads();  // previous tile's LDS reads are complete
-        stage_store(As, ra);
-        stage_store(Bs, rb);
+        stage_store<S>(As, ra);
+        stage_store<S>(Bs, rb);
         if (tid < BM) eAs[tid] = rexp;
         else if (tid < 2 * BM) eBs[tid - BM] = rexp;
         __syncthreads();
         if (t + 1 < te) {  // prefetch the next K-tile while this one is multiplied
-            stage_load(sg.Ds, sg.ds_plane, sg.ds_ld, sg.rows, row0, sg.m, t + 1, ra);
-            stage_load(sg.Ys, sg.ys_plane, sg.ys_ld, nullptr, col0, sg.m, t + 1, rb);
+            stage_load<S>(sg.Ds, sg.ds_plane, sg.ds_ld, sg.rows, row0, sg.m, t + 1, ra);
+            stage_load<S>(sg.Ys, sg.ys_plane, sg.ys_ld, nullptr, col0, sg.m, t + 1, rb);
             if (tid < BM) rexp = eok ? sg.De[erow * sg.de_ld + t + 1] : 0;
             else if (tid < 2 * BM) rexp = eok ? sg.Ye[erow * sg.ye_ld + t + 1] : 0;
         }
@@ -230,10 +234,23 @@ ax_status slice_rows_launch(const double *src, int64_t ld, const int32_t *rows, 
     return AX_OK;
 }
 
-ax_status rk_gemm_split_launch(const BatchOf<GemmArg> &A, dim3 grid, hipStream_t stream) {
-    hipLaunchKernelGGL(rk_gemm_split_kernel, grid, dim3(256), 0, stream, A);
+template <int S>
+ax_status launch_split(const BatchOf<GemmArg> &A, dim3 grid, hipStream_t stream) {
+    const size_t lds = (size_t)2 * S * PLANEB;
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(rk_gemm_split_kernel<S>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipLaunchKernelGGL(rk_gemm_split_kernel<S>, grid, dim3(256), lds, stream, A);
     AX_CHECK_LAUNCH();
     return AX_OK;
+}
+
+ax_status rk_gemm_split_launch(const BatchOf<GemmArg> &A, int slices, dim3 grid,
+                               hipStream_t stream) {
+    if (slices == kSplitSlices) return launch_split<kSplitSlices>(A, grid, stream);
+    if (slices == kSplitSlicesFine) return launch_split<kSplitSlicesFine>(A, grid, stream);
+    set_error("split contraction: unsupported slice count %d", slices);
+    return AX_ERR_VALUE;
 }
 
 ax_status slice_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream) {
@@ -251,7 +268,7 @@ ax_status slice_batch_launch(const PlanHost *const *plans, int n, hipStream_t st
         if (todo) total += (int64_t)h->n_yrows * (h->npad / KT);
         A.end[i] = total;
         const int64_t yr = h->n_yrows > 0 ? h->n_yrows : 1;
-        A.a[i] = SliceArg{h->Y, h->n, h->npad, yr * h->npad, h->Ys, h->Ye};
+        A.a[i] = SliceArg{h->Y, h->n, h->npad, yr * h->npad, h->Ys, h->Ye, h->slices, 0};
     }
     if (total == 0) return AX_OK;
     const int grid = static_cast<int>((total + 3) / 4 < 65535 ? (total + 3) / 4 : 65535);

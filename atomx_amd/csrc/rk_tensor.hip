@@ -15,6 +15,7 @@
 // order do not depend on the batch, so its results are bitwise the same
 // alone and in any batch.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -25,6 +26,7 @@ struct ax_rk_tensor {
     int nv = 0, ng = 0, n_orb = 0, kmax = 0, precision = 0;
     int64_t n = 0, npad = 0;
     int n_pairs = 0, n_yrows = 0, n_items = 0, n_work = 0, n_split = 1, split_cap = 1;
+    int slices = 0;  // int8 slices per operand (split precision)
     std::vector<int> m_k;
     std::vector<int64_t> out_off;
     int64_t out_size = 0;
@@ -55,6 +57,31 @@ void free_plan(ax_rk_tensor *p) {
     p->allocs.clear();
     if (p->done) (void)hipEventDestroy(p->done);
     p->done = nullptr;
+}
+
+// Int8 slices per operand for a split-precision plan, or 0 when the plan's
+// contraction has to run in FP64.  The slices of one 64-node K-tile share
+// one exponent, so a term's error is bounded relative to the largest |D| and
+// |Y^k| inside its tile, and across a tile r^-(k+1) and r^k change by
+// (r_last / r_first)^(k+1) = 2^bits.  The bound scales as 2^(bits - 7 S):
+// 7 slices up to 5 bits, 8 slices up to 12 bits (coarse exponential grids at
+// high k), FP64 beyond.  The first cell reaches r = 0 and is left out (the
+// Gauss weights there are of the order of its width).  Without the grid
+// (footprint estimates) 8 slices are assumed.
+int choose_slices(const double *r, int nv, int ng, int kmax) {
+    if (!r) return ax::kSplitSlicesFine;
+    const int64_t n = (int64_t)nv * ng;
+    double worst = 1.0;
+    for (int64_t x0 = 0; x0 < n; x0 += ax::kSplitKTile) {
+        const int64_t x1 = std::min(n, x0 + ax::kSplitKTile) - 1;
+        const int64_t start = std::max<int64_t>(x0, ng);
+        if (start > x1 || r[start] <= 0.0) continue;
+        worst = std::max(worst, r[x1] / r[start]);
+    }
+    const double bits = (kmax + 1) * std::log2(worst);
+    if (bits <= 5.0) return ax::kSplitSlices;
+    if (bits <= 12.0) return ax::kSplitSlicesFine;
+    return 0;
 }
 
 int choose_split(int n_work, int64_t n) {
@@ -108,29 +135,52 @@ namespace {
 
 // Builds a plan, or (estimate != nullptr) only computes the device bytes it
 // would hold, without touching the device.
+// Explicit pair list of a pair-product plan (every pair valid at every rank);
+// null for the all-pairs plan with the angular rule.
+struct PairList {
+    int n_pairs;
+    const int32_t *pa, *pb;
+};
+
 ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng, int n_orb,
                       const int32_t *l_host, int kmax, int precision, ax_rk_tensor **out_plan,
-                      int64_t *estimate) {
+                      int64_t *estimate, const PairList *explicit_pairs = nullptr) {
     AX_REQUIRE(out_plan != nullptr || estimate != nullptr, "ax_rk_tensor_create: null plan pointer");
-    AX_REQUIRE(l_host && (estimate || (r_host && w_host)),
+    AX_REQUIRE((l_host || explicit_pairs) && (estimate || (r_host && w_host)),
                "ax_rk_tensor_create: null grid or l values");
     AX_REQUIRE(nv >= 1 && ng >= 1 && ng <= 16, "ax_rk_tensor_create: bad grid nv=%d ng=%d", nv, ng);
     AX_REQUIRE(n_orb >= 1 && kmax >= 0 && kmax <= 64, "ax_rk_tensor_create: bad n_orb/kmax");
     AX_REQUIRE(precision == AX_PREC_FP64 || precision == AX_PREC_SPLIT,
                "ax_rk_tensor_create: unknown precision %d", precision);
-    for (int a = 0; a < n_orb; ++a)
-        AX_REQUIRE(l_host[a] >= 0, "ax_rk_tensor_create: negative l for orbital %d", a);
+    if (!explicit_pairs)
+        for (int a = 0; a < n_orb; ++a)
+            AX_REQUIRE(l_host[a] >= 0, "ax_rk_tensor_create: negative l for orbital %d", a);
+    if (precision == AX_PREC_SPLIT && choose_slices(r_host, nv, ng, kmax) == 0)
+        precision = AX_PREC_FP64;  // the grid is too coarse for the split error bound
     const int64_t n = (int64_t)nv * ng;
     const int64_t npad = (n + ax::kSplitKTile - 1) / ax::kSplitKTile * ax::kSplitKTile;
     const int64_t tiles = npad / ax::kSplitKTile;
 
-    // pairs a <= b in lexicographic order
+    // pairs a <= b in lexicographic order, or the caller's list
     std::vector<int32_t> pa, pb;
-    for (int a = 0; a < n_orb; ++a)
-        for (int b = a; b < n_orb; ++b) {
+    if (explicit_pairs) {
+        AX_REQUIRE(explicit_pairs->n_pairs >= 1 && explicit_pairs->pa && explicit_pairs->pb,
+                   "ax_rk_tensor_create_pairs: empty pair list");
+        for (int p = 0; p < explicit_pairs->n_pairs; ++p) {
+            const int a = explicit_pairs->pa[p], b = explicit_pairs->pb[p];
+            AX_REQUIRE(a >= 0 && a < n_orb && b >= 0 && b < n_orb,
+                       "ax_rk_tensor_create_pairs: pair %d = (%d, %d) outside [0, %d)", p, a, b,
+                       n_orb);
             pa.push_back(a);
             pb.push_back(b);
         }
+    } else {
+        for (int a = 0; a < n_orb; ++a)
+            for (int b = a; b < n_orb; ++b) {
+                pa.push_back(a);
+                pb.push_back(b);
+            }
+    }
     const int n_pairs = static_cast<int>(pa.size());
 
     // valid pairs per rank, Y^k slots, scan items
@@ -143,9 +193,11 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
         y_off[k] = yrows;
         int m = 0;
         for (int p = 0; p < n_pairs; ++p) {
-            const int la = l_host[pa[p]], lb = l_host[pb[p]];
-            const int lo = la > lb ? la - lb : lb - la;
-            if (k < lo || k > la + lb || ((la + lb + k) & 1)) continue;
+            if (!explicit_pairs) {
+                const int la = l_host[pa[p]], lb = l_host[pb[p]];
+                const int lo = la > lb ? la - lb : lb - la;
+                if (k < lo || k > la + lb || ((la + lb + k) & 1)) continue;
+            }
             rows_all.push_back(p);
             it_row.push_back(p);
             it_k.push_back(k);
@@ -191,9 +243,10 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     const int64_t o_Y = y_fp64 ? L.add(yr * n * 8) : -1;
     const int64_t o_part = L.add(ax::rk_gemm_partial_doubles(n_work, split_cap) * 8);
     int64_t o_Ds = 0, o_Ys = 0, o_De = 0, o_Ye = 0;
+    const int slices = precision == AX_PREC_SPLIT ? choose_slices(r_host, nv, ng, kmax) : 0;
     if (precision == AX_PREC_SPLIT) {
-        o_Ds = L.add((int64_t)ax::kSplitSlices * n_pairs * npad);
-        o_Ys = L.add((int64_t)ax::kSplitSlices * yr * npad);
+        o_Ds = L.add((int64_t)slices * n_pairs * npad);
+        o_Ys = L.add((int64_t)slices * yr * npad);
         o_De = L.add((int64_t)n_pairs * tiles * 4);
         o_Ye = L.add(yr * tiles * 4);
     }
@@ -216,6 +269,7 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     plan->n_work = n_work;
     plan->n_split = n_split;
     plan->split_cap = split_cap;
+    plan->slices = slices;
     plan->m_k = m_k;
     plan->out_off = out_off;
     plan->out_size = outsz;
@@ -307,6 +361,7 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     h.n_split = n_split;
     h.precision = precision;
     h.order = ax::kOrderLinear;
+    h.slices = slices;
     h.mo = static_cast<const double *>(dptr(o_mo));
     h.mi = static_cast<const double *>(dptr(o_mi));
     h.w = static_cast<const double *>(dptr(o_w));
@@ -358,6 +413,16 @@ ax_status ax_rk_tensor_create(const double *r_host, const double *w_host, int nv
     return plan_create(r_host, w_host, nv, ng, n_orb, l_host, kmax, precision, out_plan, nullptr);
 }
 
+ax_status ax_rk_tensor_create_pairs(const double *r_host, const double *w_host, int nv, int ng,
+                                    int n_fn, int n_pairs, const int32_t *pa_host,
+                                    const int32_t *pb_host, int kmax, int precision,
+                                    ax_rk_tensor **out_plan) {
+    AX_REQUIRE(out_plan != nullptr, "ax_rk_tensor_create_pairs: null plan pointer");
+    const PairList pl{n_pairs, pa_host, pb_host};
+    return plan_create(r_host, w_host, nv, ng, n_fn, nullptr, kmax, precision, out_plan, nullptr,
+                       &pl);
+}
+
 int64_t ax_rk_tensor_estimate(int nv, int ng, int n_orb, const int32_t *l_host, int kmax,
                               int precision) {
     int64_t bytes = -1;
@@ -393,6 +458,8 @@ int64_t ax_rk_tensor_offset(const ax_rk_tensor *plan, int k) {
 
 int64_t ax_rk_tensor_footprint(const ax_rk_tensor *plan) { return plan ? plan->footprint : -1; }
 
+int ax_rk_tensor_slices(const ax_rk_tensor *plan) { return plan ? plan->slices : -1; }
+
 ax_status ax_rk_tensor_run_stage(ax_rk_tensor *plan, int stage, const double *P,
                                  const double *Q, double *out, void *stream) {
     AX_REQUIRE(plan != nullptr, "ax_rk_tensor_run_stage: null plan");
@@ -414,11 +481,8 @@ ax_status ax_rk_tensor_run_batch(ax_rk_tensor *const *plans, int n_plans, const 
     AX_REQUIRE(n_plans >= 0, "ax_rk_tensor_run_batch: negative plan count");
     if (n_plans == 0) return AX_OK;
     AX_REQUIRE(plans && P && out, "ax_rk_tensor_run_batch: null argument");
-    for (int i = 0; i < n_plans; ++i) {
+    for (int i = 0; i < n_plans; ++i)
         AX_REQUIRE(plans[i] && P[i] && out[i], "ax_rk_tensor_run_batch: null entry %d", i);
-        AX_REQUIRE(plans[i]->precision == plans[0]->precision,
-                   "ax_rk_tensor_run_batch: plans of one batch need one precision mode");
-    }
     hipStream_t s = ax::as_stream(stream);
     for (int b0 = 0; b0 < n_plans; b0 += ax::kMaxBatch) {
         const int nb = std::min(ax::kMaxBatch, n_plans - b0);
@@ -611,6 +675,7 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
     h.n_work = nw;
     h.n_split = split;
     h.precision = precision;
+    h.slices = ax::kSplitSlices;
     h.segs = reinterpret_cast<const ax::RkSeg *>(ws + g.o_seg);
     h.work = reinterpret_cast<const ax::RkWork *>(ws + g.o_work);
     h.partial = reinterpret_cast<double *>(ws + g.o_part);

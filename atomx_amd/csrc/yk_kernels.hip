@@ -87,7 +87,8 @@ __global__ __launch_bounds__(256) void pair_density_batch_kernel(
             }
             if (g.Ds)
                 ax::slice_value(v, x, g.Ds + p * g.npad, g.ds_plane,
-                                g.De + p * (g.npad / ax::kSplitKTile) + x / ax::kSplitKTile);
+                                g.De + p * (g.npad / ax::kSplitKTile) + x / ax::kSplitKTile,
+                                g.slices);
         }
     }
 }
@@ -277,7 +278,7 @@ __device__ inline void scan_item(double *__restrict__ sm, ScanLds<NG> &L,
                                  const double *__restrict__ rkp, const double *__restrict__ rkmp,
                                  int64_t n, int nv, double *__restrict__ dst,
                                  int8_t *__restrict__ sdst = nullptr, int64_t splane = 0,
-                                 int32_t *__restrict__ edst = nullptr) {
+                                 int32_t *__restrict__ edst = nullptr, int slices = 0) {
     static_assert(!SLICE || NG * (64 / NG) == 64, "in-scan slicing needs whole K-tiles per wave");
     constexpr int P = NG + 1;
     double *gos = sm;                          // [nv * P] g_out
@@ -376,7 +377,7 @@ __device__ inline void scan_item(double *__restrict__ sm, ScanLds<NG> &L,
                     v = rkmp[x] * (bo[c] + po) + rkp[x] * (bi[c] + pi);
                 }
                 const int64_t x = (int64_t)c * NG + node;
-                ax::slice_value(v, x, sdst, splane, edst + x / ax::kSplitKTile);
+                ax::slice_value(v, x, sdst, splane, edst + x / ax::kSplitKTile, slices);
             }
         } else if (lane_used && c < nv) {
             const double *a = gos + (int64_t)c * P;
@@ -454,7 +455,7 @@ __global__ __launch_bounds__(kScanThreads) void yk_scan_batch_kernel(
         if constexpr (SLICE)
             scan_item<NG, true>(sm, L, src, nullptr, g.rk + (int64_t)k * n,
                                 g.rkm + (int64_t)k * n, n, g.nv, nullptr, g.Ys + orow * g.npad,
-                                g.ys_plane, g.Ye + orow * (g.npad / ax::kSplitKTile));
+                                g.ys_plane, g.Ye + orow * (g.npad / ax::kSplitKTile), g.slices);
         else
             scan_item<NG>(sm, L, src, nullptr, g.rk + (int64_t)k * n, g.rkm + (int64_t)k * n, n,
                           g.nv, g.Y + orow * n);
@@ -872,7 +873,7 @@ ax_status density_batch_launch(const PlanHost *const *plans, int n, const double
         const bool sl = h->precision == AX_PREC_SPLIT;
         A.a[i] = DensArg{P[i], Q ? Q[i] : nullptr, h->n, h->n, h->pa, h->pb, h->w, h->D,
                          sl ? h->Ds : nullptr, sl ? h->De : nullptr, h->npad,
-                         (int64_t)h->n_pairs * h->npad};
+                         (int64_t)h->n_pairs * h->npad, h->slices, 0};
         const int64_t len = sl ? h->npad : h->n;
         if (len > nmax) nmax = len;
     }
@@ -907,7 +908,7 @@ ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t str
                 const int64_t yr = h->n_yrows > 0 ? h->n_yrows : 1;
                 A.a[A.n] = ScanArg{h->D, h->Y, h->rk, h->rkm, h->item_row, h->item_k,
                                    h->item_out, h->n, h->nv, h->kmax, h->Ys, h->Ye, h->npad,
-                                   yr * h->npad};
+                                   yr * h->npad, h->slices, 0};
                 ++A.n;
                 if (h->nv > nv_max) nv_max = h->nv;
                 mo = h->mo;

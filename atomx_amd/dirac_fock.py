@@ -21,7 +21,7 @@ import numpy as np
 from . import device as dv
 from ._lib import call
 from .angular import AngularTables
-from .convol import ConvolCache, _cell_rk_many, _pad_rows, axpy, density_sym_device
+from .convol import ConvolCache, _cell_rk_many, _pad_rows, axpy
 
 _BLOCKS = (((1, 1), "pppp"), ((1, 2), "pqpq"), ((2, 1), "qpqp"), ((2, 2), "qqqq"))
 
@@ -41,62 +41,20 @@ class FockBuilder:
         Q = dv.f64(_pad_rows(list(np.asarray(Q_stack, dtype=np.float64)), ns))
         return P, Q
 
+    def prepare(self, targets=None, skip_pq_exchange=False):
+        """Everything of a build that does not depend on the orbitals: the
+        per-(block, k) coefficient matrices and target lists on the device,
+        and the banks / factors of the contractions (built here).  The
+        returned step launches only kernels (no host transfers), so a caller
+        can capture ``step.run`` in a HIP graph and replay it on new orbitals
+        written into the same device buffers."""
+        return FockStep(self, targets, skip_pq_exchange)
+
     def build_all_device(self, P_stack, Q_stack, targets=None, skip_pq_exchange=False):
         """Fock blocks of every target as device tensors: dict t -> (ns,ns)x4
         (PP, QQ, PQ, QP)."""
-        tab, cache, ns = self.tables, self.cache, self.ns
-        n_orb = self.n_orb
-        targets = list(range(n_orb)) if targets is None else [int(t) for t in targets]
         P, Q = self._stacks(P_stack, Q_stack)
-        order = {"p": cache.ksp, "q": cache.ksq}
-        dens = {c: density_sym_device(M, [(j, j) for j in range(n_orb)], ns, order[c])
-                for c, M in (("p", P), ("q", Q))}
-        H = {t: [dv.zeros((ns, ns)) for _ in range(4)] for t in targets}
-        ks_all = sorted({k for t in targets for k in tab.k_for_target[t]})
-        for (ii, jj), block in _BLOCKS:
-            c_bra = "q" if ii == 2 else "p"
-            c_ket = "q" if jj == 2 else "p"
-            ks_bra, ks_ket = order[c_bra], order[c_ket]
-            cii = Q if ii == 2 else P
-            cjj = Q if jj == 2 else P
-            direct_slot = 0 if ii == 1 else 1
-            exch_slot = {(1, 1): 0, (1, 2): 2, (2, 1): 3, (2, 2): 1}[(ii, jj)]
-            for k in ks_all:
-                tk = [t for t in targets if k in tab.k_for_target[t]]
-                # --- direct: one combination + one contraction for all targets
-                tdir = [t for t in tk if np.any(tab.direct[t, :, k])]
-                if tdir:
-                    coef = np.zeros((len(tdir), n_orb))
-                    for r, t in enumerate(tdir):
-                        c = tab.direct[t, :, k].copy()
-                        c[t] = c[t] + c[t]
-                        coef[r] = c
-                    cd = dv.f64(coef)
-                    dd = dv.empty((len(tdir), ns, ks_ket))
-                    call("ax_combine_rows", dv.ptr(dens[c_ket]), ns * ks_ket, n_orb, dv.ptr(cd),
-                         len(tdir), dv.ptr(dd), dv.stream_handle())
-                    v = cache.case1(k, block, dd)
-                    for r, t in enumerate(tdir):
-                        call("ax_banded_to_full_add", dv.ptr(H[t][direct_slot]), ns * ns,
-                             dv.ptr(v[r]), ns * ks_bra, 1, ns, ks_bra, 1.0, dv.stream_handle())
-                # --- exchange
-                if skip_pq_exchange and ii == 1 and jj == 2:
-                    continue
-                tex = [t for t in tk if np.any(tab.exchange[t, :, k])]
-                if tex:
-                    coef = np.stack([tab.exchange[t, :, k] for t in tex])
-                    cd = dv.f64(coef)
-                    xx = dv.empty((len(tex), ns, ns))
-                    call("ax_exchange_density", dv.ptr(cii), dv.ptr(cjj), ns, n_orb, ns,
-                         dv.ptr(cd), len(tex), dv.ptr(xx), dv.stream_handle())
-                    kk = cache.case4(k, block, xx, symmetric=(ii == jj))
-                    for r, t in enumerate(tex):
-                        axpy(H[t][exch_slot], kk[r])
-        for t in targets:
-            pp, qq, pq, qp = H[t]
-            call("ax_fock_finalize", dv.ptr(pp), dv.ptr(qq), dv.ptr(pq), dv.ptr(qp), ns,
-                 float(tab.divisor(t)), 1 if skip_pq_exchange else 0, dv.stream_handle())
-        return H
+        return self.prepare(targets, skip_pq_exchange).run(P, Q)
 
     def build_all(self, P_stack, Q_stack, targets=None, skip_pq_exchange=False):
         H = self.build_all_device(P_stack, Q_stack, targets, skip_pq_exchange)
@@ -135,3 +93,123 @@ class FockBuilder:
             else:
                 EK += w * v
         return EJ - EK, EJ, EK
+
+
+class FockStep:
+    """A prepared per-iteration Fock build (``FockBuilder.prepare``).
+
+    ``run(P, Q)`` takes device coefficient stacks (n_orb, ns) and returns the
+    device blocks of every target; the launches follow
+    methods/dirac_fock.py:1128-1319 in the order of ``FockBuilder``: per
+    component block and rank, one row combination of the direct densities of
+    all targets (own density doubled), one case-1 contraction, banded->full
+    adds per target; one exchange-density launch, one case-4 contraction and
+    an accumulation per target; finally symmetrisation and division by the
+    occupation."""
+
+    def __init__(self, fb, targets, skip_pq_exchange):
+        tab, cache = fb.tables, fb.cache
+        self.fb = fb
+        self.ns, self.n_orb = fb.ns, fb.n_orb
+        self.targets = list(range(fb.n_orb)) if targets is None else [int(t) for t in targets]
+        self.skip = bool(skip_pq_exchange)
+        n_orb = self.n_orb
+        order = {"p": cache.ksp, "q": cache.ksq}
+        self.order = order
+        idx = np.arange(n_orb, dtype=np.int32)
+        self.self_pairs = (dv.i32(idx), dv.i32(idx))
+        ks_all = sorted({k for t in self.targets for k in tab.k_for_target[t]})
+        self.work = []
+        for (ii, jj), block in _BLOCKS:
+            for k in ks_all:
+                tk = [t for t in self.targets if k in tab.k_for_target[t]]
+                tdir = [t for t in tk if np.any(tab.direct[t, :, k])]
+                cd = None
+                if tdir:
+                    coef = np.zeros((len(tdir), n_orb))
+                    for r, t in enumerate(tdir):
+                        c = tab.direct[t, :, k].copy()
+                        c[t] = c[t] + c[t]
+                        coef[r] = c
+                    cd = dv.f64(coef)
+                tex = []
+                ce = None
+                if not (self.skip and ii == 1 and jj == 2):
+                    tex = [t for t in tk if np.any(tab.exchange[t, :, k])]
+                    if tex:
+                        ce = dv.f64(np.stack([tab.exchange[t, :, k] for t in tex]))
+                self.work.append((ii, jj, block, k, tdir, cd, tex, ce))
+        self.divisor = {t: float(tab.divisor(t)) for t in self.targets}
+        # build every bank / factor the launches need, outside any capture
+        for ii, jj, block, k, tdir, cd, tex, ce in self.work:
+            if tdir:
+                if cache.ns >= _semi_direct():
+                    cache.factors_device(k, block)
+                else:
+                    cache.bank_device(k, block)
+            if tex:
+                if cache.ns >= _semi_exchange(block):
+                    cache.factors_device(k, block)
+                else:
+                    cache.bank_device(k, block)
+
+    def run(self, P, Q):
+        fb, cache, ns, n_orb = self.fb, self.fb.cache, self.ns, self.n_orb
+        order = self.order
+        s = dv.stream_handle()
+        pa, pb = self.self_pairs
+        dens = {}
+        for c, M in (("p", P), ("q", Q)):
+            d = dv.empty((n_orb, ns, order[c]))
+            call("ax_density_sym", dv.ptr(M), dv.ld(M), dv.ptr(pa), dv.ptr(pb), n_orb, ns, ns, ns,
+                 order[c], dv.ptr(d), s)
+            dens[c] = d
+        H = {t: [dv.zeros((ns, ns)) for _ in range(4)] for t in self.targets}
+        for ii, jj, block, k, tdir, cd, tex, ce in self.work:
+            c_ket = "q" if jj == 2 else "p"
+            c_bra = "q" if ii == 2 else "p"
+            ks_bra, ks_ket = order[c_bra], order[c_ket]
+            direct_slot = 0 if ii == 1 else 1
+            exch_slot = {(1, 1): 0, (1, 2): 2, (2, 1): 3, (2, 2): 1}[(ii, jj)]
+            if tdir:
+                dd = dv.empty((len(tdir), ns, ks_ket))
+                call("ax_combine_rows", dv.ptr(dens[c_ket]), ns * ks_ket, n_orb, dv.ptr(cd),
+                     len(tdir), dv.ptr(dd), s)
+                v = cache.case1(k, block, dd)
+                for r, t in enumerate(tdir):
+                    call("ax_banded_to_full_add", dv.ptr(H[t][direct_slot]), ns * ns,
+                         dv.ptr(v[r]), ns * ks_bra, 1, ns, ks_bra, 1.0, s)
+            if tex:
+                cii = Q if ii == 2 else P
+                cjj = Q if jj == 2 else P
+                xx = dv.empty((len(tex), ns, ns))
+                call("ax_exchange_density", dv.ptr(cii), dv.ptr(cjj), ns, n_orb, ns, dv.ptr(ce),
+                     len(tex), dv.ptr(xx), s)
+                kk = cache.case4(k, block, xx, symmetric=(ii == jj))
+                for r, t in enumerate(tex):
+                    axpy(H[t][exch_slot], kk[r])
+        for t in self.targets:
+            pp, qq, pq, qp = H[t]
+            call("ax_fock_finalize", dv.ptr(pp), dv.ptr(qq), dv.ptr(pq), dv.ptr(qp), ns,
+                 self.divisor[t], 1 if self.skip else 0, s)
+        return H
+
+    def rk_equivalents(self):
+        """(target, source, k, block) contractions with a nonzero coefficient,
+        direct and exchange counted separately."""
+        tab = self.fb.tables
+        n = 0
+        for ii, jj, block, k, tdir, cd, tex, ce in self.work:
+            n += sum(int(np.count_nonzero(tab.direct[t, :, k])) for t in tdir)
+            n += sum(int(np.count_nonzero(tab.exchange[t, :, k])) for t in tex)
+        return n
+
+
+def _semi_direct():
+    from .convol import SEMISEPARABLE_DIRECT_NS
+    return SEMISEPARABLE_DIRECT_NS
+
+
+def _semi_exchange(block):
+    from .convol import SEMISEPARABLE_EXCHANGE_NS, SEMISEPARABLE_MIXED_EXCHANGE_NS
+    return SEMISEPARABLE_EXCHANGE_NS if block in ("pppp", "qqqq") else SEMISEPARABLE_MIXED_EXCHANGE_NS

@@ -18,7 +18,10 @@ from .workloads import pair_list, valid_pairs
 
 
 class SlaterTensor:
-    def __init__(self, gr, grw, ls, kmax, precision="fp64"):
+    def __init__(self, gr, grw, ls, kmax, precision="fp64", pairs=None, n_fn=None):
+        """All-pairs plan of orbitals with angular momenta ``ls`` (angular
+        rule per rank), or, with ``pairs=(pa, pb)`` and ``n_fn`` functions, a
+        pair-product plan of that explicit list valid at every rank."""
         dv.require_gpu()
         lib = load()
         gr = np.ascontiguousarray(gr, dtype=np.float64)
@@ -29,23 +32,38 @@ class SlaterTensor:
             raise ValueError(f"precision must be one of {tuple(PRECISIONS)}")
         self.nv, self.ng = gr.shape
         self.n = self.nv * self.ng
-        self.ls = np.ascontiguousarray(ls, dtype=np.int32)
-        self.n_orb = int(self.ls.size)
         self.kmax = int(kmax)
         self.precision = precision
         self._grw = grw.ravel().copy()
         h = ctypes.c_void_p()
         dp = ctypes.POINTER(ctypes.c_double)
-        call("ax_rk_tensor_create", gr.ravel().ctypes.data_as(dp), grw.ravel().ctypes.data_as(dp),
-             self.nv, self.ng, self.n_orb,
-             self.ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), self.kmax,
-             PRECISIONS[precision], ctypes.byref(h))
+        ip = ctypes.POINTER(ctypes.c_int32)
+        if pairs is None:
+            self.ls = np.ascontiguousarray(ls, dtype=np.int32)
+            self.n_orb = int(self.ls.size)
+            self._pairs = None
+            call("ax_rk_tensor_create", gr.ravel().ctypes.data_as(dp),
+                 grw.ravel().ctypes.data_as(dp), self.nv, self.ng, self.n_orb,
+                 self.ls.ctypes.data_as(ip), self.kmax, PRECISIONS[precision], ctypes.byref(h))
+        else:
+            pa = np.ascontiguousarray(pairs[0], dtype=np.int32)
+            pb = np.ascontiguousarray(pairs[1], dtype=np.int32)
+            if pa.shape != pb.shape or pa.ndim != 1:
+                raise ValueError("pairs must be two equal-length index vectors")
+            self.ls = None
+            self.n_orb = int(n_fn)
+            self._pairs = (pa, pb)
+            call("ax_rk_tensor_create_pairs", gr.ravel().ctypes.data_as(dp),
+                 grw.ravel().ctypes.data_as(dp), self.nv, self.ng, self.n_orb, int(pa.size),
+                 pa.ctypes.data_as(ip), pb.ctypes.data_as(ip), self.kmax, PRECISIONS[precision],
+                 ctypes.byref(h))
         self._h = h
         self._lib = lib
         self.size = int(lib.ax_rk_tensor_size(h))
         self.m = [int(lib.ax_rk_tensor_pairs(h, k)) for k in range(self.kmax + 1)]
         self.offsets = [int(lib.ax_rk_tensor_offset(h, k)) for k in range(self.kmax + 1)]
         self.footprint = int(lib.ax_rk_tensor_footprint(h))
+        self.slices = int(lib.ax_rk_tensor_slices(h))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -107,6 +125,8 @@ class SlaterTensor:
 
     # -- host-side views of the packed output -----------------------------
     def rank_pairs(self, k):
+        if self._pairs is not None:
+            return np.arange(self._pairs[0].size, dtype=np.int64)
         return valid_pairs(self.ls, k)
 
     def quartets(self, k):
@@ -124,6 +144,10 @@ class SlaterTensor:
         M[i, j] = vals
         M[j, i] = vals
         return M
+
+    def pair_indices(self):
+        """(pa, pb): the functions of every pair, in output order."""
+        return self._pairs if self._pairs is not None else pair_list(self.n_orb)
 
     @staticmethod
     def pairs(n_orb):

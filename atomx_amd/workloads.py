@@ -165,3 +165,87 @@ def serving_stream(n_requests, rate=200.0, seed=0, fock_fraction=0.2, precision=
             req = SlaterRequest(gr, grw, P, Q, ls, kmax, precision=precision)
         out.append((t, req))
     return out
+
+
+# ---------------------------------------------------------------------------
+# CI / CCSD B-spline product workload
+# ---------------------------------------------------------------------------
+def bspline_product_workload(ns=120, ks=8, r_min=1e-5, r_max=50.0, ng=8):
+    """B-splines of order ``ks`` on make_exponential_knots(ns, ks, r_min, r_max)
+    sampled on the Gauss grid, and the band pairs (i, i+s), s < ks, i ascending
+    then s ascending.  Returns (gr, grw, B (ns, nv*ng), pa, pb)."""
+    from .bspline import evaluate_bsplines_on_grid
+    t = make_exponential_knots(ns, ks, r_min, r_max)
+    gr, grw = setup_gauss_points(t, ns, ks, ng)
+    nv = ns - ks + 1
+    bsp = evaluate_bsplines_on_grid(t, ns, ks, gr)[:, :, :, 0]  # (nv, ng, ks)
+    B = np.zeros((ns, nv * ng))
+    for iv in range(nv):
+        for i in range(ks):
+            B[iv + i, iv * ng:(iv + 1) * ng] = bsp[iv, :, i]
+    pa, pb = [], []
+    for i in range(ns):
+        for s in range(ks):
+            if i + s < ns:
+                pa.append(i)
+                pb.append(i + s)
+    return gr, grw, B, np.asarray(pa, np.int32), np.asarray(pb, np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Batched uneven SCF: the 26 charge states of iron
+# ---------------------------------------------------------------------------
+# relativistic subshells in filling order: (label, kappa, capacity)
+_FE_SHELLS = (("1s", -1, 2), ("2s", -1, 2), ("2p-", 1, 2), ("2p+", -2, 4), ("3s", -1, 2),
+              ("3p-", 1, 2), ("3p+", -2, 4), ("3d-", 2, 4), ("3d+", -3, 6), ("4s", -1, 2))
+
+
+def fe_configuration(charge):
+    """Ground configuration of Fe^{charge+} as [(label, kappa, occupation)]:
+    neutral and singly ionised iron keep 4s (3d^6 4s^2, 3d^6 4s^1); from
+    Fe^2+ on the 3d shell holds what is left above the argon core; below it
+    the shells fill in order (an average-of-configuration occupation per
+    relativistic subshell, j-subshells filled lower j first)."""
+    n_e = 26 - int(charge)
+    if not 1 <= n_e <= 26:
+        raise ValueError("charge must be in 0..25")
+    four_s = {0: 2, 1: 1}.get(int(charge), 0)
+    rest = n_e - four_s
+    out = []
+    for label, kappa, cap in _FE_SHELLS[:-1]:
+        if rest <= 0:
+            break
+        occ = min(cap, rest)
+        out.append((label, kappa, float(occ)))
+        rest -= occ
+    if four_s:
+        out.append(("4s", -1, float(four_s)))
+    return out
+
+
+def fe_ion_workload(seed=0, ns_range=(40, 160), ks_choices=(7, 8), r_min=1e-4, r_max=30.0):
+    """The 26 iron ions of the batched uneven Fock build: per ion a basis of
+    ns ~ U{40..160} B-splines of order ks ~ U{7, 8} (kinetic balance, P in
+    order ks-1) on make_exponential_knots(ns, ks, r_min, r_max), the ground
+    configuration's subshells, and seeded smooth coefficient vectors
+    (decaying Gaussian noise, unit norm).  Returns a list of dicts with keys
+    charge, bsp_data, kappas, occupations, P, Q."""
+    from .bspline import prepare_bspline_data
+    rng = np.random.default_rng(seed)
+    ions = []
+    for charge in range(26):
+        ns = int(rng.integers(ns_range[0], ns_range[1] + 1))
+        ks = int(rng.choice(ks_choices))
+        t = make_exponential_knots(ns, ks, r_min, r_max)
+        bd = prepare_bspline_data(t, ns, ks, kinetic_balance=True)
+        conf = fe_configuration(charge)
+        n_orb = len(conf)
+        decay = np.exp(-np.arange(bd["ns"]) / max(bd["ns"] / 6.0, 1.0))
+        P = rng.normal(size=(n_orb, bd["ns"])) * decay
+        Q = 0.05 * rng.normal(size=(n_orb, bd["ns"])) * decay
+        P[:, bd["nsp"]:] = 0.0
+        norm = np.sqrt((P * P).sum(axis=1) + (Q * Q).sum(axis=1))
+        ions.append({"charge": charge, "bsp_data": bd, "kappas": [c[1] for c in conf],
+                     "occupations": [c[2] for c in conf], "P": P / norm[:, None],
+                     "Q": Q / norm[:, None]})
+    return ions

@@ -167,6 +167,17 @@ ax_status ax_rk_tensor_create(const double *r_host, const double *w_host, int nv
                               int n_orb, const int32_t *l_host, int kmax, int precision,
                               ax_rk_tensor **plan);
 ax_status ax_rk_tensor_destroy(ax_rk_tensor *plan);
+/* Pair-product tensor (the CI / CCSD B-spline product workload): n_fn
+ * functions sampled on the grid (P rows; Q may be NULL), an explicit list of
+ * n_pairs pairs (pa_host[p], pb_host[p]), every pair at every rank
+ * k = 0..kmax (no angular rule).  Output as ax_rk_tensor_create: per k the
+ * packed upper triangle over all pairs in list order.  R^k(B_i B_j; B_k B_l)
+ * of numerics/bspline/convol.py:323-380 (_assemble_rkb_sym) in the (Y)
+ * discretisation. */
+ax_status ax_rk_tensor_create_pairs(const double *r_host, const double *w_host, int nv, int ng,
+                                    int n_fn, int n_pairs, const int32_t *pa_host,
+                                    const int32_t *pb_host, int kmax, int precision,
+                                    ax_rk_tensor **plan);
 /* Device bytes ax_rk_tensor_create would hold for this shape (admission by
  * HBM footprint before anything is allocated); -1 on invalid arguments. */
 int64_t ax_rk_tensor_estimate(int nv, int ng, int n_orb, const int32_t *l_host, int kmax,
@@ -178,12 +189,17 @@ int ax_rk_tensor_pairs(const ax_rk_tensor *plan, int k);
 int64_t ax_rk_tensor_offset(const ax_rk_tensor *plan, int k);
 /* device bytes held by the plan (admission accounting) */
 int64_t ax_rk_tensor_footprint(const ax_rk_tensor *plan);
+/* int8 slices per operand of a split-precision plan, from the radius ratio
+ * rho one 64-node K-tile spans (cells after the first) and
+ * bits = (kmax+1) log2(rho): 7 up to 5 bits, 8 up to 12 bits; beyond that
+ * (and for FP64 plans) 0: the plan contracts in FP64. */
+int ax_rk_tensor_slices(const ax_rk_tensor *plan);
 ax_status ax_rk_tensor_run(ax_rk_tensor *plan, const double *P, const double *Q, double *out,
                            void *stream);
-/* Several plans (any grids, orbital counts and k ranges; one precision
- * mode) as one set of launches: density, Y^k scan (one per n_gauss value),
- * slicing, contraction and reduction each run once for the whole batch, in
- * chunks of 32 plans.  P, Q and out are host arrays of n_plans device
+/* Several plans (any grids, orbital counts, k ranges and precision modes)
+ * as one set of launches: density, Y^k scan (one per n_gauss value),
+ * slicing, contraction (one per contraction kind) and reduction each run
+ * once for the whole batch, in chunks of 16 plans.  P, Q and out are host arrays of n_plans device
  * pointers (Q may be NULL, or hold NULL entries, for no small component).
  * Each plan's results are bitwise those of its own ax_rk_tensor_run: tiles,
  * K splits and summation order are fixed per plan. */

@@ -166,3 +166,77 @@ def test_schedule_tuner(problems):
         with pytest.raises(ValueError):
             plan.set_schedule("tile-major", 64)
         plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_pair_product_plan_matches_oracle(precision):
+    """Explicit band pairs of B-spline products, every pair at every rank
+    (the CI / CCSD product workload at a small size)."""
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor, run_batch
+    from atomx_amd.workloads import bspline_product_workload
+    gr, grw, B, pa, pb = bspline_product_workload(ns=30, ks=6, r_max=40.0)
+    kmax = 3
+    plan = SlaterTensor(gr, grw, None, kmax, precision=precision, pairs=(pa, pb),
+                        n_fn=B.shape[0])
+    out = dv.host(plan.run(B, None))
+    nv, ng = gr.shape
+    rho = B[pa] * B[pb]
+    w = grw.ravel()
+    mo, mi = O.yk_partial_matrices(ng)
+    # B_0 .. B_{ks-1} are nonzero in the first cell [0, r_min], where r^k spans
+    # many decades across the cell's Gauss nodes and the partial-interval rule
+    # of the Y^k scan cancels heavily: any FP64 summation order (the
+    # reference's included) is uncertain there at ~1e-12 of sum|D||Y| for
+    # k >= 2.  Quartets away from the origin are held to 1e-13, quartets
+    # touching the first cell to 1e-9.  On this 25-cell grid one K-tile spans
+    # ~29 bits of r^-(k+1), beyond the split error bound, so a split plan
+    # contracts in FP64 (slices == 0).
+    tol = 1e-13
+    ks = 6
+    assert plan.slices == 0
+    for k in range(kmax + 1):
+        assert plan.m[k] == len(pa)
+        p, q = plan.quartets(k)
+        ref = O.rk_quartets(rho, gr, grw, nv, ng, p, q, np.full(len(p), k))
+        got = out[plan.offsets[k]:plan.offsets[k] + len(p)]
+        ys = {j: np.abs(O.yk_flat(rho[j], gr.ravel(), w, w.size, k, nv, ng, mo, mi))
+              for j in set(q.tolist())}
+        scale = np.array([np.sum(np.abs(w * rho[i]) * ys[j]) for i, j in zip(p, q)])
+        rel = np.abs(got - ref) / scale
+        origin = (pa[p] < ks) | (pa[q] < ks)
+        err, err0 = float(np.max(rel[~origin])), float(np.max(rel[origin]))
+        print(f"{precision} k={k}: max err / sum|D||Y| = {err:.2e} (first-cell quartets {err0:.2e})")
+        assert err < tol, k
+        assert err0 < 1e-9, k
+    # in a batch with an all-pairs plan: bitwise the same
+    gr2, grw2, P2, Q2, ls2 = _problem(50, 8, 5, 9)
+    other = SlaterTensor(gr2, grw2, ls2, 2, precision=precision)
+    o1, o2 = run_batch([plan, other], [dv.f64(B), dv.f64(P2)], [None, dv.f64(Q2)])
+    assert np.array_equal(dv.host(o1), out)
+    assert np.array_equal(dv.host(o2), dv.host(other.run(P2, Q2)))
+    plan.close()
+    other.close()
+
+
+def test_slice_count_follows_the_grid():
+    """7 slices where a 64-node K-tile spans a small radius ratio (the
+    400-cell synthetic grid), 8 on coarser grids (200 cells to 60 a.u. at
+    k <= 8, the 113-cell B-spline product grid at k <= 6), and an FP64
+    contraction where even 8 slices would not meet the bound (100 cells at
+    k <= 8)."""
+    from atomx_amd.tensor import SlaterTensor
+    from atomx_amd.workloads import bspline_product_workload, synthetic_grid
+    ls = np.array([0, 1, 2], dtype=np.int32)
+    for nv, expect in ((400, 7), (200, 8), (100, 0)):
+        gr, grw = synthetic_grid(nv=nv, ng=8)
+        pl = SlaterTensor(gr, grw, ls, 8, precision="split")
+        assert pl.slices == expect, nv
+        pl.close()
+        pl = SlaterTensor(gr, grw, ls, 8, precision="fp64")
+        assert pl.slices == 0
+        pl.close()
+    gr, grw, B, pa, pb = bspline_product_workload()
+    pl = SlaterTensor(gr, grw, None, 6, precision="split", pairs=(pa, pb), n_fn=B.shape[0])
+    assert pl.slices == 8
+    pl.close()
