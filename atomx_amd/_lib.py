@@ -89,6 +89,9 @@ SIGNATURES = {
     "ax_exchange_density": (_int, [_vp, _vp, _i64, _int, _int, _vp, _int, _vp, _vp]),
     "ax_convol_case1": (_int, [_vp, _int, _int, _int, _vp, _i64, _int, _vp, _i64, _vp]),
     "ax_convol_case2": (_int, [_vp, _int, _int, _int, _int, _vp, _i64, _int, _vp, _i64, _vp]),
+    "ax_convol_case4_acc": (_int, [_vp, _int, _int, _int, _vp, _i64, _int, _int, _vp, _vp, _vp]),
+    "ax_banded_to_full_add_at": (_int, [_vp, _vp, _vp, _i64, _int, _int, _int, ctypes.c_double,
+                                        _vp]),
     "ax_convol_case4": (_int, [_vp, _int, _int, _int, _vp, _i64, _int, _int, _vp, _i64, _vp]),
     "ax_convol_semi_workspace": (_i64, [_int, _int, _int]),
     "ax_convol_case1_semi": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _int, _int, _vp, _i64, _int,

@@ -414,105 +414,144 @@ __global__ void case2_kernel(const double *__restrict__ bank, int ns, int ksl, i
 }
 
 // ---- (a16) case 4 exchange: four gather families in the order of
-// _convol_case4_t (convol.py:1000-1058); symmetric computes col <= row and
-// mirrors.  One thread per output element, nb densities per thread.
-__global__ void case4_kernel(const double *__restrict__ bank, int ns, int ksl, int ksr,
-                             const double *__restrict__ d, int64_t ld_d, int nb, int symmetric,
-                             double *__restrict__ out, int64_t ld_o) {
-    const int64_t total = (int64_t)ns * ns;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int row = static_cast<int>(t / ns);
-        const int col = static_cast<int>(t - (int64_t)row * ns);
-        if (symmetric && col > row) continue;
+// _convol_case4_t (convol.py:1000-1058), one thread per (density b, output
+// element); `symmetric` computes the lower triangle col <= row and mirrors
+// it on store.  With dst_off the result is added to out + dst_off[b] (one
+// addition per element, as the accumulation of a separate result would do);
+// without, out + b*ld_o is overwritten.
+__device__ inline int tri_row(int64_t e) {
+    int r = static_cast<int>((sqrt(8.0 * static_cast<double>(e) + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(r + 1) * (r + 2) / 2 <= e) ++r;
+    while ((int64_t)r * (r + 1) / 2 > e) --r;
+    return r;
+}
+
+__global__ void case4_bt_kernel(const double *__restrict__ bank, int ns, int ksl, int ksr,
+                                const double *__restrict__ d, int64_t ld_d, int nb, int symmetric,
+                                double *__restrict__ out, int64_t ld_o,
+                                const int64_t *__restrict__ dst_off) {
+    const int64_t per = symmetric ? (int64_t)ns * (ns + 1) / 2 : (int64_t)ns * ns;
+    const int64_t total = per * nb;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int b = static_cast<int>(u / per);
+        const int64_t e = u - (int64_t)b * per;
+        int row, col;
+        if (symmetric) {
+            row = tri_row(e);
+            col = static_cast<int>(e - (int64_t)row * (row + 1) / 2);
+        } else {
+            row = static_cast<int>(e / ns);
+            col = static_cast<int>(e - (int64_t)row * ns);
+        }
+        const double *db = d + (int64_t)b * ld_d;
 #define BANK(ip_, jp_, i_, j_) bank[((((int64_t)(ip_)) * ksr + (jp_)) * ns + (i_)) * ns + (j_)]
-        double acc[kMaxBatch];
-#pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) acc[b] = 0.0;
-        // family 1: bank[ip][jp][row-ip][col] * d[row-ip][col+jp]
-        for (int ip = 0; ip < ksl; ++ip) {
+        double acc = 0.0;
+        for (int ip = 0; ip < ksl; ++ip) {  // family 1
             const int i = row - ip;
             if (i < 0) continue;
             for (int jp = 0; jp < ksr; ++jp) {
                 if (col >= ns - jp) continue;
-                const double bv = BANK(ip, jp, i, col);
-#pragma unroll
-                for (int b = 0; b < kMaxBatch; ++b)
-                    if (b < nb) acc[b] += bv * d[(int64_t)b * ld_d + (int64_t)i * ns + col + jp];
+                acc += BANK(ip, jp, i, col) * db[(int64_t)i * ns + col + jp];
             }
         }
-        // family 2: bank[ip][jp][row][col] * d[row+ip][col+jp]
-        for (int ip = 1; ip < ksl; ++ip) {
+        for (int ip = 1; ip < ksl; ++ip) {  // family 2
             const int ii = row + ip;
             if (ii >= ns) continue;
             for (int jp = 0; jp < ksr; ++jp) {
                 if (col >= ns - jp) continue;
-                const double bv = BANK(ip, jp, row, col);
-#pragma unroll
-                for (int b = 0; b < kMaxBatch; ++b)
-                    if (b < nb) acc[b] += bv * d[(int64_t)b * ld_d + (int64_t)ii * ns + col + jp];
+                acc += BANK(ip, jp, row, col) * db[(int64_t)ii * ns + col + jp];
             }
         }
-        // family 3: bank[ip][jp][row-ip][col-jp] * d[row-ip][col-jp]
-        for (int jp = 1; jp < ksr; ++jp) {
+        for (int jp = 1; jp < ksr; ++jp) {  // family 3
             if (col < jp) continue;
             for (int ip = 0; ip < ksl; ++ip) {
                 const int i = row - ip;
                 if (i < 0) continue;
-                const double bv = BANK(ip, jp, i, col - jp);
-#pragma unroll
-                for (int b = 0; b < kMaxBatch; ++b)
-                    if (b < nb) acc[b] += bv * d[(int64_t)b * ld_d + (int64_t)i * ns + col - jp];
+                acc += BANK(ip, jp, i, col - jp) * db[(int64_t)i * ns + col - jp];
             }
         }
-        // family 4: bank[ip][jp][row][col-jp] * d[row+ip][col-jp]
-        for (int ip = 1; ip < ksl; ++ip) {
+        for (int ip = 1; ip < ksl; ++ip) {  // family 4
             const int ii = row + ip;
             if (ii >= ns) continue;
             for (int jp = 1; jp < ksr; ++jp) {
                 if (col < jp) continue;
-                const double bv = BANK(ip, jp, row, col - jp);
-#pragma unroll
-                for (int b = 0; b < kMaxBatch; ++b)
-                    if (b < nb) acc[b] += bv * d[(int64_t)b * ld_d + (int64_t)ii * ns + col - jp];
+                acc += BANK(ip, jp, row, col - jp) * db[(int64_t)ii * ns + col - jp];
             }
         }
 #undef BANK
-#pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) {
-            if (b >= nb) continue;
-            out[(int64_t)b * ld_o + (int64_t)row * ns + col] = acc[b];
-            if (symmetric && col < row) out[(int64_t)b * ld_o + (int64_t)col * ns + row] = acc[b];
+        double *o = dst_off ? out + dst_off[b] : out + (int64_t)b * ld_o;
+        if (dst_off) {
+            o[(int64_t)row * ns + col] += acc;
+            if (symmetric && col < row) o[(int64_t)col * ns + row] += acc;
+        } else {
+            o[(int64_t)row * ns + col] = acc;
+            if (symmetric && col < row) o[(int64_t)col * ns + row] = acc;
         }
+    }
+}
+
+// banded -> full add into out + dst_off[b], one thread per band entry (lo, s)
+__global__ void banded_to_full_at_kernel(double *__restrict__ V, const int64_t *__restrict__ dst_off,
+                                         const double *__restrict__ vb, int64_t ldb, int nb,
+                                         int ns, int ks, double coeff) {
+    const int64_t per = (int64_t)ns * ks;
+    const int64_t total = per * nb;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int b = static_cast<int>(u / per);
+        const int64_t e = u - (int64_t)b * per;
+        const int lo = static_cast<int>(e / ks), sft = static_cast<int>(e - (int64_t)lo * ks);
+        if (lo + sft >= ns) continue;
+        const double val = coeff * vb[(int64_t)b * ldb + e];
+        double *Vb = V + dst_off[b];
+        Vb[(int64_t)lo * ns + lo + sft] += val;
+        if (sft > 0) Vb[(int64_t)(lo + sft) * ns + lo] += val;
     }
 }
 
 // ---- (a17) semiseparable direct contraction (convol.py:1125-1162) --------------
 // pass 1: per (b, jp) sequential prefix/suffix over j into scratch
-__global__ void semi_scan_kernel(const double *__restrict__ rising_r,
-                                 const double *__restrict__ falling_r, int ns, int ksr,
-                                 const double *__restrict__ d, int64_t ld_d, int nb,
-                                 double *__restrict__ prefix, double *__restrict__ suffix) {
+// One wave per (b, jp) chain: the chain's factor rows and density column are
+// staged in LDS by all lanes, then lane 0 runs the prefix and suffix sums
+// in the reference's sequential order.
+__global__ __launch_bounds__(64) void semi_scan_kernel(const double *__restrict__ rising_r,
+                                                       const double *__restrict__ falling_r,
+                                                       int ns, int ksr,
+                                                       const double *__restrict__ d, int64_t ld_d,
+                                                       int nb, double *__restrict__ prefix,
+                                                       double *__restrict__ suffix) {
+    extern __shared__ double sh[];
+    double *r_s = sh, *f_s = sh + ns, *d_s = sh + 2 * ns;
     const int total = nb * ksr;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int b = t / ksr, jp = t - (t / ksr) * ksr;
-        double *pre = prefix + ((int64_t)b * ksr + jp) * (ns + 1);
-        double *suf = suffix + ((int64_t)b * ksr + jp) * (ns + 1);
         const double *db = d + (int64_t)b * ld_d;
-        const int end = ns - jp;
-        double s = 0.0;
-        pre[0] = 0.0;
-        for (int j = 0; j < end; ++j) {
-            s = s + rising_r[(int64_t)jp * ns + j] * db[(int64_t)j * ksr + jp];
-            pre[j + 1] = s;
+        for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+            r_s[j] = rising_r[(int64_t)jp * ns + j];
+            f_s[j] = falling_r[(int64_t)jp * ns + j];
+            d_s[j] = db[(int64_t)j * ksr + jp];
         }
-        for (int j = end + 1; j <= ns; ++j) pre[j] = s;
-        s = 0.0;
-        for (int j = ns; j >= end; --j) suf[j] = 0.0;
-        for (int j = end - 1; j >= 0; --j) {
-            s = s + falling_r[(int64_t)jp * ns + j] * db[(int64_t)j * ksr + jp];
-            suf[j] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double *pre = prefix + ((int64_t)b * ksr + jp) * (ns + 1);
+            double *suf = suffix + ((int64_t)b * ksr + jp) * (ns + 1);
+            const int end = ns - jp;
+            double s = 0.0;
+            pre[0] = 0.0;
+            for (int j = 0; j < end; ++j) {
+                s = s + r_s[j] * d_s[j];
+                pre[j + 1] = s;
+            }
+            for (int j = end + 1; j <= ns; ++j) pre[j] = s;
+            s = 0.0;
+            for (int j = ns; j >= end; --j) suf[j] = 0.0;
+            for (int j = end - 1; j >= 0; --j) {
+                s = s + f_s[j] * d_s[j];
+                suf[j] = s;
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -920,13 +959,38 @@ ax_status ax_convol_case4(const double *bankT, int ns, int ksl, int ksr, const d
     AX_REQUIRE(ns > 0 && ksl > 0 && ksr > 0 && nb >= 0 && ld_d >= (int64_t)ns * ns &&
                    ld_o >= (int64_t)ns * ns,
                "ax_convol_case4: bad sizes");
-    for (int b0 = 0; b0 < nb; b0 += kMaxBatch) {
-        const int cnt = nb - b0 < kMaxBatch ? nb - b0 : kMaxBatch;
-        hipLaunchKernelGGL(case4_kernel, dim3(blocks((int64_t)ns * ns)), dim3(kT), 0,
-                           ax::as_stream(stream), bankT, ns, ksl, ksr, d + (int64_t)b0 * ld_d, ld_d,
-                           cnt, symmetric, out + (int64_t)b0 * ld_o, ld_o);
-        AX_CHECK_LAUNCH();
-    }
+    if (nb == 0) return AX_OK;
+    const int64_t per = symmetric ? (int64_t)ns * (ns + 1) / 2 : (int64_t)ns * ns;
+    hipLaunchKernelGGL(case4_bt_kernel, dim3(blocks(per * nb)), dim3(kT), 0, ax::as_stream(stream),
+                       bankT, ns, ksl, ksr, d, ld_d, nb, symmetric, out, ld_o, nullptr);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_convol_case4_acc(const double *bankT, int ns, int ksl, int ksr, const double *d,
+                              int64_t ld_d, int nb, int symmetric, double *out_base,
+                              const int64_t *dst_off, void *stream) {
+    AX_REQUIRE(ns > 0 && ksl > 0 && ksr > 0 && nb >= 0 && ld_d >= (int64_t)ns * ns,
+               "ax_convol_case4_acc: bad sizes");
+    if (nb == 0) return AX_OK;
+    AX_REQUIRE(out_base && dst_off, "ax_convol_case4_acc: null destination");
+    const int64_t per = symmetric ? (int64_t)ns * (ns + 1) / 2 : (int64_t)ns * ns;
+    hipLaunchKernelGGL(case4_bt_kernel, dim3(blocks(per * nb)), dim3(kT), 0, ax::as_stream(stream),
+                       bankT, ns, ksl, ksr, d, ld_d, nb, symmetric, out_base, (int64_t)0, dst_off);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_banded_to_full_add_at(double *V_base, const int64_t *dst_off, const double *vb,
+                                   int64_t ldb, int nb, int ns, int ks, double coeff,
+                                   void *stream) {
+    AX_REQUIRE(nb >= 0 && ns > 0 && ks > 0 && ldb >= (int64_t)ns * ks,
+               "ax_banded_to_full_add_at: bad sizes");
+    if (nb == 0) return AX_OK;
+    AX_REQUIRE(V_base && dst_off, "ax_banded_to_full_add_at: null destination");
+    hipLaunchKernelGGL(banded_to_full_at_kernel, dim3(blocks((int64_t)nb * ns * ks)), dim3(kT), 0,
+                       ax::as_stream(stream), V_base, dst_off, vb, ldb, nb, ns, ks, coeff);
+    AX_CHECK_LAUNCH();
     return AX_OK;
 }
 
@@ -944,8 +1008,11 @@ ax_status ax_convol_case1_semi(const double *near, const double *rl, const doubl
     if (nb == 0) return AX_OK;
     double *prefix = workspace;
     double *suffix = workspace + (int64_t)nb * ksr * (ns + 1);
-    hipLaunchKernelGGL(semi_scan_kernel, dim3(ax::grid_for(nb * ksr, 64, 1024)), dim3(64), 0,
-                       ax::as_stream(stream), rr, fr, ns, ksr, d, ld_d, nb, prefix, suffix);
+    AX_REQUIRE((size_t)3 * ns * sizeof(double) <= 64 * 1024,
+               "ax_convol_case1_semi: ns=%d exceeds the LDS staging of the scan", ns);
+    hipLaunchKernelGGL(semi_scan_kernel, dim3(nb * ksr < 65535 ? nb * ksr : 65535), dim3(64),
+                       (size_t)3 * ns * sizeof(double), ax::as_stream(stream), rr, fr, ns, ksr, d,
+                       ld_d, nb, prefix, suffix);
     AX_CHECK_LAUNCH();
     hipLaunchKernelGGL(semi_case1_kernel, dim3(blocks((int64_t)nb * ns * ksl)), dim3(kT), 0,
                        ax::as_stream(stream), near, rl, fl, ns, ksl, ksr, d, ld_d, nb, prefix,

@@ -100,31 +100,42 @@ class FockStep:
 
     ``run(P, Q)`` takes device coefficient stacks (n_orb, ns) and returns the
     device blocks of every target; the launches follow
-    methods/dirac_fock.py:1128-1319 in the order of ``FockBuilder``: per
-    component block and rank, one row combination of the direct densities of
-    all targets (own density doubled), one case-1 contraction, banded->full
-    adds per target; one exchange-density launch, one case-4 contraction and
-    an accumulation per target; finally symmetrisation and division by the
-    occupation."""
+    methods/dirac_fock.py:1128-1319: per component block and rank, one row
+    combination of the direct densities of all targets (own density
+    doubled), one case-1 contraction and one banded->full add into every
+    target's block; one exchange-density launch and one case-4 contraction
+    accumulated into every target's block (the bank path; the semiseparable
+    path at ns >= 320 adds per target); finally symmetrisation and division
+    by the occupation.  The blocks live in one resident tensor
+    (n_targets, 4, ns, ns): PP, QQ, PQ, QP."""
 
     def __init__(self, fb, targets, skip_pq_exchange):
         tab, cache = fb.tables, fb.cache
         self.fb = fb
         self.ns, self.n_orb = fb.ns, fb.n_orb
+        ns = self.ns
         self.targets = list(range(fb.n_orb)) if targets is None else [int(t) for t in targets]
+        tpos = {t: i for i, t in enumerate(self.targets)}
         self.skip = bool(skip_pq_exchange)
         n_orb = self.n_orb
         order = {"p": cache.ksp, "q": cache.ksq}
         self.order = order
         idx = np.arange(n_orb, dtype=np.int32)
         self.self_pairs = (dv.i32(idx), dv.i32(idx))
+        self.H = dv.zeros((len(self.targets), 4, ns, ns))
+
+        def offsets(ts, slot):
+            return dv.to_dev(np.array([(tpos[t] * 4 + slot) * ns * ns for t in ts], np.int64))
+
         ks_all = sorted({k for t in self.targets for k in tab.k_for_target[t]})
         self.work = []
         for (ii, jj), block in _BLOCKS:
+            direct_slot = 0 if ii == 1 else 1
+            exch_slot = {(1, 1): 0, (1, 2): 2, (2, 1): 3, (2, 2): 1}[(ii, jj)]
             for k in ks_all:
                 tk = [t for t in self.targets if k in tab.k_for_target[t]]
                 tdir = [t for t in tk if np.any(tab.direct[t, :, k])]
-                cd = None
+                cd = od = None
                 if tdir:
                     coef = np.zeros((len(tdir), n_orb))
                     for r, t in enumerate(tdir):
@@ -132,23 +143,25 @@ class FockStep:
                         c[t] = c[t] + c[t]
                         coef[r] = c
                     cd = dv.f64(coef)
+                    od = offsets(tdir, direct_slot)
                 tex = []
-                ce = None
+                ce = oe = None
                 if not (self.skip and ii == 1 and jj == 2):
                     tex = [t for t in tk if np.any(tab.exchange[t, :, k])]
                     if tex:
                         ce = dv.f64(np.stack([tab.exchange[t, :, k] for t in tex]))
-                self.work.append((ii, jj, block, k, tdir, cd, tex, ce))
+                        oe = offsets(tex, exch_slot)
+                self.work.append((ii, jj, block, k, tdir, cd, od, tex, ce, oe, exch_slot))
         self.divisor = {t: float(tab.divisor(t)) for t in self.targets}
         # build every bank / factor the launches need, outside any capture
-        for ii, jj, block, k, tdir, cd, tex, ce in self.work:
+        for ii, jj, block, k, tdir, cd, od, tex, ce, oe, _ in self.work:
             if tdir:
-                if cache.ns >= _semi_direct():
+                if ns >= _semi_direct():
                     cache.factors_device(k, block)
                 else:
                     cache.bank_device(k, block)
             if tex:
-                if cache.ns >= _semi_exchange(block):
+                if ns >= _semi_exchange(block):
                     cache.factors_device(k, block)
                 else:
                     cache.bank_device(k, block)
@@ -164,42 +177,46 @@ class FockStep:
             call("ax_density_sym", dv.ptr(M), dv.ld(M), dv.ptr(pa), dv.ptr(pb), n_orb, ns, ns, ns,
                  order[c], dv.ptr(d), s)
             dens[c] = d
-        H = {t: [dv.zeros((ns, ns)) for _ in range(4)] for t in self.targets}
-        for ii, jj, block, k, tdir, cd, tex, ce in self.work:
+        H = self.H
+        H.zero_()
+        base = dv.ptr(H)
+        for ii, jj, block, k, tdir, cd, od, tex, ce, oe, exch_slot in self.work:
             c_ket = "q" if jj == 2 else "p"
             c_bra = "q" if ii == 2 else "p"
             ks_bra, ks_ket = order[c_bra], order[c_ket]
-            direct_slot = 0 if ii == 1 else 1
-            exch_slot = {(1, 1): 0, (1, 2): 2, (2, 1): 3, (2, 2): 1}[(ii, jj)]
             if tdir:
                 dd = dv.empty((len(tdir), ns, ks_ket))
                 call("ax_combine_rows", dv.ptr(dens[c_ket]), ns * ks_ket, n_orb, dv.ptr(cd),
                      len(tdir), dv.ptr(dd), s)
                 v = cache.case1(k, block, dd)
-                for r, t in enumerate(tdir):
-                    call("ax_banded_to_full_add", dv.ptr(H[t][direct_slot]), ns * ns,
-                         dv.ptr(v[r]), ns * ks_bra, 1, ns, ks_bra, 1.0, s)
+                call("ax_banded_to_full_add_at", base, dv.ptr(od), dv.ptr(v), ns * ks_bra,
+                     len(tdir), ns, ks_bra, 1.0, s)
             if tex:
                 cii = Q if ii == 2 else P
                 cjj = Q if jj == 2 else P
                 xx = dv.empty((len(tex), ns, ns))
                 call("ax_exchange_density", dv.ptr(cii), dv.ptr(cjj), ns, n_orb, ns, dv.ptr(ce),
                      len(tex), dv.ptr(xx), s)
-                kk = cache.case4(k, block, xx, symmetric=(ii == jj))
-                for r, t in enumerate(tex):
-                    axpy(H[t][exch_slot], kk[r])
-        for t in self.targets:
-            pp, qq, pq, qp = H[t]
-            call("ax_fock_finalize", dv.ptr(pp), dv.ptr(qq), dv.ptr(pq), dv.ptr(qp), ns,
-                 self.divisor[t], 1 if self.skip else 0, s)
-        return H
+                if ns >= _semi_exchange(block):
+                    kk = cache.case4(k, block, xx, symmetric=(ii == jj))
+                    tpos = {t: i for i, t in enumerate(self.targets)}
+                    for r, t in enumerate(tex):
+                        axpy(H[tpos[t], exch_slot], kk[r])
+                else:
+                    bank, bns, ksl, ksr = cache.bank_device(k, block)
+                    call("ax_convol_case4_acc", dv.ptr(bank), bns, ksl, ksr, dv.ptr(xx), ns * ns,
+                         len(tex), 1 if ii == jj else 0, base, dv.ptr(oe), s)
+        for i, t in enumerate(self.targets):
+            call("ax_fock_finalize", dv.ptr(H[i, 0]), dv.ptr(H[i, 1]), dv.ptr(H[i, 2]),
+                 dv.ptr(H[i, 3]), ns, self.divisor[t], 1 if self.skip else 0, s)
+        return {t: [H[i, 0], H[i, 1], H[i, 2], H[i, 3]] for i, t in enumerate(self.targets)}
 
     def rk_equivalents(self):
         """(target, source, k, block) contractions with a nonzero coefficient,
         direct and exchange counted separately."""
         tab = self.fb.tables
         n = 0
-        for ii, jj, block, k, tdir, cd, tex, ce in self.work:
+        for ii, jj, block, k, tdir, cd, od, tex, ce, oe, _ in self.work:
             n += sum(int(np.count_nonzero(tab.direct[t, :, k])) for t in tdir)
             n += sum(int(np.count_nonzero(tab.exchange[t, :, k])) for t in tex)
         return n

@@ -309,6 +309,12 @@ ax_status ax_convol_case2(const double *bankT, int ns, int ksl, int ksr, int n_l
 ax_status ax_convol_case4(const double *bankT, int ns, int ksl, int ksr, const double *d,
                           int64_t ld_d, int nb, int symmetric, double *out, int64_t ld_o,
                           void *stream);
+/* case 4 accumulated into nb matrices at out_base + dst_off[b] (device int64
+ * element offsets): out += the contraction (the "hfm_XX += Kk" step of
+ * methods/dirac_fock.py:1293-1301 fused into the contraction). */
+ax_status ax_convol_case4_acc(const double *bankT, int ns, int ksl, int ksr, const double *d,
+                              int64_t ld_d, int nb, int symmetric, double *out_base,
+                              const int64_t *dst_off, void *stream);
 
 /* convol.py:1125-1162 _convol_case1_semiseparable_t; workspace holds
  * ax_convol_semi_workspace(nb, ns, ksr) doubles. */
@@ -332,6 +338,11 @@ ax_status ax_sum_amb_banded(const double *a, int64_t lda, int sa, const double *
 /* convol.py:814-825 _banded_to_full_add for nb matrices. */
 ax_status ax_banded_to_full_add(double *V, int64_t ldv, const double *vb, int64_t ldb, int nb,
                                 int ns, int ks, double coeff, void *stream);
+/* The same into nb matrices at V_base + dst_off[b] (dst_off: device int64
+ * element offsets), touching only the band. */
+ax_status ax_banded_to_full_add_at(double *V_base, const int64_t *dst_off, const double *vb,
+                                   int64_t ldb, int nb, int ns, int ks, double coeff,
+                                   void *stream);
 
 /* y += alpha x (the "hfm_XX += Kk" accumulations of methods/dirac_fock.py:1293-1301) */
 ax_status ax_axpy(double *y, const double *x, int64_t n, double alpha, void *stream);
