@@ -50,6 +50,7 @@ def parse():
                     help="timed steps of the B-spline product workload (0 = skip that leg)")
     ap.add_argument("--ion-steps", type=int, default=20,
                     help="timed SCF-iteration Fock builds of the 26 iron ions (0 = skip)")
+    ap.add_argument("--no-neon", action="store_true", help="skip the neon Dirac-Fock SCF leg")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests in the served Poisson stream (0 = skip the serving leg)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -57,6 +58,30 @@ def parse():
                     help="after the run, copy a buffer of this many MiB device-to-device once "
                          "(a kernel of known traffic for calibrating HBM byte counters)")
     return ap.parse_args()
+
+
+def neon_oracle_scf(bd):
+    """The neon SCF of the neon leg with the oracle's FP64 two-electron Fock
+    blocks and energy (CPU baseline leg: the checker of the GPU SCF)."""
+    import oracle as O
+    from atomx_amd import dirac as D
+
+    shells = D.neon_subshells()
+    occ = [s.occupation for s in shells]
+    kap = [s.kappa for s in shells]
+    banks = O.Banks(bd)
+    tables = O.angular_tables(occ, kap)
+
+    class Backend:
+        def fock(self, i, P, Q):
+            return O.build_hf_matrix_dbsr(i, P, Q, occ, kap, bd, banks, skip_pq_exchange=True,
+                                          tables=tables)
+
+        def energy(self, P, Q):
+            return O.two_electron_energy(P, Q, occ, kap, bd, banks)
+
+    scf = D.DiracFockSCF(10, shells, bd, backend=Backend()).bare_nucleus_start()
+    return scf.run(), scf.iterations
 
 
 def agreement(got, ref, offsets, m):
@@ -191,6 +216,34 @@ def ions_leg(args):
             "rk_equivalents_per_iteration": n_eq,
             "ms_per_iteration_graph": ms_graph, "ms_per_iteration_eager": ms_eager,
             "rk_equivalents_per_s": n_eq / (ms_graph * 1e-3), "dtype": "f64"}
+
+
+def neon_leg(args):
+    """Ne (Z = 10) Dirac-Fock ground state 1s2 2s2 2p6 on 60 B-splines of
+    order 7 (exponential knots, r 1e-5..40, 7 Gauss points per interval,
+    point nucleus), from the bare-nucleus orbitals to convergence: the
+    two-electron Fock blocks and energies on the GPU, the one-body matrices
+    and the FP64 generalised eigensolves on the host."""
+    import torch
+
+    from atomx_amd import bspline as B
+    from atomx_amd import dirac as D
+
+    t = B.make_knots(60, 7, 1e-5, 40.0)
+    bd = B.prepare_bspline_data(t, 60, 7)
+    warm = D.DiracFockSCF(10, D.neon_subshells(), bd).bare_nucleus_start()
+    warm.sweep()
+    torch.cuda.synchronize()
+    scf = D.DiracFockSCF(10, D.neon_subshells(), bd).bare_nucleus_start()
+    t0 = time.perf_counter()
+    e = scf.run()
+    dt = time.perf_counter() - t0
+    steps = scf.backend._steps
+    n_eq = sum(st.rk_equivalents() for st in steps.values())
+    return {"workload": "Ne 1s2 2s2 2p6, 60 B-splines order 7, r 1e-5..40, point nucleus",
+            "E_total_Ha": e, "sweeps": scf.iterations, "seconds": dt,
+            "ms_per_sweep": dt / scf.iterations * 1e3,
+            "fock_rk_equivalents_per_sweep": n_eq, "_bd": bd}
 
 
 TRAFFIC_TABLE = "profiles/c2_hbm_traffic.txt"
@@ -388,6 +441,10 @@ def main():
         line["bspline_products"] = bank_leg(args)
     if args.ion_steps > 0:
         line["fe_ions"] = ions_leg(args)
+    neon = None
+    if not args.no_neon:
+        neon = neon_leg(args)
+        line["neon_dirac_fock"] = {k: v for k, v in neon.items() if not k.startswith("_")}
     if args.serve_requests > 0:
         serving = serving_leg(args)
         line["serving"] = serving
@@ -403,6 +460,10 @@ def main():
             modes[m]["vs_oracle_max_rel_err"] = el
         line["accuracy"] = modes[prec]["vs_oracle_max_rel_err"]
         line["accuracy_reference"] = "CPU FP64 oracle, all values"
+        if neon is not None:  # the same SCF with the oracle's two-electron restatement
+            e_cpu, sweeps = neon_oracle_scf(neon["_bd"])
+            line["neon_dirac_fock"]["E_total_oracle_scf_Ha"] = e_cpu
+            line["neon_dirac_fock"]["abs_diff_vs_oracle_scf_Ha"] = abs(neon["E_total_Ha"] - e_cpu)
         line["cpu_baseline"] = {"value": v, "unit": "R^k/s", "cores": threads, "kind": "port",
                                 "sample": f"full workload ({n_rk} R^k) repeated {reps}x in {dt:.1f} s "
                                           f"with {threads} OpenMP threads",
