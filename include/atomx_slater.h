@@ -361,6 +361,64 @@ ax_status ax_bilinear_stage2(const double *T, int ns, int ksr, int n_right, cons
 ax_status ax_fock_finalize(double *pp, double *qq, double *pq, double *qp, int ns, double norm,
                            int skip_pq, void *stream);
 
+/* ---- auxiliary-node Coulomb action and pair contraction ------------------
+ * Sampled columns are node-major (n_nodes, ld) FP64, or complex FP64 as
+ * interleaved (re, im) when is_complex (c-products, no conjugation). */
+
+/* numerics/bspline/coulomb_features.py:281-294 (sample) for n_col
+ * coefficient vectors (n_col, ld_coef) of one component: rows node_base +
+ * cell*width + j of out get sum_s coef[cell+s] tab[cell][j][s] (s < order,
+ * cell+s < n_coef). */
+ax_status ax_feat_sample(const double *tab, int nv, int width, int order, int n_coef,
+                         const void *coef, int64_t ld_coef, int n_col, int is_complex, void *out,
+                         int64_t ld_out, int64_t node_base, void *stream);
+
+/* coulomb_features.py:30-61 (_apply) for n_col real columns: outer
+ * (2, nv, ngo) node indices, slot[node] = cell*ngo + g for outer nodes and
+ * -1 otherwise, rising/falling (nv, ngo) weights, same-cell edges by
+ * destination (row_ptr (n_nodes+1) int64, src, w).  work holds
+ * ax_feat_apply_workspace(nv, n_col) bytes. */
+ax_status ax_feat_apply(const double *rho, int64_t ld, int n_col, int64_t n_nodes,
+                        const int *outer, const int *slot, const double *rising,
+                        const double *falling, int nv, int ngo, const int64_t *row_ptr,
+                        const int *src, const double *w, double *work, double *out,
+                        int64_t ld_out, void *stream);
+int64_t ax_feat_apply_workspace(int nv, int n_col);
+
+/* coulomb_features.py:332-337 (integral) pieces: column products
+ * out[i][p] = S[i][ia[p]] S[i][ib[p]] and column dots
+ * out[p] = sum_i X[i][ix[p]] Y[i][iy[p]]. */
+ax_status ax_feat_products(const void *S, int64_t lds, int64_t n_nodes, int n_pairs,
+                           const int *ia, const int *ib, int is_complex, void *out,
+                           int64_t ld_out, void *stream);
+ax_status ax_feat_dots(const void *X, int64_t ldx, const void *Y, int64_t ldy, int64_t n_nodes,
+                       int n_pairs, const int *ix, const int *iy, int is_complex, void *out,
+                       void *stream);
+
+/* numerics/bspline/pair_contraction.py:84-99 (moment): per cell c,
+ * rise/fall[c][i][j] = sum over both components' outer nodes o of c of
+ * w[c][g] Y[o][i] X[o][j] with w = wr / wf. */
+ax_status ax_cpc_moments(const void *X, int64_t ldx, int d2, const void *Y, int64_t ldy, int d1,
+                         const int *outer, const double *wr, const double *wf, int nv, int ngo,
+                         int is_complex, void *rise, void *fall, void *stream);
+/* pair_contraction.py:100-105: prefix[c] = sum_{c'<c} rise[c'],
+ * suffix[c] = sum_{c'>c} fall[c'] over (nv, n_el) stacks. */
+ax_status ax_cpc_scan(const void *rise, const void *fall, int nv, int64_t n_el, int is_complex,
+                      void *prefix, void *suffix, void *stream);
+/* pair_contraction.py:121-148 (__call__): out (m, n) = sum_c L_rise[c] rho
+ * sfx[c]^T + L_fall[c] rho pfx[c]^T (left_stack = [L_rise; L_fall] as
+ * (2, nv, nr, m), sfx/pfx (nv, ns, n)) + the same-cell directed edges
+ * (src, dst, w): w (B[src] rho . C[dst]) A[src]^T D[dst].  work holds
+ * ax_cpc_workspace_bytes(...) bytes. */
+int64_t ax_cpc_workspace_bytes(int nv, int nr, int ns, int m, int n, int64_t n_nodes,
+                               int64_t n_edges, int n_split, int is_complex);
+ax_status ax_cpc_apply(const void *rho, int nr, int ns, const void *A, int64_t lda, int m,
+                       const void *B, int64_t ldb, const void *C, int64_t ldc, const void *D,
+                       int64_t ldd, int n, int64_t n_nodes, int nv, const void *left_stack,
+                       const void *sfx, const void *pfx, const int *src, const int *dst,
+                       const double *w, int64_t n_edges, int n_split, int is_complex, void *work,
+                       void *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
