@@ -700,6 +700,32 @@ __global__ void sum_amb_kernel(const double *__restrict__ a, int64_t lda, int sa
     }
 }
 
+// out[q] (+)= sum_{i, ip < ks, i+ip < ns} a_q[i][ip] b_q[i][ip] for banded
+// operands addressed through device pointer tables (operands shared by many
+// q are stored once).
+__global__ void sum_amb_ptr_kernel(const double *const *__restrict__ a,
+                                   const double *const *__restrict__ b, int nq, int ns, int ks,
+                                   int accumulate, double *__restrict__ out) {
+    __shared__ double red[kT];
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        const double *x = a[q];
+        const double *y = b[q];
+        double s = 0.0;
+        for (int e = threadIdx.x; e < ns * ks; e += kT) {
+            const int ip = e / ns, i = e - (e / ns) * ns;
+            if (i + ip < ns) s += x[(int64_t)i * ks + ip] * y[(int64_t)i * ks + ip];
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = kT / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = accumulate ? out[q] + red[0] : red[0];
+        __syncthreads();
+    }
+}
+
 __global__ void banded_to_full_kernel(double *__restrict__ V, int64_t ldv,
                                       const double *__restrict__ vb, int64_t ldb, int nb, int ns,
                                       int ks, double coeff) {
@@ -1042,6 +1068,17 @@ ax_status ax_sum_amb_banded(const double *a, int64_t lda, int sa, const double *
     if (nb == 0) return AX_OK;
     hipLaunchKernelGGL(sum_amb_kernel, dim3(nb < 4096 ? nb : 4096), dim3(kT), 0,
                        ax::as_stream(stream), a, lda, sa, b, ldb, sb, nb, ns, ks, out);
+    AX_CHECK_LAUNCH();
+    return AX_OK;
+}
+
+ax_status ax_sum_amb_ptr(const double *const *a, const double *const *b, int nq, int ns,
+                         int ks, int accumulate, double *out, void *stream) {
+    AX_REQUIRE(nq >= 0 && ns > 0 && ks > 0, "ax_sum_amb_ptr: bad sizes");
+    if (nq == 0) return AX_OK;
+    AX_REQUIRE(a && b && out, "ax_sum_amb_ptr: null pointer");
+    hipLaunchKernelGGL(sum_amb_ptr_kernel, dim3(nq < 4096 ? nq : 4096), dim3(kT), 0,
+                       ax::as_stream(stream), a, b, nq, ns, ks, accumulate, out);
     AX_CHECK_LAUNCH();
     return AX_OK;
 }

@@ -166,7 +166,8 @@ def dirac_matrices(kappa, Z, bd, nuclear="point", A=None):
     (2 ns, 2 ns) with ns = max(nsp, nsq) (dirac_assembly.py:82-155): nuclear
     attraction on both diagonal blocks, -2c^2 S on the QQ block, the
     derivative and 1/r couplings, and under kinetic balance the Bloch
-    boundary terms at the last knot."""
+    boundary terms at the last knot.  ``nuclear`` is "point" (or None),
+    "fermi", or a callable V(r)."""
     nv, ks = bd["nv"], bd["ks"]
     gr, grw = bd["gr"], bd["grw"]
     ksp, ksq = bd.get("ksp", ks), bd.get("ksq", ks)
@@ -174,7 +175,12 @@ def dirac_matrices(kappa, Z, bd, nuclear="point", A=None):
     bp, bq, bqd = bd["bsp_p"], bd["bsp_q"], bd["bsp_q_d"]
     bpd = bd.get("bsp_p_d")
     ns = max(nsp, nsq)
-    vw = grw * nuclear_potential(Z, gr, nuclear, A)
+    if nuclear is None:
+        nuclear = "point"
+    if callable(nuclear):  # V(r) evaluated node by node (integrals.py:103-117)
+        vw = grw * np.vectorize(lambda x: float(nuclear(x)))(gr)
+    else:
+        vw = grw * nuclear_potential(Z, gr, nuclear, A)
     sym = lambda M: 0.5 * (M + M.T)  # noqa: E731
     S_PP = sym(assemble_integral(nv, ksp, ksp, ns, bp, bp, grw))
     S_QQ = sym(assemble_integral(nv, ksq, ksq, ns, bq, bq, grw))

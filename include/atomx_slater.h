@@ -335,6 +335,13 @@ ax_status ax_convol_case4_semi(const double *near, const double *rl, const doubl
 ax_status ax_sum_amb_banded(const double *a, int64_t lda, int sa, const double *b, int64_t ldb,
                             int sb, int nb, int ns, int ks, double *out, void *stream);
 
+/* convol.py:1348-1356 _sum_amb_banded for nq operand pairs given by device
+ * pointer tables a[q], b[q] (each (ns, ks) row-major); out[q] is
+ * overwritten, or accumulated into when accumulate != 0 (the component sum
+ * of physics/csf/matrix.py:793-797). */
+ax_status ax_sum_amb_ptr(const double *const *a, const double *const *b, int nq, int ns, int ks,
+                         int accumulate, double *out, void *stream);
+
 /* convol.py:814-825 _banded_to_full_add for nb matrices. */
 ax_status ax_banded_to_full_add(double *V, int64_t ldv, const double *vb, int64_t ldb, int nb,
                                 int ns, int ks, double coeff, void *stream);
