@@ -271,7 +271,8 @@ class DiracFockSCF:
     ``fock(i, P, Q) -> (PP, QQ, PQ, QP)`` and ``energy(P, Q) -> (E2e, EJ, EK)``.
     """
 
-    def __init__(self, Z, subshells, bsp_data, nuclear="point", A=None, backend=None):
+    def __init__(self, Z, subshells, bsp_data, nuclear="point", A=None, backend=None,
+                 mode="dbsr"):
         self.Z = Z
         self.shells = list(subshells)
         self.bd = bsp_data
@@ -284,7 +285,14 @@ class DiracFockSCF:
         occ = [s.occupation for s in self.shells]
         if backend is None:
             from .dirac_fock import FockBuilder
-            backend = _GpuBackend(FockBuilder(bsp_data, kappas, occ))
+            fb = FockBuilder(bsp_data, kappas, occ)
+            if mode == "dbsr":
+                backend = _GpuBackend(fb)
+            elif mode == "orbital":
+                from .orbital_mode import OrbitalModeFock
+                backend = _OrbitalBackend(OrbitalModeFock(bsp_data, kappas, occ, fb.cache), fb)
+            else:
+                raise ValueError(f"mode must be 'dbsr' or 'orbital', got {mode!r}")
         self.backend = backend
         self.h0 = {k: dirac_matrices(k, Z, bd, nuclear, A) for k in sorted(set(kappas))}
         n = len(self.shells)
@@ -411,3 +419,29 @@ class _GpuBackend:
 
     def energy(self, P, Q):
         return self.fb.energy(P, Q)
+
+
+class _OrbitalBackend:
+    """Orbital-mode Fock blocks; orbitals changed since the last call are
+    re-uploaded one by one so the other sources' exchange contractions stay
+    cached (as the reference drops only the updated orbital's entries)."""
+
+    def __init__(self, om, fb):
+        self.om = om
+        self.fb = fb
+        self._P = self._Q = None
+
+    def fock(self, i, P, Q):
+        P, Q = np.asarray(P), np.asarray(Q)
+        if self._P is None or self._P.shape != P.shape:
+            self.om.load(P, Q)
+        else:
+            for b in range(P.shape[0]):
+                if not (np.array_equal(P[b], self._P[b]) and np.array_equal(Q[b], self._Q[b])):
+                    self.om.update(b, P[b], Q[b])
+        self._P, self._Q = P.copy(), Q.copy()
+        return self.om.blocks(i)
+
+    def energy(self, P, Q):
+        return self.fb.energy(P, Q)
+
