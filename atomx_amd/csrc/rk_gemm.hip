@@ -10,7 +10,11 @@
 //
 // FP64 path: v_mfma_f64_16x16x4_f64.  256 threads = 4 waves in a 2x2 layout,
 // each wave owns a 32x32 sub-tile = 2x2 MFMA tiles; K is staged through LDS in
-// chunks of 16 in k-major layout.
+// chunks of 16 in k-major layout, double buffered (35 KB of LDS: four
+// work-groups per CU).  A 16x16 block that lies wholly below the diagonal of
+// its segment or past its last row/column is not stored, and a wave whose
+// four blocks are all such blocks skips its MFMAs (the lower-left wave of a
+// diagonal tile, waves past the segment edge).
 #include "ax_common.h"
 #include "rk_gemm.h"
 
@@ -19,40 +23,53 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64;
-constexpr int BK = 32;
+constexpr int BK = 16;
 constexpr int LDS_LD = BM + 4;  // padded row (in doubles) of the k-major LDS tiles
 
-constexpr int KPT = BK * BM / 256;  // k values per thread in one staged tile (8)
+constexpr int KPT = BK * BM / 256;  // k values per thread in one staged tile (4)
 
-// Loads one BK x BM k-major tile of rows [row0, row0+64) x [k0, k0+BK) into
-// registers: thread t -> row t/4, k sub-block (t%4)*KPT .. +KPT-1.
-__device__ inline void load_tile_regs(const double *__restrict__ base, int64_t ld,
-                                      const int32_t *__restrict__ rows, int row0, int m,
-                                      int64_t k0, int64_t kend, double (&v)[KPT]) {
-    const int t = threadIdx.x;
-    const int r = t >> 2;
-    const int kq = (t & 3) * KPT;
-    const int gr = row0 + r;
-    if (gr < m) {
-        const int64_t rr = rows ? (int64_t)rows[gr] : (int64_t)gr;
-        const double *p = base + rr * ld;
+// Global-address-space view of device memory: loads through it are
+// global_load (vector memory counter only), so waiting on LDS reads never
+// waits on the prefetch of the next K stage.
+typedef const double __attribute__((address_space(1))) gdouble;
+
+// Row pointer of this thread's staged row (thread t -> row t/4); rows past
+// the segment's last row read its row 0 and are zeroed when staged to LDS.
+__device__ inline gdouble *stage_row(const double *__restrict__ base, int64_t ld,
+                                     const int32_t *__restrict__ rows, int row0, int m,
+                                     bool &valid) {
+    const int gr = row0 + (threadIdx.x >> 2);
+    valid = gr < m;
+    const int r = valid ? gr : 0;
+    const int64_t rr = rows ? (int64_t)rows[r] : (int64_t)r;
+    return (gdouble *)(base + rr * ld);
+}
+
+// Loads this thread's KPT k values [k0 + kq, k0 + kq + KPT) of its row (zero
+// past kend).  The full-tile test is uniform over the work-group.
+__device__ inline void load_stage(gdouble *__restrict__ p, int64_t k0, int64_t kend,
+                                  double (&v)[KPT]) {
+    const int kq = (threadIdx.x & 3) * KPT;
+    if (k0 + BK <= kend) {
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) v[q] = p[k0 + kq + q];
+    } else {
 #pragma unroll
         for (int q = 0; q < KPT; ++q) {
             const int64_t kx = k0 + kq + q;
             v[q] = kx < kend ? p[kx] : 0.0;
         }
-    } else {
-#pragma unroll
-        for (int q = 0; q < KPT; ++q) v[q] = 0.0;
     }
 }
 
-__device__ inline void store_tile_lds(double *__restrict__ s, const double (&v)[KPT]) {
+// Stages the loaded values k-major into LDS (zeros for a row past the end).
+__device__ inline void store_tile_lds(double *__restrict__ s, const double (&v)[KPT],
+                                      bool valid) {
     const int t = threadIdx.x;
     const int r = t >> 2;
     const int kq = (t & 3) * KPT;
 #pragma unroll
-    for (int q = 0; q < KPT; ++q) s[(kq + q) * LDS_LD + r] = v[q];
+    for (int q = 0; q < KPT; ++q) s[(kq + q) * LDS_LD + r] = valid ? v[q] : 0.0;
 }
 
 __global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::BatchOf<ax::GemmArg> A) {
@@ -79,44 +96,65 @@ __global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::BatchOf<ax::
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     const int fr = lane & 15, fk = lane >> 4;
 
+    // wave-uniform mask of the 16x16 blocks that hold upper-triangle
+    // elements of the segment (i <= j < m); bit 2a+b for block (a, b)
+    int bits = 0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int i0 = row0 + wr + 16 * a, j0 = col0 + wc + 16 * b;
+            if (i0 < sg.m && j0 < sg.m && i0 <= j0 + 15) bits |= 1 << (2 * a + b);
+        }
+    const int msk = __builtin_amdgcn_readfirstlane(bits);
+
     d4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
+    bool va_ok, vb_ok;
+    gdouble *pa = stage_row(sg.D, sg.ld_d, sg.rows, row0, sg.m, va_ok);
+    gdouble *pb = stage_row(sg.Y, sg.ld_y, nullptr, col0, sg.m, vb_ok);
     double va[KPT], vb[KPT];
     int buf = 0;
     if (kbeg < kend) {
-        load_tile_regs(sg.D, sg.ld_d, sg.rows, row0, sg.m, kbeg, kend, va);
-        load_tile_regs(sg.Y, sg.ld_y, nullptr, col0, sg.m, kbeg, kend, vb);
-        store_tile_lds(As[0], va);
-        store_tile_lds(Bs[0], vb);
+        load_stage(pa, kbeg, kend, va);
+        load_stage(pb, kbeg, kend, vb);
+        store_tile_lds(As[0], va, va_ok);
+        store_tile_lds(Bs[0], vb, vb_ok);
     }
     __syncthreads();
     for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
         const int64_t kn = k0 + BK;
         if (kn < kend) {
-            load_tile_regs(sg.D, sg.ld_d, sg.rows, row0, sg.m, kn, kend, va);
-            load_tile_regs(sg.Y, sg.ld_y, nullptr, col0, sg.m, kn, kend, vb);
+            load_stage(pa, kn, kend, va);
+            load_stage(pb, kn, kend, vb);
         }
-        const double *as = As[buf];
-        const double *bs = Bs[buf];
+        if (msk) {
+            const double *as = As[buf];
+            const double *bs = Bs[buf];
+            double a0[BK / 4], a1[BK / 4], b0[BK / 4], b1[BK / 4];
 #pragma unroll
-        for (int kk = 0; kk < BK / 4; ++kk) {
-            const int kr = (kk * 4 + fk) * LDS_LD;
-            const double a0 = as[kr + wr + fr];
-            const double a1 = as[kr + wr + 16 + fr];
-            const double b0 = bs[kr + wc + fr];
-            const double b1 = bs[kr + wc + 16 + fr];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            for (int kk = 0; kk < BK / 4; ++kk) {
+                const int kr = (kk * 4 + fk) * LDS_LD;
+                a0[kk] = as[kr + wr + fr];
+                a1[kk] = as[kr + wr + 16 + fr];
+                b0[kk] = bs[kr + wc + fr];
+                b1[kk] = bs[kr + wc + 16 + fr];
+            }
+#pragma unroll
+            for (int kk = 0; kk < BK / 4; ++kk) {
+                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[kk], b0[kk], acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[kk], b1[kk], acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[kk], b0[kk], acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[kk], b1[kk], acc[1][1], 0, 0, 0);
+            }
         }
         if (kn < kend) {
-            store_tile_lds(As[buf ^ 1], va);
-            store_tile_lds(Bs[buf ^ 1], vb);
+            store_tile_lds(As[buf ^ 1], va, va_ok);
+            store_tile_lds(Bs[buf ^ 1], vb, vb_ok);
         }
         __syncthreads();
         buf ^= 1;
@@ -124,18 +162,20 @@ __global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::BatchOf<ax::
 
     // C layout of the 16x16x4 f64 MFMA: result register r of lane l holds
     // row (l/16) + 4r, column l%16 (one-row groups, unlike the 4-row groups
-    // of the i32/f32 16x16 MFMAs).
+    // of the i32/f32 16x16 MFMAs).  Skipped blocks are never read back.
     double *dst = g.partial + (e * n_split + split) * (BM * BM);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 2; ++b) {
+            if (!(msk & (1 << (2 * a + b)))) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = wr + a * 16 + fk + 4 * r;
                 const int j = wc + b * 16 + fr;
                 dst[i * BM + j] = acc[a][b][r];
             }
+        }
 }
 
 // Sums the split partials of each tile in split order and writes the packed

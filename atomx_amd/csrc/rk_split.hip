@@ -60,7 +60,11 @@ __global__ __launch_bounds__(256) void slice_batch_kernel(const ax::BatchOf<ax::
     }
 }
 
-__device__ inline v4i load16(const int8_t *p) { return *reinterpret_cast<const v4i *>(p); }
+// Global-address-space loads: they count on the vector memory counter only,
+// so waiting on LDS reads never waits on the prefetch of the next K-tile.
+typedef const v4i __attribute__((address_space(1))) gv4i;
+typedef const int32_t __attribute__((address_space(1))) gint;
+__device__ inline v4i load16(const int8_t *p) { return *(gv4i *)p; }
 
 // LDS tile of one operand for one K-tile: [S][64 rows][ROWB bytes] int8
 // (64 data bytes + 16 pad per row) and the 64 row exponents.
@@ -69,30 +73,33 @@ constexpr int PLANEB = BM * ROWB;
 
 // Global -> register staging: thread t moves S chunks of 16 bytes, chunk
 // (s, row = t/4, q = t%4) of the K-tile.
-template <int S>
-__device__ inline void stage_load(const int8_t *__restrict__ base, int64_t plane, int64_t ld,
-                                  const int32_t *__restrict__ rows, int row0, int m, int64_t t,
-                                  v4i (&reg)[S]) {
+// Base of this thread's staged row (thread t -> row t/4, 16-byte chunk t%4);
+// a row past the segment's last row reads row 0 and is zeroed when staged.
+__device__ inline const int8_t *stage_base(const int8_t *__restrict__ base, int64_t ld,
+                                           const int32_t *__restrict__ rows, int row0, int m,
+                                           bool &valid) {
     const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
     const int gr = row0 + r;
-    const v4i zero = {0, 0, 0, 0};
-    if (gr < m) {
-        const int64_t rr = rows ? (int64_t)rows[gr] : (int64_t)gr;
-        const int8_t *p = base + rr * ld + t * KT + q * 16;
-#pragma unroll
-        for (int s2 = 0; s2 < S; ++s2) reg[s2] = load16(p + s2 * plane);
-    } else {
-#pragma unroll
-        for (int s2 = 0; s2 < S; ++s2) reg[s2] = zero;
-    }
+    valid = gr < m;
+    const int g = valid ? gr : 0;
+    const int64_t rr = rows ? (int64_t)rows[g] : (int64_t)g;
+    return base + rr * ld + q * 16;
 }
 
 template <int S>
-__device__ inline void stage_store(int8_t *__restrict__ lds, const v4i (&reg)[S]) {
+__device__ inline void stage_load(const int8_t *__restrict__ p, int64_t plane, int64_t t,
+                                  v4i (&reg)[S]) {
+#pragma unroll
+    for (int s2 = 0; s2 < S; ++s2) reg[s2] = load16(p + t * KT + s2 * plane);
+}
+
+template <int S>
+__device__ inline void stage_store(int8_t *__restrict__ lds, const v4i (&reg)[S], bool valid) {
     const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const v4i zero = {0, 0, 0, 0};
 #pragma unroll
     for (int s2 = 0; s2 < S; ++s2)
-        *reinterpret_cast<v4i *>(lds + s2 * PLANEB + r * ROWB + q * 16) = reg[s2];
+        *reinterpret_cast<v4i *>(lds + s2 * PLANEB + r * ROWB + q * 16) = valid ? reg[s2] : zero;
 }
 
 template <int S>
@@ -143,27 +150,31 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
+    bool va_ok, vb_ok;
+    const int8_t *pa = stage_base(sg.Ds, sg.ds_ld, sg.rows, row0, sg.m, va_ok);
+    const int8_t *pb = stage_base(sg.Ys, sg.ys_ld, nullptr, col0, sg.m, vb_ok);
+    // exponent row of this thread's slot (rows for tid < 64, columns for < 128)
+    gint *ep = tid < BM ? (gint *)(sg.De + erow * sg.de_ld) : (gint *)(sg.Ye + erow * sg.ye_ld);
+    const bool has_e = tid < 2 * BM;
     v4i ra[S], rb[S];
     int rexp = 0;
     if (tb < te) {
-        stage_load<S>(sg.Ds, sg.ds_plane, sg.ds_ld, sg.rows, row0, sg.m, tb, ra);
-        stage_load<S>(sg.Ys, sg.ys_plane, sg.ys_ld, nullptr, col0, sg.m, tb, rb);
-        if (tid < BM) rexp = eok ? sg.De[erow * sg.de_ld + tb] : 0;
-        else if (tid < 2 * BM) rexp = eok ? sg.Ye[erow * sg.ye_ld + tb] : 0;
+        stage_load<S>(pa, sg.ds_plane, tb, ra);
+        stage_load<S>(pb, sg.ys_plane, tb, rb);
+        if (has_e) rexp = ep[tb];
     }
     const v4i zero = {0, 0, 0, 0};
     for (int64_t t = tb; t < te; ++t) {
         __syncthreads();  // previous tile's LDS reads are complete
-        stage_store<S>(As, ra);
-        stage_store<S>(Bs, rb);
-        if (tid < BM) eAs[tid] = rexp;
-        else if (tid < 2 * BM) eBs[tid - BM] = rexp;
+        stage_store<S>(As, ra, va_ok);
+        stage_store<S>(Bs, rb, vb_ok);
+        if (tid < BM) eAs[tid] = eok ? rexp : 0;
+        else if (tid < 2 * BM) eBs[tid - BM] = eok ? rexp : 0;
         __syncthreads();
         if (t + 1 < te) {  // prefetch the next K-tile while this one is multiplied
-            stage_load<S>(sg.Ds, sg.ds_plane, sg.ds_ld, sg.rows, row0, sg.m, t + 1, ra);
-            stage_load<S>(sg.Ys, sg.ys_plane, sg.ys_ld, nullptr, col0, sg.m, t + 1, rb);
-            if (tid < BM) rexp = eok ? sg.De[erow * sg.de_ld + t + 1] : 0;
-            else if (tid < 2 * BM) rexp = eok ? sg.Ye[erow * sg.ye_ld + t + 1] : 0;
+            stage_load<S>(pa, sg.ds_plane, t + 1, ra);
+            stage_load<S>(pb, sg.ys_plane, t + 1, rb);
+            if (has_e) rexp = ep[t + 1];
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a) {

@@ -85,10 +85,12 @@ int choose_slices(const double *r, int nv, int ng, int kmax) {
 }
 
 int choose_split(int n_work, int64_t n) {
-    // enough work-groups for 256 CUs (about four per CU), K chunks >= 256
-    int s = 1;
-    while ((int64_t)n_work * s < 1024 && n / (2 * s) >= 256 && s < 16) s *= 2;
-    return s;
+    // about one round of work-groups (four per CU on 256 CUs), K chunks of
+    // at least 128 nodes, at most 16 chunks
+    int64_t s = 1024 / std::max(n_work, 1);
+    s = std::min<int64_t>(s, n / 128);
+    s = std::min<int64_t>(s, 16);
+    return static_cast<int>(std::max<int64_t>(s, 1));
 }
 
 // Records the completion event of every plan after a launch sequence
@@ -541,7 +543,7 @@ ax_status ax_rk_tensor_tune(ax_rk_tensor *plan, const double *P, const double *Q
     const int save_order = plan->host.order, save_split = plan->host.n_split;
     float best = -1.0f;
     int best_order = save_order, best_split = save_split;
-    for (int split = 1; split <= plan->split_cap && st == AX_OK; split *= 2) {
+    for (int split = 1; split <= plan->split_cap && st == AX_OK; ++split) {
         for (int order = 0; order < ax::kOrderCount && st == AX_OK; ++order) {
             plan->host.order = order;
             plan->host.n_split = split;

@@ -61,8 +61,35 @@ __global__ void pair_density_kernel(const double *__restrict__ P, const double *
 // For split-precision plans the same pass writes the int8 slices of D: every
 // wave holds one 64-aligned run of x, i.e. one K-tile, so the loop runs to
 // the padded length (wave-uniform) with zeros beyond n.
+// Slices two consecutive values per lane into the int8 planes of one
+// 64-node K-tile held by 32 lanes (half a wave): same arithmetic as
+// slice_value (tile exponent from the tile maximum, truncated 7-bit digits).
+__device__ inline void slice_pair(double v0, double v1, int64_t x, int8_t *__restrict__ dst_row,
+                                  int64_t plane, int32_t *__restrict__ e_slot, int slices) {
+    double mx = fmax(fabs(v0), fabs(v1));
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    const int ex = mx > 0.0 ? ilogb(mx) + 1 : 0;
+    double r0 = ldexp(v0, 7 - ex), r1 = ldexp(v1, 7 - ex);
+    int8_t *d = dst_row + x;
+    for (int s = 0; s < slices; ++s) {
+        const double c0 = trunc(r0), c1 = trunc(r1);
+        const uint16_t word = (uint16_t)(uint8_t)static_cast<int8_t>(c0) |
+                              (uint16_t)((uint16_t)(uint8_t)static_cast<int8_t>(c1) << 8);
+        *reinterpret_cast<uint16_t *>(d + (int64_t)s * plane) = word;
+        r0 = (r0 - c0) * 128.0;
+        r1 = (r1 - c1) * 128.0;
+    }
+    if ((threadIdx.x & 31) == 0) *e_slot = ex;
+}
+
+// Grid (node chunks of 512, pairs of the batch).  Each thread forms two
+// consecutive nodes; rows of even length with 16-byte aligned bases are
+// read and written as double2.  Split plans also slice their rows (npad is
+// a multiple of 64, so every half-wave holds one whole K-tile).
 __global__ __launch_bounds__(256) void pair_density_batch_kernel(
     const ax::BatchOf<ax::DensArg> A) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
     const int64_t total = A.end[A.n - 1];
     for (int64_t gp = blockIdx.y; gp < total; gp += gridDim.y) {
         int64_t p;
@@ -76,19 +103,46 @@ __global__ __launch_bounds__(256) void pair_density_batch_kernel(
         const double *Qb = g.Q ? g.Q + b * ld : nullptr;
         double *o = g.D + p * n;
         const int64_t lim = g.Ds ? g.npad : n;
-        for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < lim;
-             x += (int64_t)gridDim.x * blockDim.x) {
-            double v = 0.0;
-            if (x < n) {
-                v = Pa[x] * Pb[x];
-                if (Qa) v += Qa[x] * Qb[x];
-                v = g.w[x] * v;
-                o[x] = v;
+        const bool vec = (n % 2 == 0) && (ld % 2 == 0) &&
+                         ((reinterpret_cast<uintptr_t>(g.P) | reinterpret_cast<uintptr_t>(g.w) |
+                           reinterpret_cast<uintptr_t>(g.D) |
+                           (g.Q ? reinterpret_cast<uintptr_t>(g.Q) : 0)) % 16 == 0);
+        for (int64_t x = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; x < lim;
+             x += (int64_t)gridDim.x * blockDim.x * 2) {
+            double v0 = 0.0, v1 = 0.0;
+            if (vec) {
+                const int64_t xc = x < n ? x : n - 2;
+                const d2 pa = *reinterpret_cast<const d2 *>(Pa + xc);
+                const d2 pb = *reinterpret_cast<const d2 *>(Pb + xc);
+                const d2 w = *reinterpret_cast<const d2 *>(g.w + xc);
+                d2 t = pa * pb;
+                if (Qa) {
+                    const d2 qa = *reinterpret_cast<const d2 *>(Qa + xc);
+                    const d2 qb = *reinterpret_cast<const d2 *>(Qb + xc);
+                    t = t + qa * qb;
+                }
+                t = w * t;
+                if (x < n) {
+                    *reinterpret_cast<d2 *>(o + x) = t;
+                    v0 = t.x;
+                    v1 = t.y;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int64_t xu = x + u;
+                    if (xu < n) {
+                        double t = Pa[xu] * Pb[xu];
+                        if (Qa) t += Qa[xu] * Qb[xu];
+                        t = g.w[xu] * t;
+                        o[xu] = t;
+                        (u ? v1 : v0) = t;
+                    }
+                }
             }
             if (g.Ds)
-                ax::slice_value(v, x, g.Ds + p * g.npad, g.ds_plane,
-                                g.De + p * (g.npad / ax::kSplitKTile) + x / ax::kSplitKTile,
-                                g.slices);
+                slice_pair(v0, v1, x, g.Ds + p * g.npad, g.ds_plane,
+                           g.De + p * (g.npad / ax::kSplitKTile) + x / ax::kSplitKTile, g.slices);
         }
     }
 }
@@ -811,6 +865,178 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double *__
     if (threadIdx.x == 0) out[0] = sa + sb;
 }
 
+// ---------------------------------------------------------------------------
+// Batched scan, one 256-thread work-group per (row, k) item, rows of up to
+// 64 * 4 * MAXQ cells.  Cells are taken in passes of 64; pass P belongs to
+// wave P % 4.  Global memory is always read and written node-per-lane
+// (lane l holds node P*64*NG + r*64 + l, fully coalesced, every load of the
+// item issued before any arithmetic); a per-wave LDS region transposes the
+// products g = d r^k, d r^-(k+1) to cell-per-lane for the cell sums, the
+// in-wave Hillis-Steele scans and the in-cell partial integrals, and the
+// two cumulative integrals back to node-per-lane for the output (or, with
+// SLICE, for the int8 slices: for a fixed r the 64 lanes hold one K-tile).
+// The per-pass wave totals are combined through LDS in a fixed order after
+// one barrier.  Same discretisation as scan_item (numerics/yk.py:87-154).
+// ---------------------------------------------------------------------------
+constexpr int kWgScanThreads = 256;
+constexpr int kWgScanWaves = kWgScanThreads / 64;
+
+template <int NG, bool SLICE, int MAXQ>
+__global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
+    const ax::BatchOf<ax::ScanArg> A, const double *__restrict__ mo,
+    const double *__restrict__ mi) {
+    constexpr int MAXP = MAXQ * kWgScanWaves;
+    constexpr int CS = NG + 1;  // padded cell stride of the transpose region
+    __shared__ double tot_o[MAXP], tot_i[MAXP];
+    __shared__ double tr[kWgScanWaves][2][64 * CS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t local;
+    const int p = ax::batch_find(A, blockIdx.x, local);
+    const ax::ScanArg &g = A.a[p];
+    const int k = g.item_k[local];
+    const int64_t n = g.n;
+    const int nv = g.nv;
+    const int passes = (nv + 63) / 64;
+    const double *src = g.D + (int64_t)g.item_row[local] * n;
+    const double *rkp = g.rk + (int64_t)k * n;
+    const double *rkmp = g.rkm + (int64_t)k * n;
+    double *to = tr[wave][0];
+    double *ti = tr[wave][1];
+
+    // node-per-lane products of every pass of this wave; the loads use a
+    // clamped index and are all issued before the first product (a load
+    // under a per-lane branch would be waited for inside the branch)
+    double no[MAXQ][NG], ni[MAXQ][NG];
+    {
+        double dv[MAXQ][NG], av[MAXQ][NG], bv[MAXQ][NG];
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int64_t base = (int64_t)(q * kWgScanWaves + wave) * 64 * NG;
+#pragma unroll
+            for (int r = 0; r < NG; ++r) {
+                const int64_t x = base + r * 64 + lane;
+                const int64_t xc = x < n ? x : n - 1;
+                dv[q][r] = src[xc];
+                av[q][r] = rkp[xc];
+                bv[q][r] = rkmp[xc];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int64_t base = (int64_t)(q * kWgScanWaves + wave) * 64 * NG;
+#pragma unroll
+            for (int r = 0; r < NG; ++r) {
+                const bool in = base + r * 64 + lane < n;
+                no[q][r] = in ? dv[q][r] * av[q][r] : 0.0;
+                ni[q][r] = in ? dv[q][r] * bv[q][r] : 0.0;
+            }
+        }
+    }
+    // cell-per-lane values, cell sums and in-wave scans
+    double go[MAXQ][NG], gi[MAXQ][NG];
+    double ex_o[MAXQ], ex_i[MAXQ];
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        const int P = q * kWgScanWaves + wave;
+#pragma unroll
+        for (int r = 0; r < NG; ++r) {
+            const int xl = r * 64 + lane;  // node of the pass, cell xl / NG
+            to[(xl / NG) * CS + xl % NG] = no[q][r];
+            ti[(xl / NG) * CS + xl % NG] = ni[q][r];
+        }
+        double so = 0.0, si = 0.0;
+#pragma unroll
+        for (int j = 0; j < NG; ++j) {
+            go[q][j] = to[lane * CS + j];
+            gi[q][j] = ti[lane * CS + j];
+            so += go[q][j];
+            si += gi[q][j];
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double va = __shfl_up(so, off, 64);
+            const double vb = __shfl_down(si, off, 64);
+            if (lane >= off) so += va;
+            if (lane + off < 64) si += vb;
+        }
+        const double ea = __shfl_up(so, 1, 64);
+        const double eb = __shfl_down(si, 1, 64);
+        ex_o[q] = lane == 0 ? 0.0 : ea;
+        ex_i[q] = lane == 63 ? 0.0 : eb;
+        const double tso = __shfl(so, 63, 64), tsi = __shfl(si, 0, 64);
+        if (lane == 0) {
+            tot_o[P] = P < passes ? tso : 0.0;
+            tot_i[P] = P < passes ? tsi : 0.0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        const int P = q * kWgScanWaves + wave;
+        if (P >= passes) break;  // wave-uniform
+        double co = 0.0, ci = 0.0;
+        for (int v = 0; v < P; ++v) co += tot_o[v];
+        for (int v = passes - 1; v > P; --v) ci += tot_i[v];
+        const double bo = co + ex_o[q], bi = ci + ex_i[q];
+        // cumulative integrals of the lane's cell, back to node-per-lane
+#pragma unroll
+        for (int m = 0; m < NG; ++m) {
+            double po = 0.0, pi = 0.0;
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                po += mo[m * NG + j] * go[q][j];
+                pi += mi[m * NG + j] * gi[q][j];
+            }
+            to[lane * CS + m] = bo + po;
+            ti[lane * CS + m] = bi + pi;
+        }
+        const int64_t base = (int64_t)P * 64 * NG;
+        double ra[NG], rb[NG];
+#pragma unroll
+        for (int r = 0; r < NG; ++r) {
+            const int64_t x = base + r * 64 + lane;
+            const int64_t xc = x < n ? x : n - 1;
+            ra[r] = rkp[xc];
+            rb[r] = rkmp[xc];
+        }
+#pragma unroll
+        for (int r = 0; r < NG; ++r) {
+            const int xl = r * 64 + lane;
+            const int64_t x = base + xl;
+            double y = rb[r] * to[(xl / NG) * CS + xl % NG] + ra[r] * ti[(xl / NG) * CS + xl % NG];
+            if (x >= n) y = 0.0;
+            if constexpr (SLICE) {
+                static_assert(!SLICE || NG == 8, "in-scan slicing needs 8-node cells");
+                const int64_t row = g.item_out[local];
+                if (base + r * 64 < g.npad)  // wave-uniform: the K-tile exists
+                    ax::slice_value(y, x, g.Ys + row * g.npad, g.ys_plane,
+                                    g.Ye + row * (g.npad / ax::kSplitKTile) + x / ax::kSplitKTile,
+                                    g.slices);
+            } else if (x < n) {
+                g.Y[(int64_t)g.item_out[local] * n + x] = y;
+            }
+        }
+    }
+}
+
+// Largest row (cells) the register-resident scan takes.
+constexpr int kWgScanMaxCells = 64 * kWgScanWaves * 4;
+
+template <int NG, bool SLICE>
+void launch_scan_wg(int nv_max, int64_t items, hipStream_t s, const ax::BatchOf<ax::ScanArg> &A,
+                    const double *mo, const double *mi) {
+    const dim3 grid(static_cast<unsigned>(items));
+    if (nv_max <= 64 * kWgScanWaves)
+        hipLaunchKernelGGL((yk_scan_wg_kernel<NG, SLICE, 1>), grid, dim3(kWgScanThreads), 0, s, A,
+                           mo, mi);
+    else if (nv_max <= 2 * 64 * kWgScanWaves)
+        hipLaunchKernelGGL((yk_scan_wg_kernel<NG, SLICE, 2>), grid, dim3(kWgScanThreads), 0, s, A,
+                           mo, mi);
+    else
+        hipLaunchKernelGGL((yk_scan_wg_kernel<NG, SLICE, 4>), grid, dim3(kWgScanThreads), 0, s, A,
+                           mo, mi);
+}
+
 template <int NG, bool SLICE>
 void launch_scan_batch(size_t lds, int grid, hipStream_t s, const ax::BatchOf<ax::ScanArg> &A,
                        const double *mo, const double *mi) {
@@ -850,7 +1076,9 @@ bool scan_lds_fits(int nv, int ng) {
 
 namespace ax {
 
-bool scan_slices_fused(int ng, int nv) { return ng == 8 && scan_lds_fits(nv, ng); }
+bool scan_slices_fused(int ng, int nv) {
+    return ng == 8 && (nv <= kWgScanMaxCells || scan_lds_fits(nv, ng));
+}
 
 ax_status density_batch_launch(const PlanHost *const *plans, int n, const double *const *P,
                                const double *const *Q, hipStream_t stream) {
@@ -878,7 +1106,8 @@ ax_status density_batch_launch(const PlanHost *const *plans, int n, const double
         if (len > nmax) nmax = len;
     }
     if (total == 0) return AX_OK;
-    dim3 grid(grid_for(nmax, 256, 64), total < 65535 ? static_cast<unsigned>(total) : 65535u);
+    const dim3 grid(static_cast<unsigned>((nmax + 511) / 512),
+                    total < 65535 ? static_cast<unsigned>(total) : 65535u);
     hipLaunchKernelGGL(pair_density_batch_kernel, grid, dim3(256), 0, stream, A);
     AX_CHECK_LAUNCH();
     return AX_OK;
@@ -890,8 +1119,9 @@ ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t str
         set_error("scan_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
         return AX_ERR_VALUE;
     }
-    // one LDS-scan launch per (n_gauss, in-scan slicing); grids the LDS scan
-    // does not serve go through the generic kernel one plan at a time
+    // rows of up to kWgScanMaxCells cells: one register-resident scan launch
+    // per (n_gauss, in-scan slicing); longer rows: the LDS scan, and rows the
+    // LDS scan does not hold go through the generic kernel one plan at a time
     for (int ng : {7, 8, 9}) {
         for (int sl = 0; sl < (ng == 8 ? 2 : 1); ++sl) {
             BatchOf<ScanArg> A{};
@@ -900,7 +1130,43 @@ ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t str
             const double *mo = nullptr, *mi = nullptr;
             for (int i = 0; i < n; ++i) {
                 const PlanHost *h = plans[i];
-                if (h->ng != ng || !scan_lds_fits(h->nv, h->ng) || h->n_items == 0) continue;
+                if (h->ng != ng || h->nv > kWgScanMaxCells || h->n_items == 0) continue;
+                const bool fused = h->precision == AX_PREC_SPLIT && h->Y == nullptr;
+                if (fused != (sl == 1)) continue;
+                total += h->n_items;
+                A.end[A.n] = total;
+                const int64_t yr = h->n_yrows > 0 ? h->n_yrows : 1;
+                A.a[A.n] = ScanArg{h->D, h->Y, h->rk, h->rkm, h->item_row, h->item_k,
+                                   h->item_out, h->n, h->nv, h->kmax, h->Ys, h->Ye, h->npad,
+                                   yr * h->npad, h->slices, 0};
+                ++A.n;
+                if (h->nv > nv_max) nv_max = h->nv;
+                mo = h->mo;
+                mi = h->mi;
+            }
+            if (A.n == 0) continue;
+            if (total > 2147483647LL) {
+                set_error("scan_batch_launch: %lld items exceed one launch", (long long)total);
+                return AX_ERR_VALUE;
+            }
+            if (ng == 7) launch_scan_wg<7, false>(nv_max, total, stream, A, mo, mi);
+            if (ng == 8 && sl == 0) launch_scan_wg<8, false>(nv_max, total, stream, A, mo, mi);
+            if (ng == 8 && sl == 1) launch_scan_wg<8, true>(nv_max, total, stream, A, mo, mi);
+            if (ng == 9) launch_scan_wg<9, false>(nv_max, total, stream, A, mo, mi);
+            AX_CHECK_LAUNCH();
+        }
+    }
+    for (int ng : {7, 8, 9}) {
+        for (int sl = 0; sl < (ng == 8 ? 2 : 1); ++sl) {
+            BatchOf<ScanArg> A{};
+            int64_t total = 0;
+            int nv_max = 0;
+            const double *mo = nullptr, *mi = nullptr;
+            for (int i = 0; i < n; ++i) {
+                const PlanHost *h = plans[i];
+                if (h->ng != ng || h->nv <= kWgScanMaxCells || !scan_lds_fits(h->nv, h->ng) ||
+                    h->n_items == 0)
+                    continue;
                 const bool fused = h->precision == AX_PREC_SPLIT && h->Y == nullptr;
                 if (fused != (sl == 1)) continue;
                 total += h->n_items;
@@ -926,7 +1192,8 @@ ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t str
     }
     for (int i = 0; i < n; ++i) {
         const PlanHost *h = plans[i];
-        if (scan_lds_fits(h->nv, h->ng) || h->n_items == 0) continue;
+        const bool wg = (h->ng == 7 || h->ng == 8 || h->ng == 9) && h->nv <= kWgScanMaxCells;
+        if (wg || scan_lds_fits(h->nv, h->ng) || h->n_items == 0) continue;
         ax_status st = ax_yk_batch(h->D, h->n, nullptr, h->rk, h->rkm, h->n, h->nv, h->ng,
                                    h->kmax, h->mo, h->mi, h->n_items, h->item_row, h->item_k,
                                    h->item_out, h->Y, h->n, stream);
