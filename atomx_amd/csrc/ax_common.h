@@ -34,6 +34,11 @@ size_t pool_class(size_t bytes);
 // Non-blocking stream of the calling thread on the current device, for
 // set-up copies (never destroyed).
 hipStream_t setup_stream();
+// Pinned host staging buffer of at least `bytes` (reused once the event of
+// its previous release has completed); staging_release records that event
+// on `stream` after the copies out of the buffer have been queued there.
+hipError_t staging_acquire(size_t bytes, void **ptr, int *slot);
+hipError_t staging_release(int slot, hipStream_t stream);
 
 }  // namespace ax
 

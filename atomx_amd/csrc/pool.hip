@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "ax_common.h"
 
@@ -80,6 +81,62 @@ void pool_free(void *ptr) {
 }
 
 size_t pool_class(size_t bytes) { return size_class(bytes == 0 ? 1 : bytes); }
+
+// ---------------------------------------------------------------------------
+// Pinned host staging for set-up uploads.  A buffer is handed out, filled by
+// the host, copied asynchronously, and given back with the stream position
+// (an event) after which the copy has read it; it is reused only once that
+// event has completed, so set-up never waits on the host.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Staging {
+    void *ptr;
+    size_t bytes;
+    hipEvent_t ready;  // recorded after the last copy out of the buffer
+    bool busy;
+};
+
+std::mutex g_smu;
+std::vector<Staging> g_staging;
+
+}  // namespace
+
+hipError_t staging_acquire(size_t bytes, void **ptr, int *slot) {
+    const size_t cls = size_class(bytes == 0 ? 1 : bytes);
+    std::lock_guard<std::mutex> lk(g_smu);
+    for (size_t i = 0; i < g_staging.size(); ++i) {
+        Staging &st = g_staging[i];
+        if (st.busy || st.bytes != cls) continue;
+        const hipError_t q = hipEventQuery(st.ready);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return q;
+        st.busy = true;
+        *ptr = st.ptr;
+        *slot = static_cast<int>(i);
+        return hipSuccess;
+    }
+    Staging st{nullptr, cls, nullptr, true};
+    hipError_t e = hipHostMalloc(&st.ptr, cls, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    e = hipEventCreateWithFlags(&st.ready, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)hipHostFree(st.ptr);
+        return e;
+    }
+    g_staging.push_back(st);
+    *ptr = st.ptr;
+    *slot = static_cast<int>(g_staging.size() - 1);
+    return hipSuccess;
+}
+
+hipError_t staging_release(int slot, hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g_smu);
+    Staging &st = g_staging[static_cast<size_t>(slot)];
+    const hipError_t e = hipEventRecord(st.ready, stream);
+    st.busy = false;
+    return e;
+}
 
 hipStream_t setup_stream() {
     // one non-blocking stream per (host thread, device) for plan set-up

@@ -15,6 +15,8 @@
 // order do not depend on the batch, so its results are bitwise the same
 // alone and in any batch.
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -33,6 +35,8 @@ struct ax_rk_tensor {
     int64_t footprint = 0;
     ax::PlanHost host{};
     hipEvent_t done = nullptr;  // recorded after the plan's last launch
+    hipEvent_t ready = nullptr; // recorded after the set-up upload
+    bool ready_pending = true;  // launches must still wait on `ready`
     bool captured = false;      // launched into a stream capture at least once
     std::vector<void *> allocs;
 };
@@ -56,7 +60,48 @@ void free_plan(ax_rk_tensor *p) {
     for (void *q : p->allocs) ax::pool_free(q);
     p->allocs.clear();
     if (p->done) (void)hipEventDestroy(p->done);
-    p->done = nullptr;
+    if (p->ready) (void)hipEventDestroy(p->ready);
+    p->done = p->ready = nullptr;
+}
+
+// Makes stream s wait for the set-up upload of every plan (a device-side
+// dependency; under stream capture, where an outside event cannot be waited
+// on, the host waits instead).
+ax_status await_ready(ax_rk_tensor *const *plans, int n, hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    AX_CHECK_HIP(hipStreamIsCapturing(s, &cs));
+    for (int i = 0; i < n; ++i) {
+        ax_rk_tensor *p = plans[i];
+        if (!p->ready_pending) continue;
+        if (hipEventQuery(p->ready) == hipSuccess) {
+            p->ready_pending = false;
+            continue;
+        }
+        if (cs != hipStreamCaptureStatusNone) {
+            AX_CHECK_HIP(hipEventSynchronize(p->ready));
+            p->ready_pending = false;
+        } else {
+            AX_CHECK_HIP(hipStreamWaitEvent(s, p->ready, 0));
+        }
+    }
+    return AX_OK;
+}
+
+// Partial-interval matrices of each n_gauss, computed once.
+ax_status partial_matrices_cached(int ng, const double **mo, const double **mi) {
+    static std::mutex mu;
+    static std::map<int, std::pair<std::vector<double>, std::vector<double>>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(ng);
+    if (it == cache.end()) {
+        std::vector<double> a(ng * ng), b(ng * ng);
+        ax_status st = ax_yk_partial_matrices(ng, a.data(), b.data());
+        if (st != AX_OK) return st;
+        it = cache.emplace(ng, std::make_pair(std::move(a), std::move(b))).first;
+    }
+    *mo = it->second.first.data();
+    *mi = it->second.second.data();
+    return AX_OK;
 }
 
 // Int8 slices per operand for a split-precision plan, or 0 when the plan's
@@ -289,23 +334,35 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     char *base = static_cast<char *>(blk);
     auto dptr = [&](int64_t off) { return static_cast<void *>(base + off); };
 
-    // host image of the uploaded head of the block
-    std::vector<char> img(static_cast<size_t>(head), 0);
+    // host image of the uploaded head of the block, built in pinned staging
+    void *stage = nullptr;
+    int stage_slot = -1;
+    he = ax::staging_acquire(static_cast<size_t>(head), &stage, &stage_slot);
+    if (he != hipSuccess) {
+        free_plan(plan);
+        delete plan;
+        ax::set_error("pinned staging of %lld bytes failed: %s", (long long)head,
+                      hipGetErrorString(he));
+        return AX_ERR_HIP;
+    }
+    char *img = static_cast<char *>(stage);
+    std::memset(img, 0, static_cast<size_t>(head));
     auto put = [&](int64_t off, const void *src, size_t bytes) {
-        if (bytes) std::memcpy(img.data() + off, src, bytes);
+        if (bytes) std::memcpy(img + off, src, bytes);
     };
     put(o_r, r_host, n * 8);
     put(o_w, w_host, n * 8);
     {
-        std::vector<double> mo(ng * ng), mi(ng * ng);
-        ax_status st = ax_yk_partial_matrices(ng, mo.data(), mi.data());
+        const double *mo = nullptr, *mi = nullptr;
+        ax_status st = partial_matrices_cached(ng, &mo, &mi);
         if (st != AX_OK) {
+            (void)ax::staging_release(stage_slot, ax::setup_stream());
             free_plan(plan);
             delete plan;
             return st;
         }
-        put(o_mo, mo.data(), mo.size() * 8);
-        put(o_mi, mi.data(), mi.size() * 8);
+        put(o_mo, mo, (size_t)ng * ng * 8);
+        put(o_mi, mi, (size_t)ng * ng * 8);
     }
     put(o_pa, pa.data(), pa.size() * 4);
     put(o_pb, pb.data(), pb.size() * 4);
@@ -384,15 +441,19 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     h.work = static_cast<const ax::RkWork *>(dptr(o_work));
     h.partial = static_cast<double *>(dptr(o_part));
 
-    // one upload, r-power tables on the device, wait for this stream only
+    // one asynchronous upload and the r-power tables on the set-up stream;
+    // launches wait on the `ready` event, the host does not
     hipStream_t ss = ax::setup_stream();
     ax_status st = AX_OK;
-    hipError_t e = hipMemcpyAsync(blk, img.data(), img.size(), hipMemcpyHostToDevice, ss);
+    hipError_t e = hipMemcpyAsync(blk, img, static_cast<size_t>(head), hipMemcpyHostToDevice, ss);
+    const hipError_t er = ax::staging_release(stage_slot, ss);
+    if (e == hipSuccess) e = er;
     if (e == hipSuccess)
         st = ax_grid_rpow(static_cast<const double *>(dptr(o_r)), n, kmax,
                           static_cast<double *>(dptr(o_rk)), static_cast<double *>(dptr(o_rkm)),
                           ss);
-    if (e == hipSuccess && st == AX_OK) e = hipStreamSynchronize(ss);
+    if (e == hipSuccess && st == AX_OK) e = hipEventCreateWithFlags(&plan->ready, hipEventDisableTiming);
+    if (e == hipSuccess && st == AX_OK) e = hipEventRecord(plan->ready, ss);
     if (e == hipSuccess && st == AX_OK) e = hipEventCreateWithFlags(&plan->done, hipEventDisableTiming);
     if (e != hipSuccess || st != AX_OK) {
         if (e != hipSuccess) ax::set_error("R^k tensor set-up failed: %s", hipGetErrorString(e));
@@ -436,7 +497,9 @@ int64_t ax_rk_tensor_estimate(int nv, int ng, int n_orb, const int32_t *l_host, 
 
 ax_status ax_rk_tensor_destroy(ax_rk_tensor *plan) {
     if (!plan) return AX_OK;
-    // the blocks return to the cache: wait until no launch can still use them
+    // the blocks return to the cache: wait until neither the set-up upload
+    // nor a launch can still use them
+    if (plan->ready) (void)hipEventSynchronize(plan->ready);
     if (plan->captured)
         (void)hipDeviceSynchronize();
     else if (plan->done)
@@ -467,7 +530,9 @@ ax_status ax_rk_tensor_run_stage(ax_rk_tensor *plan, int stage, const double *P,
     AX_REQUIRE(plan != nullptr, "ax_rk_tensor_run_stage: null plan");
     AX_REQUIRE(stage != AX_STAGE_DENSITY || P != nullptr, "ax_rk_tensor_run_stage: null orbitals");
     hipStream_t s = ax::as_stream(stream);
-    ax_status st = run_stage_batch(&plan, 1, stage, &P, &Q, &out, s);
+    ax_status st = await_ready(&plan, 1, s);
+    if (st != AX_OK) return st;
+    st = run_stage_batch(&plan, 1, stage, &P, &Q, &out, s);
     if (st != AX_OK) return st;
     return mark_done(&plan, 1, s);
 }
@@ -486,6 +551,8 @@ ax_status ax_rk_tensor_run_batch(ax_rk_tensor *const *plans, int n_plans, const 
     for (int i = 0; i < n_plans; ++i)
         AX_REQUIRE(plans[i] && P[i] && out[i], "ax_rk_tensor_run_batch: null entry %d", i);
     hipStream_t s = ax::as_stream(stream);
+    ax_status wst = await_ready(plans, n_plans, s);
+    if (wst != AX_OK) return wst;
     for (int b0 = 0; b0 < n_plans; b0 += ax::kMaxBatch) {
         const int nb = std::min(ax::kMaxBatch, n_plans - b0);
         const double *q[ax::kMaxBatch];

@@ -37,8 +37,8 @@ from typing import Any, Hashable, Optional, Sequence
 
 import numpy as np
 
-__all__ = ["SlaterRequest", "FockRequest", "SlaterResult", "FockResult", "Scheduler",
-           "GpuExecutor", "SlaterServer", "percentile"]
+__all__ = ["SlaterRequest", "SlaterBatchRequest", "FockRequest", "SlaterResult", "FockResult",
+           "Scheduler", "GpuExecutor", "SlaterServer", "percentile"]
 
 
 # ---------------------------------------------------------------------------
@@ -58,6 +58,21 @@ class SlaterRequest:
     state: Optional[Hashable] = None
 
     kind = "slater"
+
+
+@dataclass
+class SlaterBatchRequest:
+    """All R^k of several atoms in one request (one SCF iteration of a batch
+    of atoms or ions, say); every atom is a ``SlaterRequest`` with its own
+    grid, orbitals and resident state.  The atoms are scheduled together and
+    the result is the list of their ``SlaterResult``."""
+    atoms: Sequence[SlaterRequest]
+
+    kind = "slater_batch"
+
+    @property
+    def precision(self):
+        return self.atoms[0].precision if self.atoms else "fp64"
 
 
 @dataclass
@@ -136,6 +151,7 @@ class Scheduler:
       resident_bytes() -> int    bytes held by idle resident states
       evict(nbytes) -> int       drop idle resident states (LRU), bytes freed
       batch_key(req)             requests with equal keys may share a launch
+      batch_weight(req)          (optional) work units toward max_batch, default 1
       launch(reqs) -> handle     start the work of a batch (asynchronous)
       done(handle) -> bool
       finish(handle) -> list     results (or exceptions) in request order
@@ -194,6 +210,11 @@ class Scheduler:
                 "p99_ms": percentile(lat, 99) * 1e3 if lat else None,
                 "peak_bytes": self.peak_bytes, "peak_inflight": self.peak_inflight,
                 "launches": self.launches, "budget_bytes": self.budget}
+
+    def _weight(self, req):
+        """Work units of a request toward max_batch (atoms of a batch request)."""
+        fn = getattr(self.ex, "batch_weight", None)
+        return fn(req) if fn is not None else 1
 
     # -- scheduler thread --------------------------------------------------------
     def _run(self):
@@ -256,6 +277,7 @@ class Scheduler:
             return False
         admitted = []
         nbytes = 0
+        weight = 0
         while self._pending:
             t = self._pending[0]
             if not self.ex.ready(t.req):
@@ -273,9 +295,11 @@ class Scheduler:
                 used = self._inflight_bytes + nbytes + self.ex.resident_bytes()
                 if used + need > self.budget:
                     break  # wait for in-flight work to free memory (FIFO: no overtaking)
-            if admitted and (len(admitted) >= self.max_batch
+            w = self._weight(t.req)
+            if admitted and (weight + w > self.max_batch
                              or self.ex.batch_key(t.req) != self.ex.batch_key(admitted[0].req)):
                 break
+            weight += w
             self._pending.popleft()
             self.ex.reserve(t.req)
             t.nbytes = need
@@ -317,6 +341,7 @@ class _State:
         self.fock = None
         self.P = self.Q = self.out = None
         self.host_out = None
+        self.grid_ids = None
 
     def close(self):
         if self.plan is not None:
@@ -340,19 +365,35 @@ class GpuExecutor:
         self._prec = PRECISIONS
         self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
         self._free_streams = list(range(n_streams))
+        # pinned staging per in-flight slot: uploads and one-off results of
+        # the slot's launch set (reused once the slot's job has finished)
+        self._stage = [None] * n_streams
         self.states = collections.OrderedDict()  # resident, LRU order
+        self._idle_bytes = 0                     # bytes of idle resident states
+        self._estimates = {}                     # request shape -> (bytes, R^k count)
         self.plans_created = 0
         self._fock_cache = {}
 
     # -- footprints ----------------------------------------------------------------
     def _info(self, req):
-        """(shape key, device bytes of a fresh state), computed once per request."""
+        """(shape key, device bytes of a fresh state, R^k count), computed
+        once per request (the estimate once per request shape)."""
         info = getattr(req, "_ax_info", None)
         if info is None:
             if req.kind == "slater":
-                info = (self._slater_shape(req), self._slater_bytes(req))
+                nv, ng = np.shape(req.gr)
+                sk = (nv, ng, tuple(int(x) for x in req.ls), int(req.kmax), req.precision,
+                      req.Q is not None)
+                est = self._estimates.get(sk)
+                if est is None:
+                    n_rk = _count_rk(req.ls, int(req.kmax))
+                    est = (self._slater_bytes(req, n_rk), n_rk)
+                    self._estimates[sk] = est
+                    if len(self._estimates) > 4096:
+                        self._estimates.pop(next(iter(self._estimates)))
+                info = (self._slater_shape(req), est[0], est[1])
             else:
-                info = (None, self._fock_bytes(req))
+                info = (None, self._fock_bytes(req), 0)
             req._ax_info = info
         return info
 
@@ -365,7 +406,7 @@ class GpuExecutor:
         return ("slater", _grid_key(req.gr, req.grw), nv, ng, ls, int(req.kmax), req.precision,
                 req.Q is not None)
 
-    def _slater_bytes(self, req):
+    def _slater_bytes(self, req, n_rk):
         import ctypes
         nv, ng = np.shape(req.gr)
         ls = np.ascontiguousarray(req.ls, dtype=np.int32)
@@ -374,7 +415,6 @@ class GpuExecutor:
             self._prec[req.precision]))
         if plan < 0:
             raise ValueError("invalid Slater-integral request")
-        n_rk = _count_rk(ls, int(req.kmax))
         return plan + 8 * (2 * ls.size * nv * ng + n_rk)
 
     def _fock_bytes(self, req):
@@ -389,27 +429,49 @@ class GpuExecutor:
         return (req.kind, req.state) if req.state is not None else None
 
     def footprint(self, req):
+        if req.kind == "slater_batch":
+            return sum(self.footprint(a) for a in req.atoms)
         key = self._key(req)
         if key is not None and key in self.states:
             return self.states[key].nbytes
         return self._info(req)[1]
 
     def ready(self, req):
+        if req.kind == "slater_batch":
+            return all(self.ready(a) for a in req.atoms)
         key = self._key(req)
         return key is None or key not in self.states or not self.states[key].busy
 
+    def _set_busy(self, st, busy):
+        """Marks a resident state busy/idle, keeping the idle-bytes total."""
+        if st.busy == busy:
+            return
+        st.busy = busy
+        if st.key is not None and self.states.get(st.key) is st:
+            self._idle_bytes += -st.nbytes if busy else st.nbytes
+
     def reserve(self, req):
+        if req.kind == "slater_batch":
+            for a in req.atoms:
+                self.reserve(a)
+            return
         key = self._key(req)
         if key is None:
             return
-        if key not in self.states:  # placeholder until the launch builds it
-            shape, nbytes = self._info(req)
-            self.states[key] = _State(key, shape, nbytes)
-        self.states[key].busy = True
+        st = self.states.get(key)
+        if st is None:  # placeholder until the launch builds it
+            shape, nbytes = self._info(req)[:2]
+            st = _State(key, shape, nbytes)
+            st.busy = True
+            if req.kind == "slater":
+                st.grid_ids = (id(req.gr), id(req.grw))
+            self.states[key] = st
+        else:
+            self._set_busy(st, True)
         self.states.move_to_end(key)
 
     def resident_bytes(self):
-        return sum(s.nbytes for s in self.states.values() if not s.busy)
+        return self._idle_bytes
 
     def evict(self, nbytes):
         freed = 0
@@ -420,87 +482,132 @@ class GpuExecutor:
             if st.busy:
                 continue
             freed += st.nbytes
+            self._idle_bytes -= st.nbytes
             st.close()
             del self.states[key]
         return freed
 
+    def batch_weight(self, req):
+        return len(req.atoms) if req.kind == "slater_batch" else 1
+
     def batch_key(self, req):
-        return ("slater", req.precision) if req.kind == "slater" else ("fock", id(req))
+        if req.kind in ("slater", "slater_batch"):
+            return ("slater", req.precision)
+        return ("fock", id(req))
 
     # -- launches --------------------------------------------------------------------
     def _state_for(self, req):
         key = self._key(req)
-        shape, nbytes = self._info(req)
         if key is not None and key in self.states:
             st = self.states[key]
-            if st.shape_key != shape:
-                raise ValueError(f"state {req.state!r} was created with another shape")
-            st.busy = True
+            if st.shape_key is not None and req.kind == "slater":
+                # the grid is hashed again only when another array object is passed
+                if st.grid_ids != (id(req.gr), id(req.grw)):
+                    if self._slater_shape(req) != st.shape_key:
+                        raise ValueError(f"state {req.state!r} was created with another shape")
+                    st.grid_ids = (id(req.gr), id(req.grw))
+                else:
+                    nv, ng = np.shape(req.gr)
+                    if (st.shape_key[2:5] != (nv, ng, tuple(int(x) for x in req.ls)) or
+                            st.shape_key[5:] != (int(req.kmax), req.precision, req.Q is not None)):
+                        raise ValueError(f"state {req.state!r} was created with another shape")
+            self._set_busy(st, True)
             return st
+        shape, nbytes = self._info(req)[:2]
         st = _State(key, shape, nbytes)
         st.busy = True
+        if req.kind == "slater":
+            st.grid_ids = (id(req.gr), id(req.grw))
         if key is not None:
             self.states[key] = st
         return st
 
     def launch(self, reqs):
-        torch, dv = self._torch, self._dv
+        torch = self._torch
         sid = self._free_streams.pop(0)
         stream = self.streams[sid]
         job = {"sid": sid, "reqs": reqs, "states": [], "errors": [None] * len(reqs),
                "keep": [], "event": torch.cuda.Event()}
         with torch.cuda.stream(stream):
-            if reqs[0].kind == "slater":
+            if reqs[0].kind in ("slater", "slater_batch"):
                 self._launch_slater(reqs, job, stream)
             else:
                 self._launch_fock(reqs[0], job, stream)
             job["event"].record(stream)
         return job
 
-    def _upload(self, host, dev, keep):
-        torch = self._torch
-        src = torch.from_numpy(np.ascontiguousarray(host, dtype=np.float64)).pin_memory()
-        keep.append(src)
-        dev.copy_(src, non_blocking=True)
+    def _staging(self, sid, n):
+        """Pinned host and device float64 staging of at least n values for
+        slot sid (grown as needed, reused once the slot's job is finished)."""
+        torch, dv = self._torch, self._dv
+        buf = self._stage[sid]
+        if buf is None or buf[0].numel() < n:
+            n = max(n, 1 << 16)
+            buf = (torch.empty(n, dtype=torch.float64, pin_memory=True), dv.empty((n,)))
+            self._stage[sid] = buf
+        return buf
 
     def _launch_slater(self, reqs, job, stream):
-        from .tensor import SlaterTensor, run_batch
-        torch, dv = self._torch, self._dv
-        plans, Ps, Qs, outs = [], [], [], []
+        """One launch set: the inputs of every request are packed into the
+        slot's pinned staging and moved with one copy, the plans run as one
+        batch on raw device offsets of the slot's device staging, and all
+        results come back with one copy."""
+        from .tensor import SlaterTensor, run_batch_raw
+        entries = []  # (request index, atom index or -1, state, P, Q)
         for i, req in enumerate(reqs):
+            atoms = req.atoms if req.kind == "slater_batch" else [req]
+            got, sts = [], []
             try:
-                st = self._state_for(req)
-                if st.plan is None:
-                    st.plan = SlaterTensor(req.gr, req.grw, req.ls, req.kmax, req.precision)
-                    self.plans_created += 1
-                    n_orb, n = st.plan.n_orb, st.plan.n
-                    st.P = dv.empty((n_orb, n))
-                    st.Q = dv.empty((n_orb, n)) if req.Q is not None else None
-                    st.out = dv.empty((st.plan.size,))
-                if st.host_out is None or st.key is None:
-                    st.host_out = torch.empty(st.plan.size, dtype=torch.float64, pin_memory=True)
-                if np.shape(req.P) != (st.plan.n_orb, st.plan.n):
-                    raise ValueError(f"orbital samples must be ({st.plan.n_orb}, {st.plan.n})")
-                self._upload(req.P, st.P, job["keep"])
-                if req.Q is not None:
-                    self._upload(req.Q, st.Q, job["keep"])
-                job["states"].append(st)
-                plans.append(st.plan)
-                Ps.append(st.P)
-                Qs.append(st.Q)
-                outs.append(st.out)
+                for j, a in enumerate(atoms):
+                    st = self._state_for(a)
+                    sts.append(st)
+                    if st.plan is None:
+                        st.plan = SlaterTensor(a.gr, a.grw, a.ls, a.kmax, a.precision)
+                        self.plans_created += 1
+                    P = np.asarray(a.P, dtype=np.float64)
+                    Q = np.asarray(a.Q, dtype=np.float64) if a.Q is not None else None
+                    if P.shape != (st.plan.n_orb, st.plan.n) or (Q is not None and
+                                                                  Q.shape != P.shape):
+                        raise ValueError(
+                            f"orbital samples must be ({st.plan.n_orb}, {st.plan.n})")
+                    got.append((i, j if req.kind == "slater_batch" else -1, st, P, Q))
+                entries.extend(got)
             except Exception as exc:  # the request fails alone
                 job["errors"][i] = exc
-                job["states"].append(None)
-        if plans:
-            run_batch(plans, Ps, Qs, outs, stream=stream)
-            for st in job["states"]:
-                if st is not None:
-                    st.host_out.copy_(st.out, non_blocking=True)
+            job["states"].append(sts)
+        if not entries:
+            return
+        n_in = sum(P.size + (Q.size if Q is not None else 0) for _, _, _, P, Q in entries)
+        n_out = sum(st.plan.size for _, _, st, _, _ in entries)
+        host, dev = self._staging(job["sid"], n_in + n_out)
+        hv = host.numpy()
+        base = dev.data_ptr()
+        pp, qq, oo = [], [], []
+        off = 0
+        for _, _, st, P, Q in entries:
+            hv[off:off + P.size] = P.ravel()
+            pp.append(base + 8 * off)
+            off += P.size
+            if Q is not None:
+                hv[off:off + Q.size] = Q.ravel()
+                qq.append(base + 8 * off)
+                off += Q.size
+            else:
+                qq.append(None)
+        dev[:n_in].copy_(host[:n_in], non_blocking=True)
+        job["results"] = {}
+        for i, j, st, _, _ in entries:
+            oo.append(base + 8 * off)
+            job["results"][(i, j)] = (off, st.plan.size)
+            off += st.plan.size
+        run_batch_raw([st.plan for _, _, st, _, _ in entries], pp, qq, oo, stream=stream)
+        host[n_in:off].copy_(dev[n_in:off], non_blocking=True)
+        job["host"] = host
 
     def _launch_fock(self, req, job, stream):
         from .fock_yk import FockYk
         torch = self._torch
+        st = None
         try:
             st = self._state_for(req)
             if st.fock is None:
@@ -521,10 +628,10 @@ class GpuExecutor:
                 host[name] = h
             job["keep"].append(blocks)
             job["host"] = host
-            job["states"].append(st)
+            job["states"].append([st])
         except Exception as exc:
             job["errors"][0] = exc
-            job["states"].append(None)
+            job["states"].append([st] if st is not None else [])
 
     def done(self, job):
         return job["event"].query()
@@ -532,25 +639,25 @@ class GpuExecutor:
     def finish(self, job):
         job["event"].synchronize()
         results = []
-        for req, st, err in zip(job["reqs"], job["states"], job["errors"]):
+        for i, (req, sts, err) in enumerate(zip(job["reqs"], job["states"], job["errors"])):
             if err is not None:
                 results.append(err)
-                if st is not None:
-                    self._release(st)
-                continue
-            if req.kind == "slater":
-                vals = st.host_out.numpy()
-                if st.key is not None:  # a resident state reuses its staging buffer
-                    vals = vals.copy()
-                results.append(SlaterResult(vals, list(st.plan.offsets), list(st.plan.m)))
+            elif req.kind in ("slater", "slater_batch"):
+                hv = job["host"].numpy()  # the staging is reused: results are copied out
+                got = []
+                for j, st in enumerate(sts):
+                    o, k = job["results"][(i, j if req.kind == "slater_batch" else -1)]
+                    got.append(SlaterResult(hv[o:o + k].copy(), st.plan.offsets, st.plan.m))
+                results.append(got if req.kind == "slater_batch" else got[0])
             else:  # the pinned result buffers are handed over (the arrays keep them alive)
                 results.append(FockResult({k: v.numpy() for k, v in job["host"].items()}))
-            self._release(st)
+            for st in sts:
+                self._release(st)
         self._free_streams.append(job["sid"])
         return results
 
     def _release(self, st):
-        st.busy = False
+        self._set_busy(st, False)
         if st.key is None:
             st.close()
 
@@ -559,6 +666,7 @@ class GpuExecutor:
         for st in self.states.values():
             st.close()
         self.states.clear()
+        self._idle_bytes = 0
         self._fock_cache.clear()
 
 

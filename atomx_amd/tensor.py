@@ -173,3 +173,17 @@ def run_batch(plans, Ps, Qs, outs=None, stream=None):
     oo = arr(*[dv.ptr(o) for o in outs])
     call("ax_rk_tensor_run_batch", h, nb, pp, qq, oo, dv.stream_handle(stream))
     return outs
+
+
+def run_batch_raw(plans, P_ptrs, Q_ptrs, out_ptrs, stream=None):
+    """``run_batch`` on raw device addresses (int, or None for a missing Q)
+    whose shapes the caller has checked: (n_orb, n) FP64 orbitals and a
+    ``size``-long output per plan."""
+    nb = len(plans)
+    if nb == 0:
+        return
+    if not (len(P_ptrs) == len(Q_ptrs) == len(out_ptrs) == nb):
+        raise ValueError("one (P, Q, out) per plan")
+    arr = ctypes.c_void_p * nb
+    call("ax_rk_tensor_run_batch", arr(*[p._h.value for p in plans]), nb, arr(*P_ptrs),
+         arr(*Q_ptrs), arr(*out_ptrs), dv.stream_handle(stream))

@@ -249,3 +249,21 @@ def fe_ion_workload(seed=0, ns_range=(40, 160), ks_choices=(7, 8), r_min=1e-4, r
                      "occupations": [c[2] for c in conf], "P": P / norm[:, None],
                      "Q": Q / norm[:, None]})
     return ions
+
+
+def small_atom_requests(n_atoms, seed=0, precision="fp64"):
+    """All-R^k requests of small atoms and ions (the launch-bound regime of
+    per-atom processing): nv ~ U{40..160} cells of 8 Gauss nodes on
+    r 1e-4..30, 1..10 orbitals with l <= 3, k_max = 2 max(l) (at most 6)."""
+    from .serve import SlaterRequest
+    rng = np.random.default_rng(seed)
+    out = []
+    for a in range(n_atoms):
+        nv = int(rng.integers(40, 161))
+        n_orb = int(rng.integers(1, 11))
+        gr, grw = synthetic_grid(nv=nv, ng=8, r_min=1e-4, r_max=30.0)
+        P, Q, _ = synthetic_orbitals(gr, grw, n_orb=n_orb, seed=seed * 100003 + a)
+        ls = rng.integers(0, 4, size=n_orb).astype(np.int32)
+        kmax = int(min(6, 2 * ls.max()))
+        out.append(SlaterRequest(gr, grw, P, Q, ls, kmax, precision))
+    return out

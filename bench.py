@@ -330,6 +330,7 @@ def serving_leg(args):
                                                   / np.abs(ref[mask]))))
     total_rk = sum(n_rk.values())
     all_lat = lat["slater"] + lat["fock"]
+    scf = scf_serving(args, slater)
     return {
         "workload": (f"{len(stream)} requests, Poisson arrivals at {args.serve_rate:g}/s (seed "
                      f"{args.seed}); n_orb~logU[5,60], N~logU[1000,6000] (8 Gauss nodes per "
@@ -349,7 +350,124 @@ def serving_leg(args):
         "saturated_vs_sequential_fp64": t_seq / t_sat,
         "max_err_vs_fp64_over_max": worst_norm,
         "max_rel_err_vs_fp64_where_abs_ge_1e-3_max": worst_elem,
+        "scf_serving": scf,
     }
+
+
+def _plain_per_atom(atoms, reqs):
+    """Plain per-atom FP64: each atom's plan created beforehand, then per
+    request: pinned staging, upload, the plan's launches, copy back, wait.
+    Returns the seconds for ``reqs`` (request r belongs to atom r % len)."""
+    import time
+
+    import torch
+
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+
+    plans = [SlaterTensor(r.gr, r.grw, r.ls, r.kmax, precision="fp64") for r in atoms]
+    dev = [(dv.empty(r.P.shape), dv.empty(r.Q.shape), dv.empty((p.size,)))
+           for r, p in zip(atoms, plans)]
+    width = max(2 * r.P.size for r in atoms) + max(p.size for p in plans)
+    stage = torch.empty(width, dtype=torch.float64, pin_memory=True)
+    s_np = stage.numpy()
+
+    def plain(a, req):
+        P, Q, out = dev[a]
+        k = req.P.size
+        s_np[:k] = req.P.ravel()
+        s_np[k:2 * k] = req.Q.ravel()
+        P.copy_(stage[:k].view(P.shape), non_blocking=True)
+        Q.copy_(stage[k:2 * k].view(Q.shape), non_blocking=True)
+        plans[a].run(P, Q, out)
+        res = stage[2 * k:2 * k + out.numel()]
+        res.copy_(out, non_blocking=True)
+        torch.cuda.synchronize()
+        return res.numpy().copy()
+
+    for a, r in enumerate(atoms):  # warm-up
+        plain(a, r)
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        plain(i % len(atoms), r)
+    dt = time.perf_counter() - t0
+    for p in plans:
+        p.close()
+    return dt
+
+
+def _scf_requests(atoms, iterations, precision, seed):
+    import numpy as np
+
+    from atomx_amd.serve import SlaterRequest
+    rng = np.random.default_rng(seed)
+    out = []
+    for it in range(iterations):
+        f = 1.0 + 1e-3 * it * rng.standard_normal()
+        for a, r in enumerate(atoms):
+            out.append(SlaterRequest(r.gr, r.grw, r.P * f, r.Q * f, r.ls, r.kmax, precision,
+                                     state=("atom", a)))
+    return out
+
+
+def _served(reqs, warm):
+    """Seconds to serve ``reqs`` submitted at once (after ``warm`` untimed)."""
+    import time
+
+    from atomx_amd.serve import SlaterServer
+    with SlaterServer(max_inflight=4, max_batch=16) as srv:
+        for f in [srv.submit(r) for r in warm]:
+            f.result()
+        srv.reset_stats()
+        t0 = time.perf_counter()
+        for f in [srv.submit(r) for r in reqs]:
+            f.result()
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        st["plans_created"] = srv.stats()["plans_created"]
+    return dt, st
+
+
+def scf_serving(args, slater, iterations=6):
+    """SCF-style serving: every atom sends one all-R^k request per SCF
+    iteration (new orbitals, same grid and shells) under a resident state;
+    iteration 0 creates the states untimed, iterations 1.. are submitted at
+    once.  Compared with the plain per-atom FP64 form (``_plain_per_atom``).
+    Two atom sets: 48 atoms of the request mix (large: 1000-6000 nodes, up
+    to 60 orbitals, GPU-bound per atom) and 256 small atoms and ions (40-160
+    cells, 1-10 orbitals: launch-bound per atom), the latter served both as
+    one request per atom and as one request per group of 32 atoms per
+    iteration (``SlaterBatchRequest``)."""
+    from atomx_amd.serve import SlaterBatchRequest
+    from atomx_amd.workloads import count_rk, small_atom_requests
+
+    out = {}
+    for name, atoms in (("mix48", slater[:48]),
+                        ("small256", small_atom_requests(256, seed=args.seed + 11))):
+        reqs = _scf_requests(atoms, iterations, "split", args.seed + 7)
+        n = len(atoms)
+        n_rk = sum(count_rk(r.ls, r.kmax) for r in reqs[n:])
+        t_req, st = _served(reqs[n:], reqs[:n])
+        t_plain = _plain_per_atom(atoms, reqs[n:])
+        row = {
+            "atoms": n, "timed_iterations": iterations - 1, "rk_total": n_rk,
+            "plain_per_atom_fp64_rk_per_s": n_rk / t_plain,
+            "served_per_atom_requests_rk_per_s": n_rk / t_req,
+            "served_per_atom_requests_vs_plain": t_plain / t_req,
+            "served_per_atom_p50_ms": st["p50_ms"], "served_per_atom_p99_ms": st["p99_ms"],
+        }
+        if name == "small256":
+            group = 32
+            breqs = [SlaterBatchRequest(reqs[i:i + group]) for i in range(0, len(reqs), group)]
+            n_warm = n // group
+            t_grp, st_g = _served(breqs[n_warm:], breqs[:n_warm])
+            row.update({
+                "served_group32_requests_rk_per_s": n_rk / t_grp,
+                "served_group32_requests_vs_plain": t_plain / t_grp,
+                "served_group32_p50_ms": st_g["p50_ms"], "served_group32_p99_ms": st_g["p99_ms"],
+            })
+        out[name] = row
+    return out
 
 
 def main():

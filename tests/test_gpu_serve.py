@@ -156,3 +156,22 @@ def test_resident_state_and_budget():
     assert st["peak_bytes"] <= budget
     for r, ref in zip(res, alone):
         assert np.array_equal(r.values, ref)
+
+
+def test_batch_request_equals_its_atoms_alone():
+    """A SlaterBatchRequest (resident states per atom, mixed with single
+    requests) returns, per atom, bitwise what the atom gives alone."""
+    from atomx_amd.serve import SlaterBatchRequest, SlaterServer
+    atoms = [_slater_req(24 + 8 * i, 2 + i % 4, 2 + i % 3, 40 + i, precision="fp64",
+                         state=("b", i)) for i in range(5)]
+    single = _slater_req(30, 3, 2, 99, precision="fp64")
+    with SlaterServer(max_inflight=2, max_batch=4) as srv:
+        for it in range(2):
+            fb = srv.submit(SlaterBatchRequest(atoms))
+            fs = srv.submit(single)
+            got = fb.result()
+            assert len(got) == len(atoms)
+            for a, res in zip(atoms, got):
+                assert np.array_equal(res.values, _alone_slater(a))
+            assert np.array_equal(fs.result().values, _alone_slater(single))
+        assert srv.stats()["plans_created"] == len(atoms) + 2
