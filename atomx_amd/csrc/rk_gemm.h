@@ -39,6 +39,20 @@ struct RkWork {
     int pad;
 };
 
+// One contraction work-group: tile (ti, tj) of segment `seg`, K chunk
+// `split` of the tile's `count` chunks, written to partial slot `slot`.
+struct RkJob {
+    int seg;
+    int tile;   // ti | tj << 16
+    int split;  // split | count << 16
+    int slot;
+};
+// The partial slots [first, first + count) of one tile, summed in order.
+struct RkTileJobs {
+    int first;
+    int count;
+};
+
 constexpr int kSplitSlices = 7;      // int8 slices per operand (7 bits each)
 constexpr int kSplitSlicesFine = 8;  // for grids whose K-tiles span a wide radius range
 constexpr int kSplitKTile = 64;   // K-tile that shares one exponent
@@ -77,13 +91,13 @@ struct SliceArg {            // rows sliced by a separate pass
     int32_t *e;
     int slices, pad;
 };
-struct GemmArg {             // contraction tiles and split reduction
+struct GemmArg {             // contraction work-groups and split reduction
     const RkSeg *segs;
-    const RkWork *work;
-    double *partial;         // n_work * n_split * 64*64
+    const RkWork *work;      // tiles (reduction)
+    const RkJob *jobs;       // work-groups (contraction), in the launch's job order
+    const RkTileJobs *tile_jobs;
+    double *partial;         // one 64*64 tile per job slot
     double *out;             // packed R^k output
-    int n_split;
-    int pad;
 };
 
 // Launch table: end[] is the exclusive prefix of the launch's items per plan
@@ -91,19 +105,18 @@ struct GemmArg {             // contraction tiles and split reduction
 template <typename T>
 struct BatchOf {
     int n;
-    int max_split;  // contraction launches: K splits per tile in the grid
     int order;      // contraction launches: block order (kOrder*)
-    int pad;
+    int pad0, pad1;
     int64_t end[kMaxBatch];
     T a[kMaxBatch];
 };
 
-// Block orders of a contraction launch over tiles x K splits (results do not
-// depend on the order, only the L2 locality does):
+// Block orders of a contraction launch over its jobs (results do not depend
+// on the order, only the L2 locality does):
 enum {
-    kOrderLinear = 0,    // block b -> (split b / tiles, tile b % tiles)
+    kOrderLinear = 0,    // block b -> job b of the tile-fastest list
     kOrderXcd = 1,       // same, after the XCD-contiguous renumbering
-    kOrderXcdSplit = 2,  // XCD-contiguous, K split fastest
+    kOrderXcdSplit = 2,  // XCD-contiguous over the split-fastest list
     kOrderCount = 3
 };
 
@@ -118,20 +131,6 @@ __device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
-// (tile, split) of contraction block b of nb under the launch's order.
-template <typename T>
-__device__ inline int64_t gemm_block(const BatchOf<T> &A, int &split) {
-    const int64_t nb = gridDim.x, b = blockIdx.x;
-    const int64_t lb = A.order == kOrderLinear ? b : xcd_block(b, nb);
-    if (A.order == kOrderXcdSplit) {
-        split = static_cast<int>(lb % A.max_split);
-        return lb / A.max_split;
-    }
-    const int64_t tiles = A.end[A.n - 1];
-    split = static_cast<int>(lb / tiles);
-    return lb % tiles;
-}
-
 // Plan of item `it` (uniform across the caller's wave) and the item's index
 // inside that plan.
 template <typename T>
@@ -140,6 +139,27 @@ __device__ inline int batch_find(const BatchOf<T> &B, int64_t it, int64_t &local
     while (p + 1 < B.n && it >= B.end[p]) ++p;
     local = it - (p ? B.end[p - 1] : 0);
     return p;
+}
+
+// The job of this contraction work-group under the launch's block order.
+struct JobInfo {
+    int p;  // plan in the batch
+    int seg, ti, tj, split, count, slot;
+};
+__device__ inline JobInfo gemm_job(const BatchOf<GemmArg> &A) {
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t lb = A.order == kOrderLinear ? b : xcd_block(b, nb);
+    int64_t local;
+    JobInfo J;
+    J.p = batch_find(A, lb, local);
+    const RkJob j = A.a[J.p].jobs[local];
+    J.seg = j.seg;
+    J.ti = j.tile & 0xffff;
+    J.tj = static_cast<int>(static_cast<unsigned>(j.tile) >> 16);
+    J.split = j.split & 0xffff;
+    J.count = static_cast<int>(static_cast<unsigned>(j.split) >> 16);
+    J.slot = j.slot;
+    return J;
 }
 
 // Split-precision slicing of one value per lane of a wave whose 64 lanes
@@ -175,6 +195,15 @@ int gemm_tile();
 int64_t rk_gemm_partial_doubles(int n_work, int n_split);
 void upper_tiles(int m, int seg, std::vector<RkWork> &out);
 
+// Work-groups of a contraction cut into S K chunks per tile, listed
+// tile-fastest and split-fastest; the slots (and the reduction order) are
+// tile-major and do not depend on the list.
+struct JobTables {
+    std::vector<RkJob> tile_fastest, split_fastest;
+    std::vector<RkTileJobs> tiles;
+};
+JobTables make_jobs(const std::vector<RkWork> &work, int S);
+
 // Host copy of a plan: sizes and the device pointers the launchers need.
 struct PlanHost {
     int64_t n, npad;
@@ -189,6 +218,9 @@ struct PlanHost {
     int32_t *De, *Ye;
     const RkSeg *segs;
     const RkWork *work;
+    const RkJob *jobs;      // the schedule's job list (order and K-split level)
+    const RkTileJobs *tile_jobs;
+    int n_jobs;
     double *partial;
 };
 // Whether the scan of a split-precision plan on n_gauss nodes per cell writes

@@ -3,18 +3,16 @@
 //
 // Work decomposition: a list of segments (one per rank k, per request, ...),
 // each an m x m symmetric-structured output; every upper 64x64 tile of every
-// segment is one work entry; K is split into `n_split` contiguous chunks.
-// Partial tiles are written to a workspace and summed over the splits in a
+// segment is one work entry, and its K range is cut into `n_split`
+// contiguous chunks (one job = one work-group = one chunk of one tile).
+// Partial tiles are written to a workspace and summed over the chunks in a
 // fixed order by rk_reduce_kernel, so a result depends only on its own
-// segment's data (never on what else is in the batch).
+// segment's data and the schedule (never on what else is in the batch).
 //
-// FP64 path: v_mfma_f64_16x16x4_f64.  256 threads = 4 waves in a 2x2 layout,
-// each wave owns a 32x32 sub-tile = 2x2 MFMA tiles; K is staged through LDS in
-// chunks of 16 in k-major layout, double buffered (35 KB of LDS: four
-// work-groups per CU).  A 16x16 block that lies wholly below the diagonal of
-// its segment or past its last row/column is not stored, and a wave whose
-// four blocks are all such blocks skips its MFMAs (the lower-left wave of a
-// diagonal tile, waves past the segment edge).
+// FP64 path: v_mfma_f64_16x16x4_f64, K split across the four waves of a
+// work-group (below).
+#include <algorithm>
+
 #include "ax_common.h"
 #include "rk_gemm.h"
 
@@ -23,159 +21,196 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64;
-constexpr int BK = 16;
-constexpr int LDS_LD = BM + 4;  // padded row (in doubles) of the k-major LDS tiles
-
-constexpr int KPT = BK * BM / 256;  // k values per thread in one staged tile (4)
-
 // Global-address-space view of device memory: loads through it are
-// global_load (vector memory counter only), so waiting on LDS reads never
-// waits on the prefetch of the next K stage.
+// global_load (vector memory counter only), never flat loads that share a
+// counter with LDS operations.
 typedef const double __attribute__((address_space(1))) gdouble;
 
-// Row pointer of this thread's staged row (thread t -> row t/4); rows past
-// the segment's last row read its row 0 and are zeroed when staged to LDS.
-__device__ inline gdouble *stage_row(const double *__restrict__ base, int64_t ld,
-                                     const int32_t *__restrict__ rows, int row0, int m,
-                                     bool &valid) {
-    const int gr = row0 + (threadIdx.x >> 2);
-    valid = gr < m;
-    const int r = valid ? gr : 0;
-    const int64_t rr = rows ? (int64_t)rows[r] : (int64_t)r;
-    return (gdouble *)(base + rr * ld);
-}
+// ---------------------------------------------------------------------------
+// FP64 contraction.  Every wave of the work-group computes the
+// whole 64x64 tile (4x4 blocks of 16x16) over its own quarter of the block's
+// K chunk, with operands loaded straight from global memory into registers
+// (no LDS and no barrier in the K loop); the four partial tiles are summed
+// through LDS at the end.  Blocks wholly below the diagonal or past the
+// segment's last row or column are skipped at 16x16 granularity, uniformly
+// for all four waves, so the waves of a work-group carry equal work.
+//
+// K order inside a stage of 8: MFMA step s (0, 1) takes, in lane group
+// g = lane / 16, node k0 + 2g + s, so each lane reads two consecutive nodes
+// of each of its 8 rows per stage (A and B use the same assignment, so the
+// products pair up).
+// ---------------------------------------------------------------------------
+constexpr int KW = 8;         // nodes per register stage
+constexpr int KW_ALIGN = 32;  // work-group K chunks are multiples of 4 waves x KW
 
-// Loads this thread's KPT k values [k0 + kq, k0 + kq + KPT) of its row (zero
-// past kend).  The full-tile test is uniform over the work-group.
-__device__ inline void load_stage(gdouble *__restrict__ p, int64_t k0, int64_t kend,
-                                  double (&v)[KPT]) {
-    const int kq = (threadIdx.x & 3) * KPT;
-    if (k0 + BK <= kend) {
+struct KwOperands {
+    double a[4][2], b[4][2];
+};
+
+__device__ inline void kw_load(gdouble *const (&pa)[4], gdouble *const (&pb)[4], int64_t k,
+                               KwOperands &o) {
 #pragma unroll
-        for (int q = 0; q < KPT; ++q) v[q] = p[k0 + kq + q];
-    } else {
-#pragma unroll
-        for (int q = 0; q < KPT; ++q) {
-            const int64_t kx = k0 + kq + q;
-            v[q] = kx < kend ? p[kx] : 0.0;
-        }
+    for (int i = 0; i < 4; ++i) {
+        o.a[i][0] = pa[i][k];
+        o.a[i][1] = pa[i][k + 1];
+        o.b[i][0] = pb[i][k];
+        o.b[i][1] = pb[i][k + 1];
     }
 }
 
-// Stages the loaded values k-major into LDS (zeros for a row past the end).
-__device__ inline void store_tile_lds(double *__restrict__ s, const double (&v)[KPT],
-                                      bool valid) {
-    const int t = threadIdx.x;
-    const int r = t >> 2;
-    const int kq = (t & 3) * KPT;
+// Marks the loaded operands as used here (an empty asm that reads and writes
+// them), after the MFMAs that overlap their latency.
+__device__ inline void kw_pin(KwOperands &o) {
 #pragma unroll
-    for (int q = 0; q < KPT; ++q) s[(kq + q) * LDS_LD + r] = valid ? v[q] : 0.0;
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) asm volatile("" : "+v"(o.a[i][q]), "+v"(o.b[i][q]));
 }
 
-__global__ __launch_bounds__(256) void rk_gemm_f64_kernel(const ax::BatchOf<ax::GemmArg> A) {
-    __shared__ double As[2][BK * LDS_LD];
-    __shared__ double Bs[2][BK * LDS_LD];
+// MODE 0: all 16 blocks; 1: blocks a <= b (diagonal tile); 2: blocks set in
+// the wave-uniform mask `msk` (bit 4a + b).
+template <int MODE>
+__device__ inline void kw_mfma(const KwOperands &o, d4 (&acc)[4][4], int msk) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if (MODE == 1 && a > b) continue;
+                if (MODE == 2 && !(msk & (1 << (4 * a + b)))) continue;
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a[a][s], o.b[b][s], acc[a][b],
+                                                                 0, 0, 0);
+            }
+}
 
-    // one 1-D grid of tiles x max_split blocks in the launch's block order
-    int split;
-    const int64_t tile = ax::gemm_block(A, split);
-    int64_t e;
-    const int p = ax::batch_find(A, tile, e);
-    const ax::GemmArg &g = A.a[p];
-    const int n_split = g.n_split;
-    if (split >= n_split) return;
-    const ax::RkWork wk = g.work[e];
-    const ax::RkSeg sg = g.segs[wk.seg];
-    const int64_t chunk = ((sg.n + n_split - 1) / n_split + BK - 1) / BK * BK;
-    const int64_t kbeg = (int64_t)split * chunk;
+template <int MODE>
+__device__ inline void kw_loop(gdouble *const (&pa)[4], gdouble *const (&pb)[4], int64_t kb,
+                               int64_t ke, d4 (&acc)[4][4], int msk) {
+    const int64_t g2 = (threadIdx.x & 63) >> 4 << 1;
+    const int64_t nfull = ke > kb ? (ke - kb) / KW : 0;
+    if (nfull > 0) {
+        KwOperands cur, nxt;
+        kw_load(pa, pb, kb + g2, cur);
+        for (int64_t s = 0; s < nfull; ++s) {
+            // prefetch the next stage (a valid stage of this range when none is left)
+            const int64_t kn = s + 1 < nfull ? kb + (s + 1) * KW : kb;
+            kw_load(pa, pb, kn + g2, nxt);
+            kw_mfma<MODE>(cur, acc, msk);
+            // keeps the prefetch in this iteration, behind all of its MFMAs:
+            // without the pin the loads are sunk to their use in the next
+            // iteration, without the barrier the wait is hoisted among the
+            // MFMAs, and either way their latency is exposed
+            __builtin_amdgcn_sched_barrier(0);
+            kw_pin(nxt);
+            cur = nxt;
+        }
+    }
+    const int64_t kt = kb + nfull * KW;
+    if (kt < ke) {  // uniform: a last partial stage, zero past ke
+        KwOperands t;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int64_t kx = kt + g2 + q;
+            const int64_t kc = kx < ke ? kx : ke - 1;
+            const bool in = kx < ke;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double va = pa[i][kc], vb = pb[i][kc];
+                t.a[i][q] = in ? va : 0.0;
+                t.b[i][q] = in ? vb : 0.0;
+            }
+        }
+        kw_mfma<MODE>(t, acc, msk);
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void rk_gemm_f64_kernel(const ax::BatchOf<ax::GemmArg> A) {
+    __shared__ double park[2][16 * 4 * 64];
+
+    const ax::JobInfo J = ax::gemm_job(A);
+    const ax::GemmArg &g = A.a[J.p];
+    const ax::RkSeg sg = g.segs[J.seg];
+    const int64_t chunk =
+        ((sg.n + J.count - 1) / J.count + KW_ALIGN - 1) / KW_ALIGN * KW_ALIGN;
+    const int64_t kbeg = (int64_t)J.split * chunk;
     const int64_t kend = kbeg + chunk < sg.n ? kbeg + chunk : sg.n;
-    const int row0 = wk.ti * BM, col0 = wk.tj * BM;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t wch = chunk / 4;
+    const int64_t kb0 = kbeg + wave * wch;
+    const int64_t kb = kb0 < kend ? kb0 : kend;
+    const int64_t ke = kb0 + wch < kend ? kb0 + wch : kend;
+    const int row0 = J.ti * BM, col0 = J.tj * BM;
+    const int m = sg.m;
 
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const int fr = lane & 15, fk = lane >> 4;
-
-    // wave-uniform mask of the 16x16 blocks that hold upper-triangle
-    // elements of the segment (i <= j < m); bit 2a+b for block (a, b)
+    // active 16x16 blocks: rows and columns inside the segment, a <= b on
+    // the diagonal tile
+    const int rb = min(4, (m - row0 + 15) >> 4), cb = min(4, (m - col0 + 15) >> 4);
+    const bool diag = J.ti == J.tj;
     int bits = 0;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int i0 = row0 + wr + 16 * a, j0 = col0 + wc + 16 * b;
-            if (i0 < sg.m && j0 < sg.m && i0 <= j0 + 15) bits |= 1 << (2 * a + b);
-        }
+        for (int b = 0; b < 4; ++b)
+            if (a < rb && b < cb && (!diag || a <= b)) bits |= 1 << (4 * a + b);
     const int msk = __builtin_amdgcn_readfirstlane(bits);
+    const int mode = __builtin_amdgcn_readfirstlane(
+        rb == 4 && cb == 4 ? (diag ? 1 : 0) : 2);
 
-    d4 acc[2][2];
+    const int fr = lane & 15;
+    gdouble *pa[4];
+    gdouble *pb[4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < 4; ++i) {
+        const int ra = min(row0 + 16 * i + fr, m - 1);
+        const int64_t rr = sg.rows ? (int64_t)sg.rows[ra] : (int64_t)ra;
+        pa[i] = (gdouble *)(sg.D + rr * sg.ld_d);
+        const int cc = min(col0 + 16 * i + fr, m - 1);
+        pb[i] = (gdouble *)(sg.Y + (int64_t)cc * sg.ld_y);
+    }
 
-    bool va_ok, vb_ok;
-    gdouble *pa = stage_row(sg.D, sg.ld_d, sg.rows, row0, sg.m, va_ok);
-    gdouble *pb = stage_row(sg.Y, sg.ld_y, nullptr, col0, sg.m, vb_ok);
-    double va[KPT], vb[KPT];
-    int buf = 0;
-    if (kbeg < kend) {
-        load_stage(pa, kbeg, kend, va);
-        load_stage(pb, kbeg, kend, vb);
-        store_tile_lds(As[0], va, va_ok);
-        store_tile_lds(Bs[0], vb, vb_ok);
+    d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    if (mode == 0)
+        kw_loop<0>(pa, pb, kb, ke, acc, msk);
+    else if (mode == 1)
+        kw_loop<1>(pa, pb, kb, ke, acc, msk);
+    else
+        kw_loop<2>(pa, pb, kb, ke, acc, msk);
+
+    // sum the four waves' tiles in a fixed order: (w0 + w2) + (w1 + w3)
+#define KW_FOR_ACTIVE(BODY)                                   \
+    _Pragma("unroll") for (int a = 0; a < 4; ++a)             \
+    _Pragma("unroll") for (int b = 0; b < 4; ++b)             \
+    if (msk & (1 << (4 * a + b)))                             \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r) { BODY; }
+    const int slot = lane;
+    if (wave >= 2) {
+        double *dst = park[wave - 2];
+        KW_FOR_ACTIVE(dst[((4 * a + b) * 4 + r) * 64 + slot] = acc[a][b][r])
     }
     __syncthreads();
-    for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
-        const int64_t kn = k0 + BK;
-        if (kn < kend) {
-            load_stage(pa, kn, kend, va);
-            load_stage(pb, kn, kend, vb);
-        }
-        if (msk) {
-            const double *as = As[buf];
-            const double *bs = Bs[buf];
-            double a0[BK / 4], a1[BK / 4], b0[BK / 4], b1[BK / 4];
-#pragma unroll
-            for (int kk = 0; kk < BK / 4; ++kk) {
-                const int kr = (kk * 4 + fk) * LDS_LD;
-                a0[kk] = as[kr + wr + fr];
-                a1[kk] = as[kr + wr + 16 + fr];
-                b0[kk] = bs[kr + wc + fr];
-                b1[kk] = bs[kr + wc + 16 + fr];
-            }
-#pragma unroll
-            for (int kk = 0; kk < BK / 4; ++kk) {
-                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[kk], b0[kk], acc[0][0], 0, 0, 0);
-                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[kk], b1[kk], acc[0][1], 0, 0, 0);
-                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[kk], b0[kk], acc[1][0], 0, 0, 0);
-                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[kk], b1[kk], acc[1][1], 0, 0, 0);
-            }
-        }
-        if (kn < kend) {
-            store_tile_lds(As[buf ^ 1], va, va_ok);
-            store_tile_lds(Bs[buf ^ 1], vb, vb_ok);
-        }
-        __syncthreads();
-        buf ^= 1;
+    if (wave < 2) {
+        const double *src = park[wave];
+        KW_FOR_ACTIVE(acc[a][b][r] += src[((4 * a + b) * 4 + r) * 64 + slot])
     }
-
-    // C layout of the 16x16x4 f64 MFMA: result register r of lane l holds
-    // row (l/16) + 4r, column l%16 (one-row groups, unlike the 4-row groups
-    // of the i32/f32 16x16 MFMAs).  Skipped blocks are never read back.
-    double *dst = g.partial + (e * n_split + split) * (BM * BM);
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            if (!(msk & (1 << (2 * a + b)))) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = wr + a * 16 + fk + 4 * r;
-                const int j = wc + b * 16 + fr;
-                dst[i * BM + j] = acc[a][b][r];
-            }
-        }
+    __syncthreads();
+    if (wave == 1) {
+        double *dst = park[0];
+        KW_FOR_ACTIVE(dst[((4 * a + b) * 4 + r) * 64 + slot] = acc[a][b][r])
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const double *src = park[0];
+        double *out = g.partial + (int64_t)J.slot * (BM * BM);
+        const int fk = lane >> 4;
+        KW_FOR_ACTIVE(out[(16 * a + fk + 4 * r) * BM + 16 * b + fr] =
+                          acc[a][b][r] + src[((4 * a + b) * 4 + r) * 64 + slot])
+    }
+#undef KW_FOR_ACTIVE
 }
 
 // Sums the split partials of each tile in split order and writes the packed
@@ -186,9 +221,9 @@ __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::BatchOf<ax::Ge
     const ax::GemmArg &g = A.a[p];
     const ax::RkWork wk = g.work[e];
     const ax::RkSeg sg = g.segs[wk.seg];
-    const int n_split = g.n_split;
+    const ax::RkTileJobs tj = g.tile_jobs[e];
     const int64_t m = sg.m;
-    const double *src = g.partial + e * n_split * (BM * BM);
+    const double *src = g.partial + (int64_t)tj.first * (BM * BM);
     double *out = g.out + sg.out_off;
     const int per = (BM * BM) / gridDim.y;
     for (int t = blockIdx.y * per + threadIdx.x; t < (blockIdx.y + 1) * per; t += blockDim.x) {
@@ -197,7 +232,7 @@ __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::BatchOf<ax::Ge
         const int64_t j = (int64_t)wk.tj * BM + lj;
         if (i >= m || j >= m || j < i) continue;
         double s = src[t];
-        for (int sp = 1; sp < n_split; ++sp) s += src[(int64_t)sp * BM * BM + t];
+        for (int sp = 1; sp < tj.count; ++sp) s += src[(int64_t)sp * BM * BM + t];
         out[i * m - i * (i - 1) / 2 + (j - i)] = s;
     }
 }
@@ -214,20 +249,19 @@ int64_t rk_gemm_partial_doubles(int n_work, int n_split) {
 
 namespace {
 
-// Fills the per-plan prefix of contraction tiles; returns the total tile
-// count (the largest K split of the batch goes to A.max_split).
+// Fills the per-plan prefix of contraction jobs (or, for the reduction, of
+// tiles); returns the total.
 int64_t fill_work(BatchOf<GemmArg> &A, const PlanHost *const *plans, int n,
-                  double *const *out) {
+                  double *const *out, bool jobs) {
     A.n = n;
-    A.max_split = 1;
     A.order = plans[0]->order;
     int64_t total = 0;
     for (int i = 0; i < n; ++i) {
         const PlanHost *h = plans[i];
-        total += h->n_work;
+        total += jobs ? h->n_jobs : h->n_work;
         A.end[i] = total;
-        A.a[i] = GemmArg{h->segs, h->work, h->partial, out ? out[i] : nullptr, h->n_split, 0};
-        if (h->n_split > A.max_split) A.max_split = h->n_split;
+        A.a[i] = GemmArg{h->segs, h->work, h->jobs, h->tile_jobs, h->partial,
+                         out ? out[i] : nullptr};
     }
     return total;
 }
@@ -263,13 +297,14 @@ ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t
                 sub[ns++] = plans[i];
         if (ns == 0) continue;
         BatchOf<GemmArg> A{};
-        const int64_t total = fill_work(A, sub, ns, nullptr);
+        const int64_t total = fill_work(A, sub, ns, nullptr, true);
         if (total == 0) continue;
-        if (total * A.max_split > 0x7fffffff) {
-            set_error("contract_batch_launch: %lld tiles exceed one launch", (long long)total);
+        if (total > 0x7fffffff) {
+            set_error("contract_batch_launch: %lld work-groups exceed one launch",
+                      (long long)total);
             return AX_ERR_VALUE;
         }
-        const dim3 grid(static_cast<unsigned>(total * A.max_split));
+        const dim3 grid(static_cast<unsigned>(total));
         if (kd[0] == AX_PREC_FP64) {
             hipLaunchKernelGGL(rk_gemm_f64_kernel, grid, dim3(256), 0, stream, A);
             AX_CHECK_LAUNCH();
@@ -289,7 +324,7 @@ ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const
         return AX_ERR_VALUE;
     }
     BatchOf<GemmArg> A{};
-    const int64_t total = fill_work(A, plans, n, out);
+    const int64_t total = fill_work(A, plans, n, out, false);
     if (total == 0) return AX_OK;
     hipLaunchKernelGGL(rk_reduce_kernel, dim3(static_cast<unsigned>(total), 16), dim3(256), 0,
                        stream, A);
@@ -301,6 +336,34 @@ void upper_tiles(int m, int seg, std::vector<RkWork> &out) {
     const int T = (m + BM - 1) / BM;
     for (int ti = 0; ti < T; ++ti)
         for (int tj = ti; tj < T; ++tj) out.push_back(RkWork{seg, ti, tj, 0});
+}
+
+JobTables make_jobs(const std::vector<RkWork> &work, int S) {
+    JobTables t;
+    const int nw = static_cast<int>(work.size());
+    std::vector<int> cnt(nw);
+    int first = 0, most = 1;
+    t.tiles.resize(nw);
+    for (int e = 0; e < nw; ++e) {
+        // every tile in S chunks (weighting the chunk count by the tile's
+        // blocks inside the triangle was measured no faster)
+        cnt[e] = S;
+        t.tiles[e] = RkTileJobs{first, cnt[e]};
+        first += cnt[e];
+        most = std::max(most, cnt[e]);
+    }
+    auto job = [&](int e, int sp) {
+        const RkWork &w = work[e];
+        return RkJob{w.seg, w.ti | (w.tj << 16), sp | (cnt[e] << 16), t.tiles[e].first + sp};
+    };
+    t.tile_fastest.reserve(first);
+    t.split_fastest.reserve(first);
+    for (int sp = 0; sp < most; ++sp)
+        for (int e = 0; e < nw; ++e)
+            if (sp < cnt[e]) t.tile_fastest.push_back(job(e, sp));
+    for (int e = 0; e < nw; ++e)
+        for (int sp = 0; sp < cnt[e]; ++sp) t.split_fastest.push_back(job(e, sp));
+    return t;
 }
 
 }  // namespace ax

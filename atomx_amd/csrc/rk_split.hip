@@ -109,21 +109,15 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax
     int8_t *Bs = lds_split + S * PLANEB;   // [S * PLANEB]
     __shared__ int eAs[BM], eBs[BM];
 
-    // one 1-D grid of tiles x max_split blocks in the launch's block order
-    int split;
-    const int64_t tile = ax::gemm_block(A, split);
-    int64_t e;
-    const int p = ax::batch_find(A, tile, e);
-    const ax::GemmArg &g = A.a[p];
-    const int n_split = g.n_split;
-    if (split >= n_split) return;
-    const ax::RkWork wk = g.work[e];
-    const ax::RkSeg sg = g.segs[wk.seg];
+    // one 1-D grid over the launch's jobs
+    const ax::JobInfo J = ax::gemm_job(A);
+    const ax::GemmArg &g = A.a[J.p];
+    const ax::RkSeg sg = g.segs[J.seg];
     const int64_t n_tiles = (sg.n + KT - 1) / KT;
-    const int64_t per = (n_tiles + n_split - 1) / n_split;
-    const int64_t tb = (int64_t)split * per;
+    const int64_t per = (n_tiles + J.count - 1) / J.count;
+    const int64_t tb = (int64_t)J.split * per;
     const int64_t te = tb + per < n_tiles ? tb + per : n_tiles;
-    const int row0 = wk.ti * BM, col0 = wk.tj * BM;
+    const int row0 = J.ti * BM, col0 = J.tj * BM;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -211,7 +205,7 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax
         }
     }
 
-    double *dst = g.partial + (e * n_split + split) * (BM * BM);
+    double *dst = g.partial + (int64_t)J.slot * (BM * BM);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll

@@ -34,6 +34,11 @@ struct ax_rk_tensor {
     int64_t out_size = 0;
     int64_t footprint = 0;
     ax::PlanHost host{};
+    // job lists of every K-split level S = 1..split_cap (index S - 1), on
+    // the device: tile-fastest, split-fastest, per-tile slot ranges
+    std::vector<const ax::RkJob *> jobs_tf, jobs_sf;
+    std::vector<const ax::RkTileJobs *> tile_jobs;
+    std::vector<int> n_jobs;
     hipEvent_t done = nullptr;  // recorded after the plan's last launch
     hipEvent_t ready = nullptr; // recorded after the set-up upload
     bool ready_pending = true;  // launches must still wait on `ready`
@@ -55,6 +60,15 @@ struct Layout {
         return off;
     }
 };
+
+// Points the launch descriptors at the job list of the plan's schedule.
+void apply_schedule(ax_rk_tensor *p) {
+    const int S = p->host.n_split;
+    p->host.jobs = p->host.order == ax::kOrderXcdSplit ? p->jobs_sf[S - 1]
+                                                                         : p->jobs_tf[S - 1];
+    p->host.tile_jobs = p->tile_jobs[S - 1];
+    p->host.n_jobs = p->n_jobs[S - 1];
+}
 
 void free_plan(ax_rk_tensor *p) {
     for (void *q : p->allocs) ax::pool_free(q);
@@ -130,9 +144,9 @@ int choose_slices(const double *r, int nv, int ng, int kmax) {
 }
 
 int choose_split(int n_work, int64_t n) {
-    // about one round of work-groups (four per CU on 256 CUs), K chunks of
-    // at least 128 nodes, at most 16 chunks
-    int64_t s = 1024 / std::max(n_work, 1);
+    // about one round of FP64 contraction work-groups (two per CU on 256
+    // CUs), K chunks of at least 128 nodes, at most 16 chunks
+    int64_t s = 512 / std::max(n_work, 1);
     s = std::min<int64_t>(s, n / 128);
     s = std::min<int64_t>(s, 16);
     return static_cast<int>(std::max<int64_t>(s, 1));
@@ -269,7 +283,10 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     }
     const int n_work = static_cast<int>(work.size());
     const int n_split = choose_split(n_work, n);
-    const int split_cap = std::min(16, 2 * n_split);  // partials sized for tuning up to here
+    const int split_cap = std::min(16, 4 * n_split);  // partials sized for tuning up to here
+    AX_REQUIRE(n_work == 0 || work.back().tj < 0x8000, "ax_rk_tensor_create: too many tiles");
+    std::vector<ax::JobTables> jobs;
+    for (int S = 1; S <= split_cap; ++S) jobs.push_back(ax::make_jobs(work, S));
 
     // one device block for everything the plan owns
     Layout L;
@@ -283,6 +300,12 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     const int64_t o_iout = L.add((int64_t)it_out.size() * 4);
     const int64_t o_segs = L.add((int64_t)seg_k.size() * sizeof(ax::RkSeg));
     const int64_t o_work = L.add((int64_t)n_work * sizeof(ax::RkWork));
+    std::vector<int64_t> o_jtf, o_jsf, o_tj;
+    for (const ax::JobTables &t : jobs) {
+        o_jtf.push_back(L.add((int64_t)t.tile_fastest.size() * sizeof(ax::RkJob)));
+        o_jsf.push_back(L.add((int64_t)t.split_fastest.size() * sizeof(ax::RkJob)));
+        o_tj.push_back(L.add((int64_t)t.tiles.size() * sizeof(ax::RkTileJobs)));
+    }
     const int64_t head = L.size;  // everything above is uploaded from the host
     // split precision on 8-node cells: the scan writes Y^k slices only
     const bool y_fp64 = !(precision == AX_PREC_SPLIT && ax::scan_slices_fused(ng, nv));
@@ -406,6 +429,16 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
         }
         put(o_segs, segs.data(), segs.size() * sizeof(ax::RkSeg));
         put(o_work, work.data(), work.size() * sizeof(ax::RkWork));
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            const ax::JobTables &t = jobs[i];
+            put(o_jtf[i], t.tile_fastest.data(), t.tile_fastest.size() * sizeof(ax::RkJob));
+            put(o_jsf[i], t.split_fastest.data(), t.split_fastest.size() * sizeof(ax::RkJob));
+            put(o_tj[i], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
+            plan->jobs_tf.push_back(static_cast<const ax::RkJob *>(dptr(o_jtf[i])));
+            plan->jobs_sf.push_back(static_cast<const ax::RkJob *>(dptr(o_jsf[i])));
+            plan->tile_jobs.push_back(static_cast<const ax::RkTileJobs *>(dptr(o_tj[i])));
+            plan->n_jobs.push_back(static_cast<int>(t.tile_fastest.size()));
+        }
     }
     ax::PlanHost &h = plan->host;
     h.n = n;
@@ -440,6 +473,7 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     h.segs = static_cast<const ax::RkSeg *>(dptr(o_segs));
     h.work = static_cast<const ax::RkWork *>(dptr(o_work));
     h.partial = static_cast<double *>(dptr(o_part));
+    apply_schedule(plan);
 
     // one asynchronous upload and the r-power tables on the set-up stream;
     // launches wait on the `ready` event, the host does not
@@ -583,6 +617,7 @@ ax_status ax_rk_tensor_set_schedule(ax_rk_tensor *plan, int order, int n_split) 
     plan->host.order = order;
     plan->host.n_split = n_split;
     plan->n_split = n_split;
+    apply_schedule(plan);
     return AX_OK;
 }
 
@@ -614,6 +649,7 @@ ax_status ax_rk_tensor_tune(ax_rk_tensor *plan, const double *P, const double *Q
         for (int order = 0; order < ax::kOrderCount && st == AX_OK; ++order) {
             plan->host.order = order;
             plan->host.n_split = split;
+            apply_schedule(plan);
             for (int r = 0; r <= reps && st == AX_OK; ++r) {  // r = 0 warms up
                 if (r == 1 && hipEventRecord(e0, s) != hipSuccess) st = AX_ERR_HIP;
                 if (st == AX_OK)
@@ -639,6 +675,7 @@ ax_status ax_rk_tensor_tune(ax_rk_tensor *plan, const double *P, const double *Q
     plan->host.order = st == AX_OK ? best_order : save_order;
     plan->host.n_split = st == AX_OK ? best_split : save_split;
     plan->n_split = plan->host.n_split;
+    apply_schedule(plan);
     if (st != AX_OK) return st;
     if (best_ms_host) *best_ms_host = best / reps;
     // leave `out` holding the result of the chosen schedule
@@ -654,7 +691,7 @@ ax_status ax_rk_tensor_tune(ax_rk_tensor *plan, const double *P, const double *Q
 namespace {
 
 struct GemmWorkspace {
-    int64_t o_seg, o_work, o_part, o_Ds, o_Ys, o_De, o_Ye, size;
+    int64_t o_seg, o_work, o_jobs, o_tj, o_part, o_Ds, o_Ys, o_De, o_Ye, size;
 };
 
 GemmWorkspace gemm_workspace(int m, int64_t n, int precision, int n_work, int n_split) {
@@ -662,6 +699,8 @@ GemmWorkspace gemm_workspace(int m, int64_t n, int precision, int n_work, int n_
     GemmWorkspace g{};
     g.o_seg = L.add(sizeof(ax::RkSeg));
     g.o_work = L.add((int64_t)n_work * sizeof(ax::RkWork));
+    g.o_jobs = L.add((int64_t)n_work * n_split * sizeof(ax::RkJob));
+    g.o_tj = L.add((int64_t)n_work * sizeof(ax::RkTileJobs));
     g.o_part = L.add(ax::rk_gemm_partial_doubles(n_work, n_split) * 8);
     if (precision == AX_PREC_SPLIT) {
         const int64_t npad = (n + ax::kSplitKTile - 1) / ax::kSplitKTile * ax::kSplitKTile;
@@ -688,7 +727,8 @@ int64_t ax_rk_gemm_workspace_bytes(int m, int64_t n, int precision) {
 ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const double *Y,
                      int64_t ld_y, int m, int64_t n, int precision, void *workspace, double *out,
                      void *stream) {
-    AX_REQUIRE(m >= 0 && n > 0 && ld_d >= n && ld_y >= n, "ax_rk_gemm: bad sizes");
+    AX_REQUIRE(m >= 0 && m < 0x8000 * 64 && n > 0 && ld_d >= n && ld_y >= n,
+               "ax_rk_gemm: bad sizes");
     AX_REQUIRE(precision == AX_PREC_FP64 || precision == AX_PREC_SPLIT,
                "ax_rk_gemm: unknown precision %d", precision);
     if (m == 0) return AX_OK;
@@ -737,6 +777,13 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
         sg.ye_ld = tiles;
     }
     AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_seg, &sg, sizeof(sg), hipMemcpyHostToDevice, s));
+    const ax::JobTables jt = ax::make_jobs(work, split);
+    AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_jobs, jt.tile_fastest.data(),
+                                jt.tile_fastest.size() * sizeof(ax::RkJob), hipMemcpyHostToDevice,
+                                s));
+    AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_tj, jt.tiles.data(),
+                                jt.tiles.size() * sizeof(ax::RkTileJobs), hipMemcpyHostToDevice,
+                                s));
     AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_work, work.data(), work.size() * sizeof(ax::RkWork),
                                 hipMemcpyHostToDevice, s));
     ax::PlanHost h{};
@@ -744,9 +791,13 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
     h.n_work = nw;
     h.n_split = split;
     h.precision = precision;
+    h.order = ax::kOrderLinear;
     h.slices = ax::kSplitSlices;
     h.segs = reinterpret_cast<const ax::RkSeg *>(ws + g.o_seg);
     h.work = reinterpret_cast<const ax::RkWork *>(ws + g.o_work);
+    h.jobs = reinterpret_cast<const ax::RkJob *>(ws + g.o_jobs);
+    h.tile_jobs = reinterpret_cast<const ax::RkTileJobs *>(ws + g.o_tj);
+    h.n_jobs = static_cast<int>(jt.tile_fastest.size());
     h.partial = reinterpret_cast<double *>(ws + g.o_part);
     const ax::PlanHost *hp = &h;
     ax_status st = ax::contract_batch_launch(&hp, 1, s);

@@ -533,7 +533,7 @@ def main():
     else:
         peak = INT8_MFMA_PEAK / (plan.slices * (plan.slices + 1) // 2)
     achieved = flops / (contract_ms * 1e-3) / 1e12
-    kernel = "rk_gemm_f64_kernel" if prec == "fp64" else "rk_gemm_split_kernel"
+    kernel = "rk_gemm_f64_kernel" if prec == "fp64" else f"rk_gemm_split_kernel<{plan.slices}>"
     k_split = modes[prec]["schedule"]["k_split"]
     roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved,
                 "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,

@@ -82,6 +82,55 @@ def test_batch_bitwise_equals_alone(problems, precision):
         pl.close()
 
 
+def test_contraction_splits_match_oracle(problems):
+    """The FP64 contraction (each tile's K chunk split across the four waves
+    of its work-group) at several K splits and block orders: oracle parity
+    (grids of 630 and 675 nodes end in a partial 8-node stage; 3-12 orbitals
+    leave partial 16x16 blocks at the segment edge), agreement between
+    schedules, and bitwise alone-vs-batch with different K splits in one
+    batch."""
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor, run_batch
+    plans, ins, alone = [], [], []
+    for i, (gr, grw, P, Q, ls, kmax) in enumerate(problems):
+        pl = SlaterTensor(gr, grw, ls, kmax, precision="fp64")
+        Pd, Qd = dv.f64(P), dv.f64(Q)
+        pl.set_schedule("tile-major", 1)
+        ref = dv.host(pl.run(Pd, Qd))
+        scale = np.max(np.abs(ref))
+        cap = max(s for s in range(1, 17) if _accepts_split(pl, s))
+        for split in sorted({2, 3, cap}):
+            if split > cap:
+                continue
+            for order in SlaterTensor.ORDERS:
+                pl.set_schedule(order, split)
+                got = dv.host(pl.run(Pd, Qd))
+                err = np.max(np.abs(got - ref)) / scale
+                print(f"plan {i} {SHAPES[i]} {order} split {split}: {err:.2e} vs split 1")
+                assert err <= 1e-13
+            if i < 4 and split == cap:
+                assert _oracle_scaled_err(pl, got, gr, grw, P, Q, kmax) < 1e-12
+        if i < 4:
+            assert _oracle_scaled_err(pl, ref, gr, grw, P, Q, kmax) < 1e-12
+        pl.set_schedule("xcd-contiguous" if i % 2 else "tile-major", min(1 + i, cap))
+        plans.append(pl)
+        ins.append((Pd, Qd))
+        alone.append(dv.host(pl.run(Pd, Qd)))
+    outs = run_batch(plans, [a for a, _ in ins], [b for _, b in ins])
+    for i, (o, ref) in enumerate(zip(outs, alone)):
+        assert np.array_equal(dv.host(o), ref), f"plan {i} differs in the batch"
+    for pl in plans:
+        pl.close()
+
+
+def _accepts_split(plan, split):
+    try:
+        plan.set_schedule("tile-major", split)
+    except ValueError:
+        return False
+    return True
+
+
 def test_batch_more_plans_than_one_launch():
     """35 plans (two launch chunks of at most 32)."""
     from atomx_amd import device as dv
