@@ -1,0 +1,35 @@
+#!/bin/bash
+# Hardware-counter runs of the benchmark tensor through the native driver
+# (no Python in the profiled process).  Per precision mode: a kernel trace,
+# then one pass each of FETCH_SIZE, WRITE_SIZE and the MFMA counters, each a
+# profiler run of its own; every run ends with a 256 MiB device copy that
+# calibrates the byte counters.
+#
+#   bash atomx_amd/native_bench/counters.sh OUTDIR
+#
+# Summarise with: python -m atomx_amd.native_bench.summarize OUTDIR
+set -o pipefail
+ROOT=$(cd "$(dirname "$0")/../.." && pwd)
+OUT=$(mkdir -p "$1" && cd "$1" && pwd)
+BIN=$ROOT/atomx_amd/_lib/rk_tensor_bench
+(cd "$ROOT" && python3 -m atomx_amd.native_bench.write_workload "$OUT/w.bin") || exit 1
+cd /tmp && export TMPDIR=/tmp
+pass() {  # name, precision, splits, profiler options...
+    local name=$1 prec=$2 splits=$3
+    shift 3
+    echo "== $name (start)"
+    timeout -s KILL 120 rocprofv3 "$@" --output-format csv -d "$OUT/$name" -o nb -- \
+        "$BIN" "$OUT/w.bin" "$prec" 1 "$splits" 10 256 > "$OUT/$name.log" 2>&1
+    local rc=$?
+    grep -E "^workload|^copy" "$OUT/$name.log"
+    if [ $rc -ne 0 ] && [ $rc -ne 139 ]; then echo "$name: rc=$rc"; return $rc; fi
+    return 0
+}
+MFMA="SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_VALU_MFMA_MOPS_I8 SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE"
+for prec in fp64 split; do
+    splits=6,7,8,9,10,12
+    pass "$prec-trace" $prec $splits --kernel-trace --stats &&
+    pass "$prec-fetch" $prec $splits --pmc FETCH_SIZE &&
+    pass "$prec-write" $prec $splits --pmc WRITE_SIZE &&
+    pass "$prec-mfma" $prec $splits --pmc $MFMA || exit 1
+done
