@@ -39,14 +39,48 @@ struct ax_rk_tensor {
     std::vector<const ax::RkJob *> jobs_tf, jobs_sf;
     std::vector<const ax::RkTileJobs *> tile_jobs;
     std::vector<int> n_jobs;
-    hipEvent_t done = nullptr;  // recorded after the plan's last launch
+    struct DoneMark *done = nullptr;  // recorded after the plan's last launch set
     hipEvent_t ready = nullptr; // recorded after the set-up upload
     bool ready_pending = true;  // launches must still wait on `ready`
     bool captured = false;      // launched into a stream capture at least once
     std::vector<void *> allocs;
 };
 
+// Completion mark of a launch set: one event recorded once per set, shared
+// (reference-counted) by every plan of the set.
+struct DoneMark {
+    hipEvent_t ev = nullptr;
+    int refs = 0;
+};
+
 namespace {
+
+std::mutex g_mark_mu;
+std::vector<DoneMark *> g_mark_free;
+
+// A mark with no references (a recycled one or a new one); nullptr on failure.
+DoneMark *mark_acquire() {
+    {
+        std::lock_guard<std::mutex> lk(g_mark_mu);
+        if (!g_mark_free.empty()) {
+            DoneMark *m = g_mark_free.back();
+            g_mark_free.pop_back();
+            return m;
+        }
+    }
+    DoneMark *m = new DoneMark();
+    if (hipEventCreateWithFlags(&m->ev, hipEventDisableTiming) != hipSuccess) {
+        delete m;
+        return nullptr;
+    }
+    return m;
+}
+
+void mark_release(DoneMark *m) {
+    if (!m) return;
+    std::lock_guard<std::mutex> lk(g_mark_mu);
+    if (--m->refs == 0) g_mark_free.push_back(m);
+}
 
 int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
@@ -73,9 +107,10 @@ void apply_schedule(ax_rk_tensor *p) {
 void free_plan(ax_rk_tensor *p) {
     for (void *q : p->allocs) ax::pool_free(q);
     p->allocs.clear();
-    if (p->done) (void)hipEventDestroy(p->done);
+    mark_release(p->done);
     if (p->ready) (void)hipEventDestroy(p->ready);
-    p->done = p->ready = nullptr;
+    p->done = nullptr;
+    p->ready = nullptr;
 }
 
 // Makes stream s wait for the set-up upload of every plan (a device-side
@@ -152,17 +187,36 @@ int choose_split(int n_work, int64_t n) {
     return static_cast<int>(std::max<int64_t>(s, 1));
 }
 
-// Records the completion event of every plan after a launch sequence
-// (skipped under stream capture, where destroy synchronises the device).
+// Records one completion mark after a launch sequence and hands it to every
+// plan of the sequence (skipped under stream capture, where destroy
+// synchronises the device).
 ax_status mark_done(ax_rk_tensor *const *plans, int n, hipStream_t s) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     AX_CHECK_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+        for (int i = 0; i < n; ++i) plans[i]->captured = true;
+        return AX_OK;
+    }
+    if (n <= 0) return AX_OK;
+    DoneMark *m = mark_acquire();
+    if (!m) {
+        ax::set_error("completion event creation failed");
+        return AX_ERR_HIP;
+    }
+    const hipError_t e = hipEventRecord(m->ev, s);
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lk(g_mark_mu);
+        g_mark_free.push_back(m);
+        ax::set_error("hipEventRecord: %s", hipGetErrorString(e));
+        return AX_ERR_HIP;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_mark_mu);
+        m->refs = n;
+    }
     for (int i = 0; i < n; ++i) {
-        if (cs != hipStreamCaptureStatusNone) {
-            plans[i]->captured = true;
-            continue;
-        }
-        AX_CHECK_HIP(hipEventRecord(plans[i]->done, s));
+        mark_release(plans[i]->done);
+        plans[i]->done = m;
     }
     return AX_OK;
 }
@@ -488,7 +542,6 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
                           ss);
     if (e == hipSuccess && st == AX_OK) e = hipEventCreateWithFlags(&plan->ready, hipEventDisableTiming);
     if (e == hipSuccess && st == AX_OK) e = hipEventRecord(plan->ready, ss);
-    if (e == hipSuccess && st == AX_OK) e = hipEventCreateWithFlags(&plan->done, hipEventDisableTiming);
     if (e != hipSuccess || st != AX_OK) {
         if (e != hipSuccess) ax::set_error("R^k tensor set-up failed: %s", hipGetErrorString(e));
         free_plan(plan);
@@ -537,7 +590,7 @@ ax_status ax_rk_tensor_destroy(ax_rk_tensor *plan) {
     if (plan->captured)
         (void)hipDeviceSynchronize();
     else if (plan->done)
-        (void)hipEventSynchronize(plan->done);
+        (void)hipEventSynchronize(plan->done->ev);
     free_plan(plan);
     delete plan;
     return AX_OK;
@@ -613,7 +666,7 @@ ax_status ax_rk_tensor_set_schedule(ax_rk_tensor *plan, int order, int n_split) 
                order);
     AX_REQUIRE(n_split >= 1 && n_split <= plan->split_cap,
                "ax_rk_tensor_set_schedule: split %d outside [1, %d]", n_split, plan->split_cap);
-    if (plan->done) AX_CHECK_HIP(hipEventSynchronize(plan->done));
+    if (plan->done) AX_CHECK_HIP(hipEventSynchronize(plan->done->ev));
     plan->host.order = order;
     plan->host.n_split = n_split;
     plan->n_split = n_split;

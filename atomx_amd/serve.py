@@ -341,7 +341,10 @@ class _State:
         self.fock = None
         self.P = self.Q = self.out = None
         self.host_out = None
-        self.grid_ids = None
+        # the grid, grid-weight and l arrays and the scalars of the request
+        # that last matched this state: a request passing the same objects
+        # is not checked again (the arrays are kept, so their ids stay theirs)
+        self.match = None
 
     def close(self):
         if self.plan is not None:
@@ -373,6 +376,7 @@ class GpuExecutor:
         self._estimates = {}                     # request shape -> (bytes, R^k count)
         self.plans_created = 0
         self._fock_cache = {}
+        self._groups = collections.OrderedDict()  # batch layouts by their states' keys
 
     # -- footprints ----------------------------------------------------------------
     def _info(self, req):
@@ -430,7 +434,11 @@ class GpuExecutor:
 
     def footprint(self, req):
         if req.kind == "slater_batch":
-            return sum(self.footprint(a) for a in req.atoms)
+            states, total = self.states, 0
+            for a in req.atoms:
+                st = states.get((a.kind, a.state)) if a.state is not None else None
+                total += st.nbytes if st is not None else self._info(a)[1]
+            return total
         key = self._key(req)
         if key is not None and key in self.states:
             return self.states[key].nbytes
@@ -438,7 +446,13 @@ class GpuExecutor:
 
     def ready(self, req):
         if req.kind == "slater_batch":
-            return all(self.ready(a) for a in req.atoms)
+            states = self.states
+            for a in req.atoms:
+                if a.state is not None:
+                    st = states.get((a.kind, a.state))
+                    if st is not None and st.busy:
+                        return False
+            return True
         key = self._key(req)
         return key is None or key not in self.states or not self.states[key].busy
 
@@ -452,8 +466,16 @@ class GpuExecutor:
 
     def reserve(self, req):
         if req.kind == "slater_batch":
+            states = self.states
             for a in req.atoms:
-                self.reserve(a)
+                st = states.get((a.kind, a.state)) if a.state is not None else None
+                if st is not None and not st.busy:  # the common case, inline
+                    st.busy = True
+                    if st.key is not None:
+                        self._idle_bytes -= st.nbytes
+                    states.move_to_end(st.key)
+                else:
+                    self.reserve(a)
             return
         key = self._key(req)
         if key is None:
@@ -464,7 +486,7 @@ class GpuExecutor:
             st = _State(key, shape, nbytes)
             st.busy = True
             if req.kind == "slater":
-                st.grid_ids = (id(req.gr), id(req.grw))
+                st.match = self._match(req)
             self.states[key] = st
         else:
             self._set_busy(st, True)
@@ -496,28 +518,32 @@ class GpuExecutor:
         return ("fock", id(req))
 
     # -- launches --------------------------------------------------------------------
+    @staticmethod
+    def _match(req):
+        return (req.gr, req.grw, req.ls, req.kmax, req.precision, req.Q is not None)
+
+    @staticmethod
+    def _same(match, req):
+        return (match is not None and match[0] is req.gr and match[1] is req.grw and
+                match[2] is req.ls and match[3] == req.kmax and match[4] == req.precision and
+                match[5] == (req.Q is not None))
+
     def _state_for(self, req):
         key = self._key(req)
         if key is not None and key in self.states:
             st = self.states[key]
-            if st.shape_key is not None and req.kind == "slater":
-                # the grid is hashed again only when another array object is passed
-                if st.grid_ids != (id(req.gr), id(req.grw)):
-                    if self._slater_shape(req) != st.shape_key:
-                        raise ValueError(f"state {req.state!r} was created with another shape")
-                    st.grid_ids = (id(req.gr), id(req.grw))
-                else:
-                    nv, ng = np.shape(req.gr)
-                    if (st.shape_key[2:5] != (nv, ng, tuple(int(x) for x in req.ls)) or
-                            st.shape_key[5:] != (int(req.kmax), req.precision, req.Q is not None)):
-                        raise ValueError(f"state {req.state!r} was created with another shape")
+            if st.shape_key is not None and req.kind == "slater" and not self._same(st.match, req):
+                # hashed and compared again only when other objects are passed
+                if self._slater_shape(req) != st.shape_key:
+                    raise ValueError(f"state {req.state!r} was created with another shape")
+                st.match = self._match(req)
             self._set_busy(st, True)
             return st
         shape, nbytes = self._info(req)[:2]
         st = _State(key, shape, nbytes)
         st.busy = True
         if req.kind == "slater":
-            st.grid_ids = (id(req.gr), id(req.grw))
+            st.match = self._match(req)
         if key is not None:
             self.states[key] = st
         return st
@@ -547,61 +573,173 @@ class GpuExecutor:
             self._stage[sid] = buf
         return buf
 
+    def _group(self, req):
+        """Cached layout of a multi-atom request whose atoms all have built
+        resident states (the same group again, next SCF iteration), or None:
+        (states, plans, plan handles, input sizes, input offsets, output
+        sizes, output offsets)."""
+        atoms = req.atoms
+        try:
+            gkey = tuple([(a.kind, a.state) for a in atoms])
+        except TypeError:
+            return None
+        if any(k[1] is None for k in gkey):
+            return None
+        g = self._groups.get(gkey)
+        if g is None:
+            sts = [self.states.get(k) for k in gkey]
+            if any(st is None or st.plan is None for st in sts):
+                return None
+            import ctypes
+            plans = [st.plan for st in sts]
+            sizes = []
+            for a, pl in zip(atoms, plans):
+                sizes.append(pl.n_orb * pl.n)
+                if a.Q is not None:
+                    sizes.append(pl.n_orb * pl.n)
+            offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+            outs = np.array([pl.size for pl in plans], dtype=np.int64)
+            ooff = offs[-1] + np.concatenate([[0], np.cumsum(outs)]).astype(np.int64)
+            hasq = [a.Q is not None for a in atoms]
+            handles = (ctypes.c_void_p * len(plans))(*[pl._h.value for pl in plans])
+            g = (sts, plans, handles, offs, outs, ooff, hasq)
+            self._groups[gkey] = g
+            if len(self._groups) > 256:
+                self._groups.popitem(last=False)
+        sts, plans, _, _, _, _, hasq = g
+        for k, st, pl in zip(gkey, sts, plans):
+            if self.states.get(k) is not st or st.plan is not pl:  # a state was rebuilt
+                del self._groups[gkey]
+                return None
+        # the same objects and shapes as when the layout was made
+        f64 = np.float64
+        for a, st, pl, q in zip(atoms, sts, plans, hasq):
+            if (not self._same(st.match, a) or type(a.P) is not np.ndarray
+                    or a.P.dtype != f64 or a.P.shape != (pl.n_orb, pl.n) or (a.Q is not None) != q
+                    or (q and (type(a.Q) is not np.ndarray or a.Q.dtype != f64
+                               or a.Q.shape != a.P.shape))):
+                return None
+        return g
+
+    def _launch_group(self, req, g, job, stream):
+        """A cached multi-atom layout: one concatenation into the staging,
+        one upload, one batch launch from prebuilt pointer tables, one
+        download."""
+        import ctypes
+
+        from .tensor import run_batch_arrays
+        sts, plans, handles, offs, outs, ooff, hasq = g
+        for st in sts:
+            self._set_busy(st, True)
+        chunks = []
+        for a, q in zip(req.atoms, hasq):
+            chunks.append(a.P.reshape(-1))
+            if q:
+                chunks.append(a.Q.reshape(-1))
+        n_in = int(offs[-1])
+        end = int(ooff[-1])
+        host, dev = self._staging(job["sid"], end)
+        hv = host.numpy()
+        np.concatenate(chunks, out=hv[:n_in])
+        base = np.uint64(dev.data_ptr())
+        addr = base + np.uint64(8) * offs[:-1].astype(np.uint64)
+        n = len(plans)
+        pp = np.empty(n, dtype=np.uint64)
+        qq = np.zeros(n, dtype=np.uint64)
+        k = 0
+        for i, q in enumerate(hasq):
+            pp[i] = addr[k]
+            k += 1
+            if q:
+                qq[i] = addr[k]
+                k += 1
+        oo = base + np.uint64(8) * ooff[:-1].astype(np.uint64)
+        dev[:n_in].copy_(host[:n_in], non_blocking=True)
+        run_batch_arrays(handles, n, pp, qq, oo, stream=stream)
+        host[n_in:end].copy_(dev[n_in:end], non_blocking=True)
+        job["group"] = (n_in, end, ooff, outs)
+        job["host"] = host
+        job["states"].append(list(sts))
+
     def _launch_slater(self, reqs, job, stream):
         """One launch set: the inputs of every request are packed into the
         slot's pinned staging and moved with one copy, the plans run as one
         batch on raw device offsets of the slot's device staging, and all
         results come back with one copy."""
         from .tensor import SlaterTensor, run_batch_raw
-        entries = []  # (request index, atom index or -1, state, P, Q)
+        if len(reqs) == 1 and reqs[0].kind == "slater_batch":
+            g = self._group(reqs[0])
+            if g is not None:
+                self._launch_group(reqs[0], g, job, stream)
+                return
+        entries = []  # (request index, atom index or -1, plan, P, Q)
+        f64 = np.float64
         for i, req in enumerate(reqs):
-            atoms = req.atoms if req.kind == "slater_batch" else [req]
+            batch = req.kind == "slater_batch"
+            atoms = req.atoms if batch else (req,)
             got, sts = [], []
             try:
                 for j, a in enumerate(atoms):
                     st = self._state_for(a)
                     sts.append(st)
-                    if st.plan is None:
-                        st.plan = SlaterTensor(a.gr, a.grw, a.ls, a.kmax, a.precision)
+                    plan = st.plan
+                    if plan is None:
+                        plan = st.plan = SlaterTensor(a.gr, a.grw, a.ls, a.kmax, a.precision)
                         self.plans_created += 1
-                    P = np.asarray(a.P, dtype=np.float64)
-                    Q = np.asarray(a.Q, dtype=np.float64) if a.Q is not None else None
-                    if P.shape != (st.plan.n_orb, st.plan.n) or (Q is not None and
-                                                                  Q.shape != P.shape):
-                        raise ValueError(
-                            f"orbital samples must be ({st.plan.n_orb}, {st.plan.n})")
-                    got.append((i, j if req.kind == "slater_batch" else -1, st, P, Q))
+                    P, Q = a.P, a.Q
+                    if type(P) is not np.ndarray or P.dtype != f64:
+                        P = np.asarray(P, dtype=f64)
+                    if Q is not None and (type(Q) is not np.ndarray or Q.dtype != f64):
+                        Q = np.asarray(Q, dtype=f64)
+                    shape = (plan.n_orb, plan.n)
+                    if P.shape != shape or (Q is not None and Q.shape != shape):
+                        raise ValueError(f"orbital samples must be {shape}")
+                    got.append((i, j if batch else -1, plan, P, Q))
                 entries.extend(got)
             except Exception as exc:  # the request fails alone
                 job["errors"][i] = exc
+                # every state the scheduler reserved for the request is
+                # released with it, also those of atoms after the failing one
+                seen = {id(st) for st in sts}
+                for a in atoms:
+                    k = self._key(a)
+                    st = self.states.get(k) if k is not None else None
+                    if st is not None and id(st) not in seen:
+                        sts.append(st)
+                        seen.add(id(st))
             job["states"].append(sts)
         if not entries:
             return
-        n_in = sum(P.size + (Q.size if Q is not None else 0) for _, _, _, P, Q in entries)
-        n_out = sum(st.plan.size for _, _, st, _, _ in entries)
+        chunks = []
+        for e in entries:
+            chunks.append(e[3].reshape(-1))
+            if e[4] is not None:
+                chunks.append(e[4].reshape(-1))
+        n_in = sum(c.size for c in chunks)
+        n_out = sum(e[2].size for e in entries)
         host, dev = self._staging(job["sid"], n_in + n_out)
         hv = host.numpy()
+        np.concatenate(chunks, out=hv[:n_in])
         base = dev.data_ptr()
-        pp, qq, oo = [], [], []
-        off = 0
-        for _, _, st, P, Q in entries:
-            hv[off:off + P.size] = P.ravel()
+        pp, qq, oo, plans = [], [], [], []
+        results = {}
+        off, out = 0, n_in
+        for i, j, plan, P, Q in entries:
             pp.append(base + 8 * off)
             off += P.size
             if Q is not None:
-                hv[off:off + Q.size] = Q.ravel()
                 qq.append(base + 8 * off)
                 off += Q.size
             else:
                 qq.append(None)
+            oo.append(base + 8 * out)
+            results[(i, j)] = (out, plan.size)
+            out += plan.size
+            plans.append(plan)
         dev[:n_in].copy_(host[:n_in], non_blocking=True)
-        job["results"] = {}
-        for i, j, st, _, _ in entries:
-            oo.append(base + 8 * off)
-            job["results"][(i, j)] = (off, st.plan.size)
-            off += st.plan.size
-        run_batch_raw([st.plan for _, _, st, _, _ in entries], pp, qq, oo, stream=stream)
-        host[n_in:off].copy_(dev[n_in:off], non_blocking=True)
+        run_batch_raw(plans, pp, qq, oo, stream=stream)
+        host[n_in:out].copy_(dev[n_in:out], non_blocking=True)
+        job["results"] = results
         job["host"] = host
 
     def _launch_fock(self, req, job, stream):
@@ -631,6 +769,9 @@ class GpuExecutor:
             job["states"].append([st])
         except Exception as exc:
             job["errors"][0] = exc
+            if st is None:  # the state the scheduler reserved is released too
+                k = self._key(req)
+                st = self.states.get(k) if k is not None else None
             job["states"].append([st] if st is not None else [])
 
     def done(self, job):
@@ -642,6 +783,14 @@ class GpuExecutor:
         for i, (req, sts, err) in enumerate(zip(job["reqs"], job["states"], job["errors"])):
             if err is not None:
                 results.append(err)
+            elif "group" in job:
+                n_in, end, ooff, outs = job["group"]
+                vals = job["host"].numpy()[n_in:end].copy()  # the staging is reused
+                got = []
+                for j, st in enumerate(sts):
+                    o = int(ooff[j]) - n_in
+                    got.append(SlaterResult(vals[o:o + int(outs[j])], st.plan.offsets, st.plan.m))
+                results.append(got)
             elif req.kind in ("slater", "slater_batch"):
                 hv = job["host"].numpy()  # the staging is reused: results are copied out
                 got = []
@@ -666,6 +815,7 @@ class GpuExecutor:
         for st in self.states.values():
             st.close()
         self.states.clear()
+        self._groups.clear()
         self._idle_bytes = 0
         self._fock_cache.clear()
 

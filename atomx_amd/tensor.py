@@ -187,3 +187,19 @@ def run_batch_raw(plans, P_ptrs, Q_ptrs, out_ptrs, stream=None):
     arr = ctypes.c_void_p * nb
     call("ax_rk_tensor_run_batch", arr(*[p._h.value for p in plans]), nb, arr(*P_ptrs),
          arr(*Q_ptrs), arr(*out_ptrs), dv.stream_handle(stream))
+
+
+def run_batch_arrays(handles, n, P_addr, Q_addr, out_addr, stream=None):
+    """``run_batch_raw`` from prebuilt tables: a ctypes array of n plan
+    handles and numpy uint64 arrays of n device addresses (0 for a missing
+    Q); the caller has checked every shape."""
+    if n == 0:
+        return
+    P_addr = np.ascontiguousarray(P_addr, dtype=np.uint64)
+    Q_addr = np.ascontiguousarray(Q_addr, dtype=np.uint64)
+    out_addr = np.ascontiguousarray(out_addr, dtype=np.uint64)
+    if not (P_addr.size == Q_addr.size == out_addr.size == n):
+        raise ValueError("one (P, Q, out) address per plan")
+    vpp = ctypes.POINTER(ctypes.c_void_p)
+    call("ax_rk_tensor_run_batch", handles, n, P_addr.ctypes.data_as(vpp),
+         Q_addr.ctypes.data_as(vpp), out_addr.ctypes.data_as(vpp), dv.stream_handle(stream))

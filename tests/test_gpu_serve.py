@@ -175,3 +175,19 @@ def test_batch_request_equals_its_atoms_alone():
                 assert np.array_equal(res.values, _alone_slater(a))
             assert np.array_equal(fs.result().values, _alone_slater(single))
         assert srv.stats()["plans_created"] == len(atoms) + 2
+        # later iterations: new request objects and orbitals on the same
+        # states (the cached group layout), then one atom with wrong shapes
+        from dataclasses import replace
+        for it in range(2):
+            fresh = [replace(a, P=a.P * (1.0 + 0.01 * (it + 1)),
+                             Q=None if a.Q is None else a.Q * (1.0 - 0.01 * it)) for a in atoms]
+            got = srv.submit(SlaterBatchRequest(fresh)).result()
+            for a, res in zip(fresh, got):
+                assert np.array_equal(res.values, _alone_slater(a))
+        bad = list(atoms)
+        bad[2] = replace(atoms[2], P=atoms[2].P[:, :-1])
+        with pytest.raises(ValueError):
+            srv.submit(SlaterBatchRequest(bad)).result()
+        got = srv.submit(SlaterBatchRequest(atoms)).result()
+        for a, res in zip(atoms, got):
+            assert np.array_equal(res.values, _alone_slater(a))
