@@ -573,12 +573,11 @@ class GpuExecutor:
             self._stage[sid] = buf
         return buf
 
-    def _group(self, req):
-        """Cached layout of a multi-atom request whose atoms all have built
-        resident states (the same group again, next SCF iteration), or None:
-        (states, plans, plan handles, input sizes, input offsets, output
-        sizes, output offsets)."""
-        atoms = req.atoms
+    def _group(self, atoms):
+        """Cached layout of a launch set whose atoms (those of one multi-atom
+        request, or single requests) all have built resident states (the
+        same group again, next SCF iteration), or None: (states, plans, plan
+        handles, input offsets, output sizes, output offsets, Q present)."""
         try:
             gkey = tuple([(a.kind, a.state) for a in atoms])
         except TypeError:
@@ -621,18 +620,15 @@ class GpuExecutor:
                 return None
         return g
 
-    def _launch_group(self, req, g, job, stream):
-        """A cached multi-atom layout: one concatenation into the staging,
-        one upload, one batch launch from prebuilt pointer tables, one
-        download."""
-        import ctypes
-
+    def _launch_group(self, atoms, g, job, stream):
+        """A cached layout: one concatenation into the staging, one upload,
+        one batch launch from prebuilt pointer tables, one download."""
         from .tensor import run_batch_arrays
         sts, plans, handles, offs, outs, ooff, hasq = g
         for st in sts:
             self._set_busy(st, True)
         chunks = []
-        for a, q in zip(req.atoms, hasq):
+        for a, q in zip(atoms, hasq):
             chunks.append(a.P.reshape(-1))
             if q:
                 chunks.append(a.Q.reshape(-1))
@@ -659,7 +655,10 @@ class GpuExecutor:
         host[n_in:end].copy_(dev[n_in:end], non_blocking=True)
         job["group"] = (n_in, end, ooff, outs)
         job["host"] = host
-        job["states"].append(list(sts))
+        if job["reqs"][0].kind == "slater_batch":
+            job["states"].append(list(sts))
+        else:
+            job["states"].extend([st] for st in sts)
 
     def _launch_slater(self, reqs, job, stream):
         """One launch set: the inputs of every request are packed into the
@@ -668,9 +667,15 @@ class GpuExecutor:
         results come back with one copy."""
         from .tensor import SlaterTensor, run_batch_raw
         if len(reqs) == 1 and reqs[0].kind == "slater_batch":
-            g = self._group(reqs[0])
+            atoms = reqs[0].atoms
+        elif all(r.kind == "slater" for r in reqs):
+            atoms = reqs
+        else:
+            atoms = None
+        if atoms:
+            g = self._group(atoms)
             if g is not None:
-                self._launch_group(reqs[0], g, job, stream)
+                self._launch_group(atoms, g, job, stream)
                 return
         entries = []  # (request index, atom index or -1, plan, P, Q)
         f64 = np.float64
@@ -785,12 +790,16 @@ class GpuExecutor:
                 results.append(err)
             elif "group" in job:
                 n_in, end, ooff, outs = job["group"]
-                vals = job["host"].numpy()[n_in:end].copy()  # the staging is reused
+                if "values" not in job:  # the staging is reused: one copy out
+                    job["values"] = job["host"].numpy()[n_in:end].copy()
+                vals = job["values"]
+                first = 0 if req.kind == "slater_batch" else i  # atom index in the set
                 got = []
                 for j, st in enumerate(sts):
-                    o = int(ooff[j]) - n_in
-                    got.append(SlaterResult(vals[o:o + int(outs[j])], st.plan.offsets, st.plan.m))
-                results.append(got)
+                    o = int(ooff[first + j]) - n_in
+                    got.append(SlaterResult(vals[o:o + int(outs[first + j])], st.plan.offsets,
+                                            st.plan.m))
+                results.append(got if req.kind == "slater_batch" else got[0])
             elif req.kind in ("slater", "slater_batch"):
                 hv = job["host"].numpy()  # the staging is reused: results are copied out
                 got = []

@@ -191,3 +191,10 @@ def test_batch_request_equals_its_atoms_alone():
         got = srv.submit(SlaterBatchRequest(atoms)).result()
         for a, res in zip(atoms, got):
             assert np.array_equal(res.values, _alone_slater(a))
+        # the same states as single requests, twice (launch sets of four
+        # recurring requests take the cached layout the second time)
+        for it in range(2):
+            fresh = [replace(a, P=a.P * (1.0 + 0.02 * it)) for a in atoms]
+            futs = [srv.submit(a) for a in fresh]
+            for a, f in zip(fresh, futs):
+                assert np.array_equal(f.result().values, _alone_slater(a))
