@@ -107,6 +107,7 @@ struct BatchOf {
     int n;
     int order;      // contraction launches: block order (kOrder*)
     int pad0, pad1;
+    int *counter;   // dynamic contraction order: job ticket, finished work-groups
     int64_t end[kMaxBatch];
     T a[kMaxBatch];
 };
@@ -117,7 +118,8 @@ enum {
     kOrderLinear = 0,    // block b -> job b of the tile-fastest list
     kOrderXcd = 1,       // same, after the XCD-contiguous renumbering
     kOrderXcdSplit = 2,  // XCD-contiguous over the split-fastest list
-    kOrderCount = 3
+    kOrderDynamic = 3,   // FP64: resident work-groups take jobs heaviest first from a ticket
+    kOrderCount = 4
 };
 
 // Work-group renumbering for L2 locality: the dispatcher deals consecutive
@@ -141,14 +143,15 @@ __device__ inline int batch_find(const BatchOf<T> &B, int64_t it, int64_t &local
     return p;
 }
 
-// The job of this contraction work-group under the launch's block order.
+// The job of this contraction work-group under the launch's block order
+// (job index jb of the launch for the dynamic order).
 struct JobInfo {
     int p;  // plan in the batch
     int seg, ti, tj, split, count, slot;
 };
-__device__ inline JobInfo gemm_job(const BatchOf<GemmArg> &A) {
+__device__ inline JobInfo gemm_job(const BatchOf<GemmArg> &A, int64_t jb = -1) {
     const int64_t nb = gridDim.x, b = blockIdx.x;
-    const int64_t lb = A.order == kOrderLinear ? b : xcd_block(b, nb);
+    const int64_t lb = jb >= 0 ? jb : A.order == kOrderLinear ? b : xcd_block(b, nb);
     int64_t local;
     JobInfo J;
     J.p = batch_find(A, lb, local);
@@ -196,13 +199,14 @@ int64_t rk_gemm_partial_doubles(int n_work, int n_split);
 void upper_tiles(int m, int seg, std::vector<RkWork> &out);
 
 // Work-groups of a contraction cut into S K chunks per tile, listed
-// tile-fastest and split-fastest; the slots (and the reduction order) are
-// tile-major and do not depend on the list.
+// tile-fastest, split-fastest and heaviest first (most 16x16 blocks inside
+// the triangle of segment rows seg_m); the slots (and the reduction order)
+// are tile-major and do not depend on the list.
 struct JobTables {
-    std::vector<RkJob> tile_fastest, split_fastest;
+    std::vector<RkJob> tile_fastest, split_fastest, heaviest;
     std::vector<RkTileJobs> tiles;
 };
-JobTables make_jobs(const std::vector<RkWork> &work, int S);
+JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m, int S);
 
 // Host copy of a plan: sizes and the device pointers the launchers need.
 struct PlanHost {
@@ -221,6 +225,7 @@ struct PlanHost {
     const RkJob *jobs;      // the schedule's job list (order and K-split level)
     const RkTileJobs *tile_jobs;
     int n_jobs;
+    int *counter;           // two ints, zero between launches (dynamic order)
     double *partial;
 };
 // Whether the scan of a split-precision plan on n_gauss nodes per cell writes

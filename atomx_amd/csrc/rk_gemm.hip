@@ -125,10 +125,9 @@ __device__ inline void kw_loop(gdouble *const (&pa)[4], gdouble *const (&pb)[4],
     }
 }
 
-__global__ __launch_bounds__(256, 2) void rk_gemm_f64_kernel(const ax::BatchOf<ax::GemmArg> A) {
-    __shared__ double park[2][16 * 4 * 64];
-
-    const ax::JobInfo J = ax::gemm_job(A);
+// One job (a K chunk of a tile) of the FP64 contraction by one work-group.
+__device__ inline void kw_job(const ax::BatchOf<ax::GemmArg> &A, const ax::JobInfo &J,
+                              double (&park)[2][16 * 4 * 64]) {
     const ax::GemmArg &g = A.a[J.p];
     const ax::RkSeg sg = g.segs[J.seg];
     const int64_t chunk =
@@ -213,6 +212,32 @@ __global__ __launch_bounds__(256, 2) void rk_gemm_f64_kernel(const ax::BatchOf<a
 #undef KW_FOR_ACTIVE
 }
 
+__global__ __launch_bounds__(256, 2) void rk_gemm_f64_kernel(const ax::BatchOf<ax::GemmArg> A) {
+    __shared__ double park[2][16 * 4 * 64];
+    if (A.order != ax::kOrderDynamic) {
+        kw_job(A, ax::gemm_job(A), park);
+        return;
+    }
+    // dynamic order: every resident work-group takes the next job from the
+    // ticket until none is left (the jobs are listed heaviest first); the
+    // last work-group out resets the ticket and the exit count
+    __shared__ int ticket;
+    const int64_t total = A.end[A.n - 1];
+    for (;;) {
+        if (threadIdx.x == 0) ticket = atomicAdd(&A.counter[0], 1);
+        __syncthreads();
+        const int jb = ticket;
+        __syncthreads();
+        if (jb >= total) break;
+        kw_job(A, ax::gemm_job(A, jb), park);
+        __syncthreads();  // the partial-sum area is reused by the next job
+    }
+    if (threadIdx.x == 0 && atomicAdd(&A.counter[1], 1) == static_cast<int>(gridDim.x) - 1) {
+        atomicExch(&A.counter[0], 0);
+        atomicExch(&A.counter[1], 0);
+    }
+}
+
 // Sums the split partials of each tile in split order and writes the packed
 // upper triangle of the segment.
 __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::BatchOf<ax::GemmArg> A) {
@@ -242,6 +267,10 @@ __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::BatchOf<ax::Ge
 namespace ax {
 
 int gemm_tile() { return BM; }
+
+// Resident work-groups of a dynamic-order FP64 contraction: two per CU on the
+// 256 CUs of an MI355X (64 KB of LDS and 232 VGPRs each).
+constexpr int64_t kDynamicGroups = 512;
 
 int64_t rk_gemm_partial_doubles(int n_work, int n_split) {
     return (int64_t)n_work * n_split * BM * BM;
@@ -304,8 +333,13 @@ ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t
                       (long long)total);
             return AX_ERR_VALUE;
         }
-        const dim3 grid(static_cast<unsigned>(total));
+        dim3 grid(static_cast<unsigned>(total));
         if (kd[0] == AX_PREC_FP64) {
+            if (A.order == kOrderDynamic) {
+                // as many work-groups as fit at once (two per CU), each taking jobs
+                A.counter = sub[0]->counter;
+                grid = dim3(static_cast<unsigned>(std::min<int64_t>(total, kDynamicGroups)));
+            }
             hipLaunchKernelGGL(rk_gemm_f64_kernel, grid, dim3(256), 0, stream, A);
             AX_CHECK_LAUNCH();
         } else {
@@ -338,7 +372,7 @@ void upper_tiles(int m, int seg, std::vector<RkWork> &out) {
         for (int tj = ti; tj < T; ++tj) out.push_back(RkWork{seg, ti, tj, 0});
 }
 
-JobTables make_jobs(const std::vector<RkWork> &work, int S) {
+JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m, int S) {
     JobTables t;
     const int nw = static_cast<int>(work.size());
     std::vector<int> cnt(nw);
@@ -363,6 +397,23 @@ JobTables make_jobs(const std::vector<RkWork> &work, int S) {
             if (sp < cnt[e]) t.tile_fastest.push_back(job(e, sp));
     for (int e = 0; e < nw; ++e)
         for (int sp = 0; sp < cnt[e]; ++sp) t.split_fastest.push_back(job(e, sp));
+    // heaviest first: tiles by their 16x16 blocks inside the triangle (the
+    // MFMA work of each of their jobs), ties in tile-fastest order
+    std::vector<int> blocks(nw);
+    for (int e = 0; e < nw; ++e) {
+        const RkWork &w = work[e];
+        const int m = seg_m[w.seg];
+        const int rb = std::min(4, (m - w.ti * BM + 15) / 16);
+        const int cb = std::min(4, (m - w.tj * BM + 15) / 16);
+        for (int a = 0; a < rb; ++a)
+            for (int b = 0; b < cb; ++b) blocks[e] += (w.ti != w.tj || a <= b) ? 1 : 0;
+    }
+    std::vector<int> ord(nw);
+    for (int e = 0; e < nw; ++e) ord[e] = e;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return blocks[x] > blocks[y]; });
+    t.heaviest.reserve(first);
+    for (int e : ord)
+        for (int sp = 0; sp < cnt[e]; ++sp) t.heaviest.push_back(job(e, sp));
     return t;
 }
 

@@ -36,7 +36,7 @@ struct ax_rk_tensor {
     ax::PlanHost host{};
     // job lists of every K-split level S = 1..split_cap (index S - 1), on
     // the device: tile-fastest, split-fastest, per-tile slot ranges
-    std::vector<const ax::RkJob *> jobs_tf, jobs_sf;
+    std::vector<const ax::RkJob *> jobs_tf, jobs_sf, jobs_hv;
     std::vector<const ax::RkTileJobs *> tile_jobs;
     std::vector<int> n_jobs;
     struct DoneMark *done = nullptr;  // recorded after the plan's last launch set
@@ -98,8 +98,10 @@ struct Layout {
 // Points the launch descriptors at the job list of the plan's schedule.
 void apply_schedule(ax_rk_tensor *p) {
     const int S = p->host.n_split;
-    p->host.jobs = p->host.order == ax::kOrderXcdSplit ? p->jobs_sf[S - 1]
-                                                                         : p->jobs_tf[S - 1];
+    const int o = p->host.order;
+    p->host.jobs = o == ax::kOrderXcdSplit   ? p->jobs_sf[S - 1]
+                   : o == ax::kOrderDynamic ? p->jobs_hv[S - 1]
+                                            : p->jobs_tf[S - 1];
     p->host.tile_jobs = p->tile_jobs[S - 1];
     p->host.n_jobs = p->n_jobs[S - 1];
 }
@@ -339,8 +341,10 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     const int n_split = choose_split(n_work, n);
     const int split_cap = std::min(16, 4 * n_split);  // partials sized for tuning up to here
     AX_REQUIRE(n_work == 0 || work.back().tj < 0x8000, "ax_rk_tensor_create: too many tiles");
+    std::vector<int> seg_m;
+    for (int k : seg_k) seg_m.push_back(m_k[k]);
     std::vector<ax::JobTables> jobs;
-    for (int S = 1; S <= split_cap; ++S) jobs.push_back(ax::make_jobs(work, S));
+    for (int S = 1; S <= split_cap; ++S) jobs.push_back(ax::make_jobs(work, seg_m, S));
 
     // one device block for everything the plan owns
     Layout L;
@@ -354,12 +358,14 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     const int64_t o_iout = L.add((int64_t)it_out.size() * 4);
     const int64_t o_segs = L.add((int64_t)seg_k.size() * sizeof(ax::RkSeg));
     const int64_t o_work = L.add((int64_t)n_work * sizeof(ax::RkWork));
-    std::vector<int64_t> o_jtf, o_jsf, o_tj;
+    std::vector<int64_t> o_jtf, o_jsf, o_jhv, o_tj;
     for (const ax::JobTables &t : jobs) {
         o_jtf.push_back(L.add((int64_t)t.tile_fastest.size() * sizeof(ax::RkJob)));
         o_jsf.push_back(L.add((int64_t)t.split_fastest.size() * sizeof(ax::RkJob)));
+        o_jhv.push_back(L.add((int64_t)t.heaviest.size() * sizeof(ax::RkJob)));
         o_tj.push_back(L.add((int64_t)t.tiles.size() * sizeof(ax::RkTileJobs)));
     }
+    const int64_t o_ctr = L.add(2 * sizeof(int));  // dynamic-order ticket (zero)
     const int64_t head = L.size;  // everything above is uploaded from the host
     // split precision on 8-node cells: the scan writes Y^k slices only
     const bool y_fp64 = !(precision == AX_PREC_SPLIT && ax::scan_slices_fused(ng, nv));
@@ -487,9 +493,11 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
             const ax::JobTables &t = jobs[i];
             put(o_jtf[i], t.tile_fastest.data(), t.tile_fastest.size() * sizeof(ax::RkJob));
             put(o_jsf[i], t.split_fastest.data(), t.split_fastest.size() * sizeof(ax::RkJob));
+            put(o_jhv[i], t.heaviest.data(), t.heaviest.size() * sizeof(ax::RkJob));
             put(o_tj[i], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
             plan->jobs_tf.push_back(static_cast<const ax::RkJob *>(dptr(o_jtf[i])));
             plan->jobs_sf.push_back(static_cast<const ax::RkJob *>(dptr(o_jsf[i])));
+            plan->jobs_hv.push_back(static_cast<const ax::RkJob *>(dptr(o_jhv[i])));
             plan->tile_jobs.push_back(static_cast<const ax::RkTileJobs *>(dptr(o_tj[i])));
             plan->n_jobs.push_back(static_cast<int>(t.tile_fastest.size()));
         }
@@ -527,6 +535,7 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     h.segs = static_cast<const ax::RkSeg *>(dptr(o_segs));
     h.work = static_cast<const ax::RkWork *>(dptr(o_work));
     h.partial = static_cast<double *>(dptr(o_part));
+    h.counter = static_cast<int *>(dptr(o_ctr));
     apply_schedule(plan);
 
     // one asynchronous upload and the r-power tables on the set-up stream;
@@ -830,7 +839,7 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
         sg.ye_ld = tiles;
     }
     AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_seg, &sg, sizeof(sg), hipMemcpyHostToDevice, s));
-    const ax::JobTables jt = ax::make_jobs(work, split);
+    const ax::JobTables jt = ax::make_jobs(work, std::vector<int>{m}, split);
     AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_jobs, jt.tile_fastest.data(),
                                 jt.tile_fastest.size() * sizeof(ax::RkJob), hipMemcpyHostToDevice,
                                 s));

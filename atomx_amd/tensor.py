@@ -87,7 +87,10 @@ class SlaterTensor:
         return out
 
     STAGES = ("density", "scan", "slice", "contract", "reduce")
-    ORDERS = ("tile-major", "xcd-contiguous", "xcd-contiguous-split-fastest")
+    # block orders of the contraction launch; the last one (FP64) keeps one
+    # round of work-groups that take jobs heaviest first from a ticket
+    ORDERS = ("tile-major", "xcd-contiguous", "xcd-contiguous-split-fastest",
+              "dynamic-heaviest-first")
 
     def schedule(self):
         """(block order name, K split) of the contraction."""
