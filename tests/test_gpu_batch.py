@@ -123,6 +123,44 @@ def test_contraction_splits_match_oracle(problems):
         pl.close()
 
 
+def test_dynamic_order_repeats_and_replays(problems):
+    """The dynamic contraction order resets its own ticket: ten launches in a
+    row and three replays of a captured HIP graph (eager launches between
+    them) all give the same bits."""
+    import torch
+
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    gr, grw, P, Q, ls, kmax = problems[0]
+    plan = SlaterTensor(gr, grw, ls, kmax, precision="fp64")
+    cap = max(s for s in range(1, 17) if _accepts_split(plan, s))
+    plan.set_schedule("dynamic-heaviest-first", cap)
+    Pd, Qd = dv.f64(P), dv.f64(Q)
+    ref = dv.host(plan.run(Pd, Qd))
+    for _ in range(10):
+        assert np.array_equal(dv.host(plan.run(Pd, Qd)), ref)
+    out = dv.empty((plan.size,))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.run(Pd, Qd, out)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.run(Pd, Qd, out)
+    for rep in range(3):
+        out.fill_(0.0)
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(dv.host(out), ref), f"replay {rep}"
+        assert np.array_equal(dv.host(plan.run(Pd, Qd)), ref)
+        torch.cuda.synchronize()
+    del g
+    torch.cuda.synchronize()
+    plan.close()
+
+
 def _accepts_split(plan, split):
     try:
         plan.set_schedule("tile-major", split)
