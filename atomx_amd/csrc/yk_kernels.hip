@@ -888,7 +888,11 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
     constexpr int MAXP = MAXQ * kWgScanWaves;
     constexpr int CS = NG + 1;  // padded cell stride of the transpose region
     __shared__ double tot_o[MAXP], tot_i[MAXP];
-    __shared__ double tr[kWgScanWaves][2][64 * CS];
+    // transpose regions: only 7- and 9-node cells (loads) and in-scan
+    // slicing (output) go through them; 8-node cells are read and written
+    // cell-per-lane
+    constexpr int TW = (SLICE || NG != 8) ? kWgScanWaves : 1;
+    __shared__ double tr[TW][2][64 * CS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int64_t local;
     const int p = ax::batch_find(A, blockIdx.x, local);
@@ -900,14 +904,43 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
     const double *src = g.D + (int64_t)g.item_row[local] * n;
     const double *rkp = g.rk + (int64_t)k * n;
     const double *rkmp = g.rkm + (int64_t)k * n;
-    double *to = tr[wave][0];
-    double *ti = tr[wave][1];
+    double *to = tr[TW > 1 ? wave : 0][0];
+    double *ti = tr[TW > 1 ? wave : 0][1];
 
     // node-per-lane products of every pass of this wave; the loads use a
     // clamped index and are all issued before the first product (a load
     // under a per-lane branch would be waited for inside the branch)
     double no[MAXQ][NG], ni[MAXQ][NG];
-    {
+    double go[MAXQ][NG], gi[MAXQ][NG];
+    if constexpr (NG == 8) {
+        // 8-node cells: the lane's cell is read directly (64 contiguous
+        // bytes per row), no transpose
+        typedef double dd2 __attribute__((ext_vector_type(2)));
+        dd2 d2[MAXQ][NG / 2], a2[MAXQ][NG / 2], b2[MAXQ][NG / 2];
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int64_t c0 = (int64_t)(q * kWgScanWaves + wave) * 64 * NG + (int64_t)lane * NG;
+            const int64_t cc = c0 + NG <= n ? c0 : 0;
+#pragma unroll
+            for (int h = 0; h < NG / 2; ++h) {
+                d2[q][h] = *reinterpret_cast<const dd2 *>(src + cc + 2 * h);
+                a2[q][h] = *reinterpret_cast<const dd2 *>(rkp + cc + 2 * h);
+                b2[q][h] = *reinterpret_cast<const dd2 *>(rkmp + cc + 2 * h);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int64_t c0 = (int64_t)(q * kWgScanWaves + wave) * 64 * NG + (int64_t)lane * NG;
+            const bool in = c0 + NG <= n;
+#pragma unroll
+            for (int h = 0; h < NG / 2; ++h) {
+                go[q][2 * h] = in ? d2[q][h].x * a2[q][h].x : 0.0;
+                go[q][2 * h + 1] = in ? d2[q][h].y * a2[q][h].y : 0.0;
+                gi[q][2 * h] = in ? d2[q][h].x * b2[q][h].x : 0.0;
+                gi[q][2 * h + 1] = in ? d2[q][h].y * b2[q][h].y : 0.0;
+            }
+        }
+    } else {
         double dv[MAXQ][NG], av[MAXQ][NG], bv[MAXQ][NG];
 #pragma unroll
         for (int q = 0; q < MAXQ; ++q) {
@@ -933,22 +966,26 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
         }
     }
     // cell-per-lane values, cell sums and in-wave scans
-    double go[MAXQ][NG], gi[MAXQ][NG];
     double ex_o[MAXQ], ex_i[MAXQ];
 #pragma unroll
     for (int q = 0; q < MAXQ; ++q) {
         const int P = q * kWgScanWaves + wave;
+        if constexpr (NG != 8) {
 #pragma unroll
-        for (int r = 0; r < NG; ++r) {
-            const int xl = r * 64 + lane;  // node of the pass, cell xl / NG
-            to[(xl / NG) * CS + xl % NG] = no[q][r];
-            ti[(xl / NG) * CS + xl % NG] = ni[q][r];
+            for (int r = 0; r < NG; ++r) {
+                const int xl = r * 64 + lane;  // node of the pass, cell xl / NG
+                to[(xl / NG) * CS + xl % NG] = no[q][r];
+                ti[(xl / NG) * CS + xl % NG] = ni[q][r];
+            }
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                go[q][j] = to[lane * CS + j];
+                gi[q][j] = ti[lane * CS + j];
+            }
         }
         double so = 0.0, si = 0.0;
 #pragma unroll
         for (int j = 0; j < NG; ++j) {
-            go[q][j] = to[lane * CS + j];
-            gi[q][j] = ti[lane * CS + j];
             so += go[q][j];
             si += gi[q][j];
         }
@@ -978,6 +1015,39 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
         for (int v = 0; v < P; ++v) co += tot_o[v];
         for (int v = passes - 1; v > P; --v) ci += tot_i[v];
         const double bo = co + ex_o[q], bi = ci + ex_i[q];
+        const int64_t base = (int64_t)P * 64 * NG;
+        if constexpr (!SLICE && NG == 8) {
+            // the lane's cell: its 8 consecutive nodes are read and written
+            // by this lane (64 contiguous bytes per row)
+            typedef double dd2 __attribute__((ext_vector_type(2)));
+            const int64_t c0 = base + (int64_t)lane * NG;
+            if (c0 + NG <= n) {
+                dd2 a2[NG / 2], b2[NG / 2];
+#pragma unroll
+                for (int h = 0; h < NG / 2; ++h) {
+                    a2[h] = *reinterpret_cast<const dd2 *>(rkp + c0 + 2 * h);
+                    b2[h] = *reinterpret_cast<const dd2 *>(rkmp + c0 + 2 * h);
+                }
+                dd2 y2[NG / 2];
+#pragma unroll
+                for (int m = 0; m < NG; ++m) {
+                    double po = 0.0, pi = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NG; ++j) {
+                        po += mo[m * NG + j] * go[q][j];
+                        pi += mi[m * NG + j] * gi[q][j];
+                    }
+                    const double ra = (m & 1) ? a2[m >> 1].y : a2[m >> 1].x;
+                    const double rb = (m & 1) ? b2[m >> 1].y : b2[m >> 1].x;
+                    const double y = rb * (bo + po) + ra * (bi + pi);
+                    if (m & 1) y2[m >> 1].y = y; else y2[m >> 1].x = y;
+                }
+                double *yo = g.Y + (int64_t)g.item_out[local] * n + c0;
+#pragma unroll
+                for (int h = 0; h < NG / 2; ++h) *reinterpret_cast<dd2 *>(yo + 2 * h) = y2[h];
+            }
+            continue;
+        }
         // cumulative integrals of the lane's cell, back to node-per-lane
 #pragma unroll
         for (int m = 0; m < NG; ++m) {
@@ -990,7 +1060,6 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
             to[lane * CS + m] = bo + po;
             ti[lane * CS + m] = bi + pi;
         }
-        const int64_t base = (int64_t)P * 64 * NG;
         double ra[NG], rb[NG];
 #pragma unroll
         for (int r = 0; r < NG; ++r) {
