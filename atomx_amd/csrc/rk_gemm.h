@@ -97,7 +97,9 @@ struct GemmArg {             // contraction work-groups and split reduction
     const RkJob *jobs;       // work-groups (contraction), in the launch's job order
     const RkTileJobs *tile_jobs;
     double *partial;         // one 64*64 tile per job slot
-    double *out;             // packed R^k output
+    double *out;             // packed R^k output: a tile contracted in one K
+                             // chunk is written here by the contraction, the
+                             // others by rk_reduce_kernel
 };
 
 // Launch table: end[] is the exclusive prefix of the launch's items per plan
@@ -194,6 +196,11 @@ __device__ inline void slice_tile(const double *__restrict__ src_row, int64_t n,
     slice_value(x < n ? src_row[x] : 0.0, x, dst_row, dst_plane, e_slot, slices);
 }
 
+// Packed upper-triangle index of (i, j), i <= j < m.
+__device__ inline int64_t packed_index(int64_t i, int64_t j, int64_t m) {
+    return i * m - i * (i - 1) / 2 + (j - i);
+}
+
 int gemm_tile();
 int64_t rk_gemm_partial_doubles(int n_work, int n_split);
 void upper_tiles(int m, int seg, std::vector<RkWork> &out);
@@ -236,7 +243,10 @@ ax_status density_batch_launch(const PlanHost *const *plans, int n, const double
                                const double *const *Q, hipStream_t stream);
 ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream);
 ax_status slice_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream);
-ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream);
+// out: the plans' packed outputs (tiles contracted in one K chunk are
+// written by the contraction itself).
+ax_status contract_batch_launch(const PlanHost *const *plans, int n, double *const *out,
+                                hipStream_t stream);
 ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const *out,
                               hipStream_t stream);
 ax_status rk_gemm_split_launch(const BatchOf<GemmArg> &A, int slices, dim3 grid,

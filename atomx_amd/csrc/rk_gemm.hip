@@ -5,9 +5,14 @@
 // each an m x m symmetric-structured output; every upper 64x64 tile of every
 // segment is one work entry, and its K range is cut into `n_split`
 // contiguous chunks (one job = one work-group = one chunk of one tile).
-// Partial tiles are written to a workspace and summed over the chunks in a
-// fixed order by rk_reduce_kernel, so a result depends only on its own
+// A tile of one chunk is written to the packed output by its work-group;
+// the chunks of the others are written to a partial workspace and summed in
+// a fixed order by rk_reduce_kernel, so a result depends only on its own
 // segment's data and the schedule (never on what else is in the batch).
+// (Summing a tile's chunks in the contraction, by the work-group that
+// finishes last, needs an agent-scope release per work-group — an L2
+// write-back — and measured 30-70 us slower per launch than the separate
+// reduction.)
 //
 // FP64 path: v_mfma_f64_16x16x4_f64, K split across the four waves of a
 // work-group (below).
@@ -204,10 +209,18 @@ __device__ inline void kw_job(const ax::BatchOf<ax::GemmArg> &A, const ax::JobIn
     __syncthreads();
     if (wave == 0) {
         const double *src = park[0];
-        double *out = g.partial + (int64_t)J.slot * (BM * BM);
         const int fk = lane >> 4;
-        KW_FOR_ACTIVE(out[(16 * a + fk + 4 * r) * BM + 16 * b + fr] =
-                          acc[a][b][r] + src[((4 * a + b) * 4 + r) * 64 + slot])
+        if (J.count == 1) {  // a single chunk: the packed output directly
+            double *out = g.out + sg.out_off;
+            KW_FOR_ACTIVE(const int64_t i = row0 + 16 * a + fk + 4 * r;
+                          const int64_t j = col0 + 16 * b + fr;
+                          if (i < m && j < m && j >= i) out[ax::packed_index(i, j, m)] =
+                              acc[a][b][r] + src[((4 * a + b) * 4 + r) * 64 + slot])
+        } else {
+            double *out = g.partial + (int64_t)J.slot * (BM * BM);
+            KW_FOR_ACTIVE(out[(16 * a + fk + 4 * r) * BM + 16 * b + fr] =
+                              acc[a][b][r] + src[((4 * a + b) * 4 + r) * 64 + slot])
+        }
     }
 #undef KW_FOR_ACTIVE
 }
@@ -239,7 +252,8 @@ __global__ __launch_bounds__(256, 2) void rk_gemm_f64_kernel(const ax::BatchOf<a
 }
 
 // Sums the split partials of each tile in split order and writes the packed
-// upper triangle of the segment.
+// upper triangle of the segment (tiles of one chunk were written by the
+// contraction).
 __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::BatchOf<ax::GemmArg> A) {
     int64_t e;
     const int p = ax::batch_find(A, blockIdx.x, e);
@@ -247,6 +261,7 @@ __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::BatchOf<ax::Ge
     const ax::RkWork wk = g.work[e];
     const ax::RkSeg sg = g.segs[wk.seg];
     const ax::RkTileJobs tj = g.tile_jobs[e];
+    if (tj.count == 1) return;
     const int64_t m = sg.m;
     const double *src = g.partial + (int64_t)tj.first * (BM * BM);
     double *out = g.out + sg.out_off;
@@ -297,7 +312,8 @@ int64_t fill_work(BatchOf<GemmArg> &A, const PlanHost *const *plans, int n,
 
 }  // namespace
 
-ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t stream) {
+ax_status contract_batch_launch(const PlanHost *const *plans, int n, double *const *out,
+                                hipStream_t stream) {
     if (n <= 0) return AX_OK;
     if (n > kMaxBatch) {
         set_error("contract_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
@@ -305,6 +321,10 @@ ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t
     }
     for (int i = 0; i < n; ++i) {
         const PlanHost *h = plans[i];
+        if (!(out && out[i])) {
+            set_error("contract_batch_launch: null output for plan %d", i);
+            return AX_ERR_VALUE;
+        }
         const bool ok = h->precision == AX_PREC_FP64 ||
                         (h->precision == AX_PREC_SPLIT &&
                          (h->slices == kSplitSlices || h->slices == kSplitSlicesFine));
@@ -320,13 +340,17 @@ ax_status contract_batch_launch(const PlanHost *const *plans, int n, hipStream_t
                              {AX_PREC_SPLIT, kSplitSlicesFine}};
     for (const auto &kd : kinds) {
         const PlanHost *sub[kMaxBatch];
+        double *sub_out[kMaxBatch];
         int ns = 0;
         for (int i = 0; i < n; ++i)
-            if (plans[i]->precision == kd[0] && (kd[0] == AX_PREC_FP64 || plans[i]->slices == kd[1]))
+            if (plans[i]->precision == kd[0] &&
+                (kd[0] == AX_PREC_FP64 || plans[i]->slices == kd[1])) {
+                sub_out[ns] = out[i];
                 sub[ns++] = plans[i];
+            }
         if (ns == 0) continue;
         BatchOf<GemmArg> A{};
-        const int64_t total = fill_work(A, sub, ns, nullptr, true);
+        const int64_t total = fill_work(A, sub, ns, sub_out, true);
         if (total == 0) continue;
         if (total > 0x7fffffff) {
             set_error("contract_batch_launch: %lld work-groups exceed one launch",
@@ -357,8 +381,18 @@ ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const
         set_error("reduce_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
         return AX_ERR_VALUE;
     }
+    // plans contracted in one K chunk per tile have nothing to reduce
+    const PlanHost *sub[kMaxBatch];
+    double *sub_out[kMaxBatch];
+    int ns = 0;
+    for (int i = 0; i < n; ++i)
+        if (plans[i]->n_split > 1) {
+            sub_out[ns] = out[i];
+            sub[ns++] = plans[i];
+        }
+    if (ns == 0) return AX_OK;
     BatchOf<GemmArg> A{};
-    const int64_t total = fill_work(A, plans, n, out, false);
+    const int64_t total = fill_work(A, sub, ns, sub_out, false);
     if (total == 0) return AX_OK;
     hipLaunchKernelGGL(rk_reduce_kernel, dim3(static_cast<unsigned>(total), 16), dim3(256), 0,
                        stream, A);

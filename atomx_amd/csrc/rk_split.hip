@@ -205,6 +205,21 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax
         }
     }
 
+    if (J.count == 1) {  // a single chunk: the packed output directly
+        double *out = g.out + sg.out_off;
+        const int64_t m = sg.m;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t i = row0 + wr + a * 16 + 4 * fg + r;
+                    const int64_t j = col0 + wc + b * 16 + fr;
+                    if (i < m && j < m && j >= i) out[ax::packed_index(i, j, m)] = acc[a][b][r];
+                }
+        return;
+    }
     double *dst = g.partial + (int64_t)J.slot * (BM * BM);
 #pragma unroll
     for (int a = 0; a < 2; ++a)

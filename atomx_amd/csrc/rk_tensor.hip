@@ -7,8 +7,10 @@
 //   2. yk scan        Y^k[slot(p,k)] for every pair valid at rank k
 //   3. (split) slice  int8 planes + exponents of D and of every Y^k row
 //   4. contraction    packed upper triangle of D[V_k] Y_k^T for every k,
-//                     all ranks in one launch (one segment per rank)
-//   5. reduction      split partials summed in a fixed order
+//                     all ranks in one launch (one segment per rank); tiles
+//                     of one K chunk are written to the output directly
+//   5. reduction      K-chunk partials summed in a fixed order (no launch
+//                     when every tile is one chunk)
 // A batch of plans (different grids, orbital counts, k ranges) runs the same
 // five launches once for all of them; every kernel looks its plan up from a
 // prefix table in its arguments.  A plan's tiles, K splits and summation
@@ -235,7 +237,7 @@ ax_status run_stage_batch(ax_rk_tensor *const *plans, int n, int stage, const do
         case AX_STAGE_SLICE:
             return ax::slice_batch_launch(h, n, s);
         case AX_STAGE_CONTRACT:
-            return ax::contract_batch_launch(h, n, s);
+            return ax::contract_batch_launch(h, n, out, s);
         case AX_STAGE_REDUCE:
             for (int i = 0; i < n; ++i)
                 AX_REQUIRE(out[i] != nullptr, "R^k tensor run: null output for plan %d", i);
@@ -649,8 +651,14 @@ ax_status ax_rk_tensor_run_batch(ax_rk_tensor *const *plans, int n_plans, const 
     hipStream_t s = ax::as_stream(stream);
     ax_status wst = await_ready(plans, n_plans, s);
     if (wst != AX_OK) return wst;
-    for (int b0 = 0; b0 < n_plans; b0 += ax::kMaxBatch) {
-        const int nb = std::min(ax::kMaxBatch, n_plans - b0);
+    // launch sets of up to 16 plans; a plan listed again starts a new set, so
+    // its buffers (densities, Y^k, partials, tile counts) serve one listing
+    // at a time, in list order
+    for (int b0 = 0, nb = 0; b0 < n_plans; b0 += nb) {
+        nb = 0;
+        while (b0 + nb < n_plans && nb < ax::kMaxBatch &&
+               std::find(plans + b0, plans + b0 + nb, plans[b0 + nb]) == plans + b0 + nb)
+            ++nb;
         const double *q[ax::kMaxBatch];
         for (int i = 0; i < nb; ++i) q[i] = Q ? Q[b0 + i] : nullptr;
         for (int stage = AX_STAGE_DENSITY; stage <= AX_STAGE_REDUCE; ++stage) {
@@ -862,7 +870,7 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
     h.n_jobs = static_cast<int>(jt.tile_fastest.size());
     h.partial = reinterpret_cast<double *>(ws + g.o_part);
     const ax::PlanHost *hp = &h;
-    ax_status st = ax::contract_batch_launch(&hp, 1, s);
+    ax_status st = ax::contract_batch_launch(&hp, 1, &out, s);
     if (st != AX_OK) return st;
     st = ax::reduce_batch_launch(&hp, 1, &out, s);
     if (st != AX_OK) return st;

@@ -199,7 +199,9 @@ ax_status ax_rk_tensor_run(ax_rk_tensor *plan, const double *P, const double *Q,
 /* Several plans (any grids, orbital counts, k ranges and precision modes)
  * as one set of launches: density, Y^k scan (one per n_gauss value),
  * slicing, contraction (one per contraction kind) and reduction each run
- * once for the whole batch, in chunks of 16 plans.  P, Q and out are host arrays of n_plans device
+ * once for the whole batch, in chunks of 16 plans; a plan listed more than
+ * once starts a new chunk at each repeat, so its listings run one after
+ * the other.  P, Q and out are host arrays of n_plans device
  * pointers (Q may be NULL, or hold NULL entries, for no small component).
  * Each plan's results are bitwise those of its own ax_rk_tensor_run: tiles,
  * K splits and summation order are fixed per plan. */
@@ -208,7 +210,9 @@ ax_status ax_rk_tensor_run_batch(ax_rk_tensor *const *plans, int n_plans,
                                  double *const *out, void *stream);
 /* One stage of run(), for per-kernel timing: the stages in order are
  * density (needs P, Q), Y^k scan, slicing (split precision only), the
- * contraction (split partials) and the fixed-order reduction (needs out).
+ * contraction (needs out: K-chunk partials, and the packed values of tiles
+ * contracted in one chunk) and the fixed-order reduction of the partials
+ * (needs out; nothing to launch at K split 1).
  * Running all five in order equals run(). */
 enum {
     AX_STAGE_DENSITY = 0,

@@ -75,6 +75,16 @@ def test_batch_bitwise_equals_alone(problems, precision):
                       [ins[i][1] for i in order])
     for j, i in enumerate(order):
         assert np.array_equal(dv.host(outs2[j]), alone[i])
+    # one plan listed twice with different orbitals: each listing gets its own
+    P0h, Q0h = problems[0][2], problems[0][3]
+    P0b, Q0b = dv.f64(0.5 * P0h), dv.f64(0.75 * Q0h)
+    alone_b = dv.host(plans[0].run(P0b, Q0b))
+    outs3 = run_batch([plans[0], plans[1], plans[0]], [ins[0][0], ins[1][0], P0b],
+                      [ins[0][1], ins[1][1], Q0b])
+    assert np.array_equal(dv.host(outs3[0]), alone[0])
+    assert np.array_equal(dv.host(outs3[1]), alone[1])
+    assert np.array_equal(dv.host(outs3[2]), alone_b)
+    assert not np.array_equal(alone_b, alone[0])
     if precision == "fp64":
         gr, grw, P, Q, ls, kmax = problems[1]
         assert _oracle_scaled_err(plans[1], alone[1], gr, grw, P, Q, kmax) < 1e-13
@@ -120,6 +130,33 @@ def test_contraction_splits_match_oracle(problems):
     for i, (o, ref) in enumerate(zip(outs, alone)):
         assert np.array_equal(dv.host(o), ref), f"plan {i} differs in the batch"
     for pl in plans:
+        pl.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_single_chunk_tiles_write_every_value(problems, precision):
+    """At K split 1 the contraction writes the packed values itself (no
+    reduction launch): every value of an output pre-filled with NaN is
+    written, under every block order, and agrees with the default
+    schedule's result (split partials summed by the reduction)."""
+    import torch
+
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    for i in (0, 1, 3):
+        gr, grw, P, Q, ls, kmax = problems[i]
+        pl = SlaterTensor(gr, grw, ls, kmax, precision=precision)
+        Pd, Qd = dv.f64(P), dv.f64(Q)
+        ref = dv.host(pl.run(Pd, Qd))
+        scale = np.max(np.abs(ref))
+        for order in SlaterTensor.ORDERS:
+            pl.set_schedule(order, 1)
+            out = torch.full((pl.size,), float("nan"), dtype=torch.float64, device=dv.device())
+            got = dv.host(pl.run(Pd, Qd, out))
+            assert not np.isnan(got).any(), f"plan {i} {order}: values left unwritten"
+            err = np.max(np.abs(got - ref)) / scale
+            print(f"{precision} plan {i} {SHAPES[i]} {order} split 1: {err:.2e}")
+            assert err <= 1e-13
         pl.close()
 
 
