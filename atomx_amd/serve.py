@@ -29,9 +29,10 @@ from __future__ import annotations
 
 import collections
 import itertools
+import os
 import threading
 import time
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Any, Hashable, Optional, Sequence
 
@@ -368,9 +369,13 @@ class GpuExecutor:
         self._prec = PRECISIONS
         self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
         self._free_streams = list(range(n_streams))
-        # pinned staging per in-flight slot: uploads and one-off results of
-        # the slot's launch set (reused once the slot's job has finished)
+        # pinned staging per in-flight slot: the uploads of the slot's launch
+        # set (reused once the slot's job has finished)
         self._stage = [None] * n_streams
+        # host copies into the staging, split across threads (numpy copies
+        # run without the interpreter lock)
+        self._pack_threads = max(1, min(8, (os.cpu_count() or 1) // 2))
+        self._pool = ThreadPoolExecutor(self._pack_threads) if self._pack_threads > 1 else None
         self.states = collections.OrderedDict()  # resident, LRU order
         self._idle_bytes = 0                     # bytes of idle resident states
         self._estimates = {}                     # request shape -> (bytes, R^k count)
@@ -562,16 +567,57 @@ class GpuExecutor:
             job["event"].record(stream)
         return job
 
-    def _staging(self, sid, n):
-        """Pinned host and device float64 staging of at least n values for
-        slot sid (grown as needed, reused once the slot's job is finished)."""
+    def _staging(self, sid, n_in, n):
+        """Pinned host staging of at least n_in input values and device
+        staging of at least n values (inputs, then outputs) for slot sid,
+        grown as needed and reused once the slot's job is finished."""
         torch, dv = self._torch, self._dv
         buf = self._stage[sid]
-        if buf is None or buf[0].numel() < n:
-            n = max(n, 1 << 16)
-            buf = (torch.empty(n, dtype=torch.float64, pin_memory=True), dv.empty((n,)))
+        if buf is None or buf[0].numel() < n_in or buf[1].numel() < n:
+            n_in = max(n_in, 1 << 16, buf[0].numel() if buf is not None else 0)
+            n = max(n, 1 << 16, buf[1].numel() if buf is not None else 0)
+            buf = (torch.empty(n_in, dtype=torch.float64, pin_memory=True), dv.empty((n,)))
             self._stage[sid] = buf
         return buf
+
+    _PACK_SERIAL = 1 << 19  # values below which one thread packs
+
+    def _pack(self, chunks, dst):
+        """Concatenates the 1-D arrays ``chunks`` into ``dst``; large inputs
+        are cut into equal ranges of the concatenation, one per thread."""
+        n = dst.size
+        if self._pool is None or n < self._PACK_SERIAL:
+            np.concatenate(chunks, out=dst)
+            return
+        starts = np.concatenate([[0], np.cumsum([c.size for c in chunks])])
+        if int(starts[-1]) != n:
+            raise ValueError("packed inputs do not fill the staging range")
+        parts = self._pack_threads
+        bounds = [n * w // parts for w in range(parts + 1)]
+
+        def copy_range(lo, hi):
+            c = int(np.searchsorted(starts, lo, side="right")) - 1
+            while lo < hi:
+                c0 = int(starts[c])
+                take = min(hi, int(starts[c + 1])) - lo
+                dst[lo:lo + take] = chunks[c][lo - c0:lo - c0 + take]
+                lo += take
+                c += 1
+
+        futs = [self._pool.submit(copy_range, bounds[w], bounds[w + 1]) for w in range(parts)
+                if bounds[w] < bounds[w + 1]]
+        for f in futs:
+            f.result()
+
+    def _results_out(self, dev, n_in, end):
+        """Pinned host buffer of a launch set's packed results (from the
+        caching host allocator), filled by one copy on the current stream.
+        Its views are handed to the results, so nothing is copied again on
+        the host and the buffer lives as long as any result of the set."""
+        torch = self._torch
+        out = torch.empty(end - n_in, dtype=torch.float64, pin_memory=True)
+        out.copy_(dev[n_in:end], non_blocking=True)
+        return out
 
     def _group(self, atoms):
         """Cached layout of a launch set whose atoms (those of one multi-atom
@@ -634,9 +680,9 @@ class GpuExecutor:
                 chunks.append(a.Q.reshape(-1))
         n_in = int(offs[-1])
         end = int(ooff[-1])
-        host, dev = self._staging(job["sid"], end)
+        host, dev = self._staging(job["sid"], n_in, end)
         hv = host.numpy()
-        np.concatenate(chunks, out=hv[:n_in])
+        self._pack(chunks, hv[:n_in])
         base = np.uint64(dev.data_ptr())
         addr = base + np.uint64(8) * offs[:-1].astype(np.uint64)
         n = len(plans)
@@ -652,9 +698,8 @@ class GpuExecutor:
         oo = base + np.uint64(8) * ooff[:-1].astype(np.uint64)
         dev[:n_in].copy_(host[:n_in], non_blocking=True)
         run_batch_arrays(handles, n, pp, qq, oo, stream=stream)
-        host[n_in:end].copy_(dev[n_in:end], non_blocking=True)
         job["group"] = (n_in, end, ooff, outs)
-        job["host"] = host
+        job["host"] = self._results_out(dev, n_in, end)
         if job["reqs"][0].kind == "slater_batch":
             job["states"].append(list(sts))
         else:
@@ -722,9 +767,9 @@ class GpuExecutor:
                 chunks.append(e[4].reshape(-1))
         n_in = sum(c.size for c in chunks)
         n_out = sum(e[2].size for e in entries)
-        host, dev = self._staging(job["sid"], n_in + n_out)
+        host, dev = self._staging(job["sid"], n_in, n_in + n_out)
         hv = host.numpy()
-        np.concatenate(chunks, out=hv[:n_in])
+        self._pack(chunks, hv[:n_in])
         base = dev.data_ptr()
         pp, qq, oo, plans = [], [], [], []
         results = {}
@@ -743,9 +788,9 @@ class GpuExecutor:
             plans.append(plan)
         dev[:n_in].copy_(host[:n_in], non_blocking=True)
         run_batch_raw(plans, pp, qq, oo, stream=stream)
-        host[n_in:out].copy_(dev[n_in:out], non_blocking=True)
         job["results"] = results
-        job["host"] = host
+        job["n_in"] = n_in
+        job["host"] = self._results_out(dev, n_in, out)
 
     def _launch_fock(self, req, job, stream):
         from .fock_yk import FockYk
@@ -790,9 +835,7 @@ class GpuExecutor:
                 results.append(err)
             elif "group" in job:
                 n_in, end, ooff, outs = job["group"]
-                if "values" not in job:  # the staging is reused: one copy out
-                    job["values"] = job["host"].numpy()[n_in:end].copy()
-                vals = job["values"]
+                vals = job["host"].numpy()  # the set's own pinned buffer: views
                 first = 0 if req.kind == "slater_batch" else i  # atom index in the set
                 got = []
                 for j, st in enumerate(sts):
@@ -801,11 +844,13 @@ class GpuExecutor:
                                             st.plan.m))
                 results.append(got if req.kind == "slater_batch" else got[0])
             elif req.kind in ("slater", "slater_batch"):
-                hv = job["host"].numpy()  # the staging is reused: results are copied out
+                hv = job["host"].numpy()  # the set's own pinned buffer: views
+                n_in = job["n_in"]
                 got = []
                 for j, st in enumerate(sts):
                     o, k = job["results"][(i, j if req.kind == "slater_batch" else -1)]
-                    got.append(SlaterResult(hv[o:o + k].copy(), st.plan.offsets, st.plan.m))
+                    o -= n_in
+                    got.append(SlaterResult(hv[o:o + k], st.plan.offsets, st.plan.m))
                 results.append(got if req.kind == "slater_batch" else got[0])
             else:  # the pinned result buffers are handed over (the arrays keep them alive)
                 results.append(FockResult({k: v.numpy() for k, v in job["host"].items()}))
@@ -827,6 +872,9 @@ class GpuExecutor:
         self._groups.clear()
         self._idle_bytes = 0
         self._fock_cache.clear()
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
 
 
 def _count_rk(ls, kmax):

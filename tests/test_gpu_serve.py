@@ -170,6 +170,8 @@ def test_batch_request_equals_its_atoms_alone():
             fb = srv.submit(SlaterBatchRequest(atoms))
             fs = srv.submit(single)
             got = fb.result()
+            if it == 0:
+                first = got  # results are views of their launch set's own buffer
             assert len(got) == len(atoms)
             for a, res in zip(atoms, got):
                 assert np.array_equal(res.values, _alone_slater(a))
@@ -198,3 +200,6 @@ def test_batch_request_equals_its_atoms_alone():
             futs = [srv.submit(a) for a in fresh]
             for a, f in zip(fresh, futs):
                 assert np.array_equal(f.result().values, _alone_slater(a))
+    # the first results are unchanged by every launch set served after them
+    for a, res in zip(atoms, first):
+        assert np.array_equal(res.values, _alone_slater(a))

@@ -163,3 +163,32 @@ def test_percentile_nearest_rank():
     v = list(range(1, 101))
     assert percentile(v, 50) == 50 and percentile(v, 99) == 99 and percentile(v, 100) == 100
     assert percentile([3.0], 99) == 3.0
+
+
+def test_threaded_packing_equals_concatenate():
+    """The executor's input packing (equal ranges of the concatenation, one
+    per thread, crossing chunk boundaries, empty chunks included) writes
+    exactly np.concatenate of the chunks and nothing past them."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from atomx_amd.serve import GpuExecutor
+    ex = object.__new__(GpuExecutor)  # packing state only: no device needed
+    ex._pack_threads = 5
+    ex._pool = ThreadPoolExecutor(5)
+    ex._PACK_SERIAL = 0
+    rng = np.random.default_rng(0)
+    try:
+        for _ in range(200):
+            chunks = [rng.standard_normal(int(rng.integers(0, 40)))
+                      for _ in range(int(rng.integers(1, 10)))]
+            n = sum(c.size for c in chunks)
+            dst = np.full(n + 3, np.nan)
+            ex._pack(chunks, dst[:n])
+            assert np.array_equal(dst[:n], np.concatenate(chunks))
+            assert np.isnan(dst[n:]).all()
+        with pytest.raises(ValueError):
+            ex._pack([np.ones(4)], np.empty(5))
+    finally:
+        ex._pool.shutdown()
