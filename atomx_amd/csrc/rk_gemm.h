@@ -39,13 +39,16 @@ struct RkWork {
     int pad;
 };
 
-// One contraction work-group: tile (ti, tj) of segment `seg`, K chunk
-// `split` of the tile's `count` chunks, written to partial slot `slot`.
+// One contraction job: tile (ti, tj) of segment `seg` over the K range
+// [kb, ke) (nodes for the FP64 kernel, 64-node K-tiles for the split kernel),
+// piece `split` of the tile's `count` pieces in K order, written to partial
+// slot `slot` (or, for a tile of one piece, to the packed output).
 struct RkJob {
     int seg;
     int tile;   // ti | tj << 16
     int split;  // split | count << 16
     int slot;
+    int kb, ke;
 };
 // The partial slots [first, first + count) of one tile, summed in order.
 struct RkTileJobs {
@@ -94,7 +97,8 @@ struct SliceArg {            // rows sliced by a separate pass
 struct GemmArg {             // contraction work-groups and split reduction
     const RkSeg *segs;
     const RkWork *work;      // tiles (reduction)
-    const RkJob *jobs;       // work-groups (contraction), in the launch's job order
+    const RkJob *jobs;       // contraction jobs, in the schedule's order
+    const int *groups;       // work-group w runs jobs [groups[w], groups[w + 1])
     const RkTileJobs *tile_jobs;
     double *partial;         // one 64*64 tile per job slot
     double *out;             // packed R^k output: a tile contracted in one K
@@ -103,7 +107,7 @@ struct GemmArg {             // contraction work-groups and split reduction
 };
 
 // Launch table: end[] is the exclusive prefix of the launch's items per plan
-// (pairs, scan items, slice tiles or contraction tiles).
+// (pairs, scan items, slice tiles, contraction work-groups or tiles).
 template <typename T>
 struct BatchOf {
     int n;
@@ -121,7 +125,9 @@ enum {
     kOrderXcd = 1,       // same, after the XCD-contiguous renumbering
     kOrderXcdSplit = 2,  // XCD-contiguous over the split-fastest list
     kOrderDynamic = 3,   // FP64: resident work-groups take jobs heaviest first from a ticket
-    kOrderCount = 4
+    kOrderBalanced = 4,  // 64 x S work-groups: per XCD one K slice, its tiles
+                         // heaviest first to the least-loaded work-group
+    kOrderCount = 5
 };
 
 // Work-group renumbering for L2 locality: the dispatcher deals consecutive
@@ -145,25 +151,40 @@ __device__ inline int batch_find(const BatchOf<T> &B, int64_t it, int64_t &local
     return p;
 }
 
-// The job of this contraction work-group under the launch's block order
-// (job index jb of the launch for the dynamic order).
+// The work-group of a static contraction launch under its block order:
+// plan p of the batch and its jobs [j0, j1).  gb >= 0 names the work-group
+// directly (dynamic order).
+struct GroupInfo {
+    int p, j0, j1;
+};
+__device__ inline GroupInfo gemm_group(const BatchOf<GemmArg> &A, int64_t gb = -1) {
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t lb = gb >= 0 ? gb : A.order == kOrderLinear ? b : xcd_block(b, nb);
+    int64_t local;
+    GroupInfo G;
+    G.p = batch_find(A, lb, local);
+    const int *grp = A.a[G.p].groups;
+    G.j0 = grp[local];
+    G.j1 = grp[local + 1];
+    return G;
+}
+
 struct JobInfo {
     int p;  // plan in the batch
-    int seg, ti, tj, split, count, slot;
+    int seg, ti, tj, split, count, slot, kb, ke;
 };
-__device__ inline JobInfo gemm_job(const BatchOf<GemmArg> &A, int64_t jb = -1) {
-    const int64_t nb = gridDim.x, b = blockIdx.x;
-    const int64_t lb = jb >= 0 ? jb : A.order == kOrderLinear ? b : xcd_block(b, nb);
-    int64_t local;
+__device__ inline JobInfo gemm_job(const BatchOf<GemmArg> &A, int p, int jb) {
+    const RkJob j = A.a[p].jobs[jb];
     JobInfo J;
-    J.p = batch_find(A, lb, local);
-    const RkJob j = A.a[J.p].jobs[local];
+    J.p = p;
     J.seg = j.seg;
     J.ti = j.tile & 0xffff;
     J.tj = static_cast<int>(static_cast<unsigned>(j.tile) >> 16);
     J.split = j.split & 0xffff;
     J.count = static_cast<int>(static_cast<unsigned>(j.split) >> 16);
     J.slot = j.slot;
+    J.kb = j.kb;
+    J.ke = j.ke;
     return J;
 }
 
@@ -205,15 +226,46 @@ int gemm_tile();
 int64_t rk_gemm_partial_doubles(int n_work, int n_split);
 void upper_tiles(int m, int seg, std::vector<RkWork> &out);
 
-// Work-groups of a contraction cut into S K chunks per tile, listed
-// tile-fastest, split-fastest and heaviest first (most 16x16 blocks inside
-// the triangle of segment rows seg_m); the slots (and the reduction order)
-// are tile-major and do not depend on the list.
+// K extent of a contraction's jobs: k_len units (nodes for the FP64 kernel,
+// 64-node K-tiles for the split kernel), chunk boundaries on multiples of
+// k_align units.
+struct KSpace {
+    int64_t k_len;
+    int k_align;
+    bool fp64;  // FP64 kernel: a tile's cost is its 16x16 blocks inside the triangle
+};
+KSpace k_space(int64_t n, bool fp64);
+
+// Jobs of a contraction cut into S K chunks per tile (one job per
+// work-group), listed tile-fastest, split-fastest and heaviest first (most
+// 16x16 blocks inside the triangle of segment rows seg_m); the slots (and the
+// reduction order) are tile-major and do not depend on the list.
 struct JobTables {
     std::vector<RkJob> tile_fastest, split_fastest, heaviest;
     std::vector<RkTileJobs> tiles;
 };
-JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m, int S);
+JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m, int S,
+                    const KSpace &ks);
+
+// Balanced jobs: the K range is cut into one slice per XCD (at most 8);
+// each slice's tiles, heaviest first (cost = 16x16 blocks inside the
+// triangle for the FP64 kernel), go whole to the least-loaded of the
+// slice's n_groups / slices work-groups, listed slice by slice (so the
+// XCD-contiguous block order gives each XCD one slice and dispatches its
+// heavy tiles first).  A tile's pieces in K order are its slots.  groups
+// has one entry per work-group plus one; work-groups beyond the slice's
+// tile count stay empty.  (Cutting the slices into equal-weight ranges of
+// tile pieces instead measured 1.6x slower: many short pieces per
+// work-group.)
+struct BalancedJobs {
+    std::vector<RkJob> jobs;
+    std::vector<int> groups;
+    std::vector<RkTileJobs> tiles;
+};
+BalancedJobs make_balanced_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m,
+                                int n_groups, const KSpace &ks);
+// Work-groups of the balanced order at level S.
+int balanced_groups(int S);
 
 // Host copy of a plan: sizes and the device pointers the launchers need.
 struct PlanHost {
@@ -230,8 +282,10 @@ struct PlanHost {
     const RkSeg *segs;
     const RkWork *work;
     const RkJob *jobs;      // the schedule's job list (order and K-split level)
+    const int *groups;      // jobs of each work-group (GemmArg::groups)
     const RkTileJobs *tile_jobs;
-    int n_jobs;
+    int n_groups;           // work-groups of a static launch
+    int needs_reduce;       // some tile has more than one K piece
     int *counter;           // two ints, zero between launches (dynamic order)
     double *partial;
 };

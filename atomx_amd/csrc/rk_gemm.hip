@@ -17,6 +17,7 @@
 // FP64 path: v_mfma_f64_16x16x4_f64, K split across the four waves of a
 // work-group (below).
 #include <algorithm>
+#include <numeric>
 
 #include "ax_common.h"
 #include "rk_gemm.h"
@@ -46,7 +47,7 @@ typedef const double __attribute__((address_space(1))) gdouble;
 // products pair up).
 // ---------------------------------------------------------------------------
 constexpr int KW = 8;         // nodes per register stage
-constexpr int KW_ALIGN = 32;  // work-group K chunks are multiples of 4 waves x KW
+constexpr int KW_ALIGN = 32;  // job K ranges start on multiples of 4 waves x KW nodes
 
 struct KwOperands {
     double a[4][2], b[4][2];
@@ -135,12 +136,10 @@ __device__ inline void kw_job(const ax::BatchOf<ax::GemmArg> &A, const ax::JobIn
                               double (&park)[2][16 * 4 * 64]) {
     const ax::GemmArg &g = A.a[J.p];
     const ax::RkSeg sg = g.segs[J.seg];
-    const int64_t chunk =
-        ((sg.n + J.count - 1) / J.count + KW_ALIGN - 1) / KW_ALIGN * KW_ALIGN;
-    const int64_t kbeg = (int64_t)J.split * chunk;
-    const int64_t kend = kbeg + chunk < sg.n ? kbeg + chunk : sg.n;
+    // the job's K range in quarters of whole 8-node stages, one per wave
+    const int64_t kbeg = J.kb, kend = J.ke > J.kb ? J.ke : J.kb;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t wch = chunk / 4;
+    const int64_t wch = ((kend - kbeg + 3) / 4 + KW - 1) / KW * KW;
     const int64_t kb0 = kbeg + wave * wch;
     const int64_t kb = kb0 < kend ? kb0 : kend;
     const int64_t ke = kb0 + wch < kend ? kb0 + wch : kend;
@@ -225,10 +224,19 @@ __device__ inline void kw_job(const ax::BatchOf<ax::GemmArg> &A, const ax::JobIn
 #undef KW_FOR_ACTIVE
 }
 
+// The jobs of one work-group, one after the other.
+__device__ inline void kw_group(const ax::BatchOf<ax::GemmArg> &A, const ax::GroupInfo &G,
+                                double (&park)[2][16 * 4 * 64]) {
+    for (int j = G.j0; j < G.j1; ++j) {
+        if (j > G.j0) __syncthreads();  // the partial-sum area is reused
+        kw_job(A, ax::gemm_job(A, G.p, j), park);
+    }
+}
+
 __global__ __launch_bounds__(256, 2) void rk_gemm_f64_kernel(const ax::BatchOf<ax::GemmArg> A) {
     __shared__ double park[2][16 * 4 * 64];
     if (A.order != ax::kOrderDynamic) {
-        kw_job(A, ax::gemm_job(A), park);
+        kw_group(A, ax::gemm_group(A), park);
         return;
     }
     // dynamic order: every resident work-group takes the next job from the
@@ -242,7 +250,7 @@ __global__ __launch_bounds__(256, 2) void rk_gemm_f64_kernel(const ax::BatchOf<a
         const int jb = ticket;
         __syncthreads();
         if (jb >= total) break;
-        kw_job(A, ax::gemm_job(A, jb), park);
+        kw_group(A, ax::gemm_group(A, jb), park);
         __syncthreads();  // the partial-sum area is reused by the next job
     }
     if (threadIdx.x == 0 && atomicAdd(&A.counter[1], 1) == static_cast<int>(gridDim.x) - 1) {
@@ -293,8 +301,8 @@ int64_t rk_gemm_partial_doubles(int n_work, int n_split) {
 
 namespace {
 
-// Fills the per-plan prefix of contraction jobs (or, for the reduction, of
-// tiles); returns the total.
+// Fills the per-plan prefix of contraction work-groups (or, for the
+// reduction, of tiles); returns the total.
 int64_t fill_work(BatchOf<GemmArg> &A, const PlanHost *const *plans, int n,
                   double *const *out, bool jobs) {
     A.n = n;
@@ -302,9 +310,9 @@ int64_t fill_work(BatchOf<GemmArg> &A, const PlanHost *const *plans, int n,
     int64_t total = 0;
     for (int i = 0; i < n; ++i) {
         const PlanHost *h = plans[i];
-        total += jobs ? h->n_jobs : h->n_work;
+        total += jobs ? h->n_groups : h->n_work;
         A.end[i] = total;
-        A.a[i] = GemmArg{h->segs, h->work, h->jobs, h->tile_jobs, h->partial,
+        A.a[i] = GemmArg{h->segs, h->work, h->jobs, h->groups, h->tile_jobs, h->partial,
                          out ? out[i] : nullptr};
     }
     return total;
@@ -381,12 +389,12 @@ ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const
         set_error("reduce_batch_launch: %d plans exceed the batch limit %d", n, kMaxBatch);
         return AX_ERR_VALUE;
     }
-    // plans contracted in one K chunk per tile have nothing to reduce
+    // plans contracted in one K piece per tile have nothing to reduce
     const PlanHost *sub[kMaxBatch];
     double *sub_out[kMaxBatch];
     int ns = 0;
     for (int i = 0; i < n; ++i)
-        if (plans[i]->n_split > 1) {
+        if (plans[i]->needs_reduce) {
             sub_out[ns] = out[i];
             sub[ns++] = plans[i];
         }
@@ -406,49 +414,123 @@ void upper_tiles(int m, int seg, std::vector<RkWork> &out) {
         for (int tj = ti; tj < T; ++tj) out.push_back(RkWork{seg, ti, tj, 0});
 }
 
-JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m, int S) {
+namespace {
+
+// 16x16 blocks of a tile inside the triangle of its segment (the FP64
+// kernel's MFMA work per K unit).
+int tile_blocks(const RkWork &w, int m) {
+    const int rb = std::min(4, (m - w.ti * BM + 15) / 16);
+    const int cb = std::min(4, (m - w.tj * BM + 15) / 16);
+    int n = 0;
+    for (int a = 0; a < rb; ++a)
+        for (int b = 0; b < cb; ++b) n += (w.ti != w.tj || a <= b) ? 1 : 0;
+    return n;
+}
+
+RkJob make_job(const RkWork &w, int split, int count, int slot, int64_t kb, int64_t ke) {
+    return RkJob{w.seg, w.ti | (w.tj << 16), split | (count << 16), slot, static_cast<int>(kb),
+                 static_cast<int>(ke)};
+}
+
+}  // namespace
+
+KSpace k_space(int64_t n, bool fp64) {
+    return fp64 ? KSpace{n, KW_ALIGN, true}
+                : KSpace{(n + kSplitKTile - 1) / kSplitKTile, 1, false};
+}
+
+JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m, int S,
+                    const KSpace &ks) {
     JobTables t;
     const int nw = static_cast<int>(work.size());
-    std::vector<int> cnt(nw);
-    int first = 0, most = 1;
+    // every tile in S chunks of equal length (weighting the chunk count by
+    // the tile's blocks inside the triangle was measured no faster)
+    const int64_t per = ((ks.k_len + S - 1) / S + ks.k_align - 1) / ks.k_align * ks.k_align;
     t.tiles.resize(nw);
-    for (int e = 0; e < nw; ++e) {
-        // every tile in S chunks (weighting the chunk count by the tile's
-        // blocks inside the triangle was measured no faster)
-        cnt[e] = S;
-        t.tiles[e] = RkTileJobs{first, cnt[e]};
-        first += cnt[e];
-        most = std::max(most, cnt[e]);
-    }
+    for (int e = 0; e < nw; ++e) t.tiles[e] = RkTileJobs{e * S, S};
     auto job = [&](int e, int sp) {
-        const RkWork &w = work[e];
-        return RkJob{w.seg, w.ti | (w.tj << 16), sp | (cnt[e] << 16), t.tiles[e].first + sp};
+        const int64_t kb = std::min<int64_t>(sp * per, ks.k_len);
+        const int64_t ke = std::min<int64_t>(kb + per, ks.k_len);
+        return make_job(work[e], sp, S, e * S + sp, kb, ke);
     };
-    t.tile_fastest.reserve(first);
-    t.split_fastest.reserve(first);
-    for (int sp = 0; sp < most; ++sp)
-        for (int e = 0; e < nw; ++e)
-            if (sp < cnt[e]) t.tile_fastest.push_back(job(e, sp));
+    t.tile_fastest.reserve((size_t)nw * S);
+    t.split_fastest.reserve((size_t)nw * S);
+    for (int sp = 0; sp < S; ++sp)
+        for (int e = 0; e < nw; ++e) t.tile_fastest.push_back(job(e, sp));
     for (int e = 0; e < nw; ++e)
-        for (int sp = 0; sp < cnt[e]; ++sp) t.split_fastest.push_back(job(e, sp));
+        for (int sp = 0; sp < S; ++sp) t.split_fastest.push_back(job(e, sp));
     // heaviest first: tiles by their 16x16 blocks inside the triangle (the
     // MFMA work of each of their jobs), ties in tile-fastest order
     std::vector<int> blocks(nw);
-    for (int e = 0; e < nw; ++e) {
-        const RkWork &w = work[e];
-        const int m = seg_m[w.seg];
-        const int rb = std::min(4, (m - w.ti * BM + 15) / 16);
-        const int cb = std::min(4, (m - w.tj * BM + 15) / 16);
-        for (int a = 0; a < rb; ++a)
-            for (int b = 0; b < cb; ++b) blocks[e] += (w.ti != w.tj || a <= b) ? 1 : 0;
-    }
+    for (int e = 0; e < nw; ++e) blocks[e] = tile_blocks(work[e], seg_m[work[e].seg]);
     std::vector<int> ord(nw);
     for (int e = 0; e < nw; ++e) ord[e] = e;
     std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return blocks[x] > blocks[y]; });
-    t.heaviest.reserve(first);
+    t.heaviest.reserve((size_t)nw * S);
     for (int e : ord)
-        for (int sp = 0; sp < cnt[e]; ++sp) t.heaviest.push_back(job(e, sp));
+        for (int sp = 0; sp < S; ++sp) t.heaviest.push_back(job(e, sp));
     return t;
+}
+
+int balanced_groups(int S) { return 64 * S; }
+
+BalancedJobs make_balanced_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m,
+                                int n_groups, const KSpace &ks) {
+    BalancedJobs b;
+    const int nw = static_cast<int>(work.size());
+    const int64_t U = (ks.k_len + ks.k_align - 1) / ks.k_align;  // K units per tile
+    if (nw == 0 || U == 0) {
+        b.groups.assign(1, 0);
+        b.tiles.assign(nw, RkTileJobs{0, 0});
+        return b;
+    }
+    std::vector<int64_t> wt(nw);  // cost of one K unit of each tile
+    for (int e = 0; e < nw; ++e) wt[e] = ks.fp64 ? tile_blocks(work[e], seg_m[work[e].seg]) : 16;
+    // K slices, one per XCD (each XCD reads its own K slice of every operand
+    // row)
+    const int X = static_cast<int>(std::min<int64_t>(8, U));
+    const int gx = std::max(1, n_groups / X);
+    struct Piece {
+        int e;
+        int64_t u0, u1;  // K units of tile e
+    };
+    std::vector<Piece> pieces;
+    b.groups.push_back(0);
+    // per slice: whole tile-slices, heaviest first, each to the least
+    // loaded work-group of the slice (ties to the lowest index)
+    std::vector<int> ord(nw);
+    for (int e = 0; e < nw; ++e) ord[e] = e;
+    std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return wt[p] > wt[q]; });
+    for (int x = 0; x < X; ++x) {
+        const int64_t s0 = U * x / X, s1 = U * (x + 1) / X;
+        const int G = std::min(gx, nw);
+        std::vector<int64_t> load(G, 0);
+        std::vector<std::vector<int>> bins(G);
+        for (int e : ord) {
+            const int g = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            load[g] += wt[e];
+            bins[g].push_back(e);
+        }
+        for (int g = 0; g < gx; ++g) {
+            if (g < G)
+                for (int e : bins[g]) pieces.push_back(Piece{e, s0, s1});
+            b.groups.push_back(static_cast<int>(pieces.size()));
+        }
+    }
+    // slots tile-major, a tile's pieces in K order (one per slice)
+    std::vector<int> count(nw, 0), first(nw, 0), seen(nw, 0);
+    for (const Piece &pc : pieces) ++count[pc.e];
+    for (int t = 1; t < nw; ++t) first[t] = first[t - 1] + count[t - 1];
+    b.tiles.resize(nw);
+    for (int t = 0; t < nw; ++t) b.tiles[t] = RkTileJobs{first[t], count[t]};
+    b.jobs.reserve(pieces.size());
+    for (const Piece &pc : pieces) {
+        const int sp = seen[pc.e]++;
+        const int64_t kb = std::min<int64_t>(pc.u0 * ks.k_align, ks.k_len);
+        const int64_t ke = std::min<int64_t>(pc.u1 * ks.k_align, ks.k_len);
+        b.jobs.push_back(make_job(work[pc.e], sp, count[pc.e], first[pc.e] + sp, kb, ke));
+    }
+    return b;
 }
 
 }  // namespace ax

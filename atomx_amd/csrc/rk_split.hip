@@ -102,21 +102,14 @@ __device__ inline void stage_store(int8_t *__restrict__ lds, const v4i (&reg)[S]
         *reinterpret_cast<v4i *>(lds + s2 * PLANEB + r * ROWB + q * 16) = valid ? reg[s2] : zero;
 }
 
+// One job (a K-tile range of a tile) of the split contraction.
 template <int S>
-__global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax::GemmArg> A) {
-    extern __shared__ __attribute__((aligned(16))) int8_t lds_split[];
-    int8_t *As = lds_split;                // [S * PLANEB]
-    int8_t *Bs = lds_split + S * PLANEB;   // [S * PLANEB]
-    __shared__ int eAs[BM], eBs[BM];
-
-    // one 1-D grid over the launch's jobs
-    const ax::JobInfo J = ax::gemm_job(A);
+__device__ inline void split_job(const ax::BatchOf<ax::GemmArg> &A, const ax::JobInfo &J,
+                                 int8_t *__restrict__ As, int8_t *__restrict__ Bs, int *eAs,
+                                 int *eBs) {
     const ax::GemmArg &g = A.a[J.p];
     const ax::RkSeg sg = g.segs[J.seg];
-    const int64_t n_tiles = (sg.n + KT - 1) / KT;
-    const int64_t per = (n_tiles + J.count - 1) / J.count;
-    const int64_t tb = (int64_t)J.split * per;
-    const int64_t te = tb + per < n_tiles ? tb + per : n_tiles;
+    const int64_t tb = J.kb, te = J.ke;  // K-tiles [tb, te)
     const int row0 = J.ti * BM, col0 = J.tj * BM;
 
     const int tid = threadIdx.x;
@@ -231,6 +224,18 @@ __global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax
                 const int j = wc + b * 16 + fr;
                 dst[i * BM + j] = acc[a][b][r];
             }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void rk_gemm_split_kernel(const ax::BatchOf<ax::GemmArg> A) {
+    extern __shared__ __attribute__((aligned(16))) int8_t lds_split[];
+    int8_t *As = lds_split;                // [S * PLANEB]
+    int8_t *Bs = lds_split + S * PLANEB;   // [S * PLANEB]
+    __shared__ int eAs[BM], eBs[BM];
+    // this work-group's jobs, one after the other (every K-tile step starts
+    // with a barrier, so the LDS tiles are free when the next job stages)
+    const ax::GroupInfo G = ax::gemm_group(A);
+    for (int j = G.j0; j < G.j1; ++j) split_job<S>(A, ax::gemm_job(A, G.p, j), As, Bs, eAs, eBs);
 }
 
 }  // namespace

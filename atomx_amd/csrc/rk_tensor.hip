@@ -37,10 +37,14 @@ struct ax_rk_tensor {
     int64_t footprint = 0;
     ax::PlanHost host{};
     // job lists of every K-split level S = 1..split_cap (index S - 1), on
-    // the device: tile-fastest, split-fastest, per-tile slot ranges
-    std::vector<const ax::RkJob *> jobs_tf, jobs_sf, jobs_hv;
-    std::vector<const ax::RkTileJobs *> tile_jobs;
-    std::vector<int> n_jobs;
+    // the device: tile-fastest, split-fastest, heaviest first (one job per
+    // work-group: `identity` groups) with their per-tile slot ranges, and the
+    // balanced order's jobs, work-groups and slot ranges
+    std::vector<const ax::RkJob *> jobs_tf, jobs_sf, jobs_hv, jobs_bal;
+    std::vector<const ax::RkTileJobs *> tile_jobs, tile_jobs_bal;
+    std::vector<const int *> groups_bal;
+    std::vector<int> n_jobs, n_groups_bal, reduce_bal;
+    const int *identity = nullptr;
     struct DoneMark *done = nullptr;  // recorded after the plan's last launch set
     hipEvent_t ready = nullptr; // recorded after the set-up upload
     bool ready_pending = true;  // launches must still wait on `ready`
@@ -101,11 +105,21 @@ struct Layout {
 void apply_schedule(ax_rk_tensor *p) {
     const int S = p->host.n_split;
     const int o = p->host.order;
+    if (o == ax::kOrderBalanced) {
+        p->host.jobs = p->jobs_bal[S - 1];
+        p->host.groups = p->groups_bal[S - 1];
+        p->host.tile_jobs = p->tile_jobs_bal[S - 1];
+        p->host.n_groups = p->n_groups_bal[S - 1];
+        p->host.needs_reduce = p->reduce_bal[S - 1];
+        return;
+    }
     p->host.jobs = o == ax::kOrderXcdSplit   ? p->jobs_sf[S - 1]
                    : o == ax::kOrderDynamic ? p->jobs_hv[S - 1]
                                             : p->jobs_tf[S - 1];
+    p->host.groups = p->identity;
     p->host.tile_jobs = p->tile_jobs[S - 1];
-    p->host.n_jobs = p->n_jobs[S - 1];
+    p->host.n_groups = p->n_jobs[S - 1];
+    p->host.needs_reduce = S > 1;
 }
 
 void free_plan(ax_rk_tensor *p) {
@@ -345,8 +359,17 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     AX_REQUIRE(n_work == 0 || work.back().tj < 0x8000, "ax_rk_tensor_create: too many tiles");
     std::vector<int> seg_m;
     for (int k : seg_k) seg_m.push_back(m_k[k]);
+    const ax::KSpace ks = ax::k_space(n, precision == AX_PREC_FP64);
     std::vector<ax::JobTables> jobs;
-    for (int S = 1; S <= split_cap; ++S) jobs.push_back(ax::make_jobs(work, seg_m, S));
+    std::vector<ax::BalancedJobs> bal;
+    int64_t slots = (int64_t)n_work * split_cap;  // partial tiles the schedules need
+    for (int S = 1; S <= split_cap; ++S) {
+        jobs.push_back(ax::make_jobs(work, seg_m, S, ks));
+        bal.push_back(ax::make_balanced_jobs(work, seg_m, ax::balanced_groups(S), ks));
+        slots = std::max<int64_t>(slots, (int64_t)bal.back().jobs.size());
+    }
+    std::vector<int> ident((size_t)n_work * split_cap + 1);
+    for (size_t i = 0; i < ident.size(); ++i) ident[i] = static_cast<int>(i);
 
     // one device block for everything the plan owns
     Layout L;
@@ -360,20 +383,26 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     const int64_t o_iout = L.add((int64_t)it_out.size() * 4);
     const int64_t o_segs = L.add((int64_t)seg_k.size() * sizeof(ax::RkSeg));
     const int64_t o_work = L.add((int64_t)n_work * sizeof(ax::RkWork));
-    std::vector<int64_t> o_jtf, o_jsf, o_jhv, o_tj;
+    std::vector<int64_t> o_jtf, o_jsf, o_jhv, o_tj, o_jb, o_gb, o_tjb;
     for (const ax::JobTables &t : jobs) {
         o_jtf.push_back(L.add((int64_t)t.tile_fastest.size() * sizeof(ax::RkJob)));
         o_jsf.push_back(L.add((int64_t)t.split_fastest.size() * sizeof(ax::RkJob)));
         o_jhv.push_back(L.add((int64_t)t.heaviest.size() * sizeof(ax::RkJob)));
         o_tj.push_back(L.add((int64_t)t.tiles.size() * sizeof(ax::RkTileJobs)));
     }
+    for (const ax::BalancedJobs &t : bal) {
+        o_jb.push_back(L.add((int64_t)t.jobs.size() * sizeof(ax::RkJob)));
+        o_gb.push_back(L.add((int64_t)t.groups.size() * sizeof(int)));
+        o_tjb.push_back(L.add((int64_t)t.tiles.size() * sizeof(ax::RkTileJobs)));
+    }
+    const int64_t o_ident = L.add((int64_t)ident.size() * sizeof(int));
     const int64_t o_ctr = L.add(2 * sizeof(int));  // dynamic-order ticket (zero)
     const int64_t head = L.size;  // everything above is uploaded from the host
     // split precision on 8-node cells: the scan writes Y^k slices only
     const bool y_fp64 = !(precision == AX_PREC_SPLIT && ax::scan_slices_fused(ng, nv));
     const int64_t o_D = L.add((int64_t)n_pairs * n * 8);
     const int64_t o_Y = y_fp64 ? L.add(yr * n * 8) : -1;
-    const int64_t o_part = L.add(ax::rk_gemm_partial_doubles(n_work, split_cap) * 8);
+    const int64_t o_part = L.add(slots * ax::gemm_tile() * ax::gemm_tile() * 8);
     int64_t o_Ds = 0, o_Ys = 0, o_De = 0, o_Ye = 0;
     const int slices = precision == AX_PREC_SPLIT ? choose_slices(r_host, nv, ng, kmax) : 0;
     if (precision == AX_PREC_SPLIT) {
@@ -503,6 +532,19 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
             plan->tile_jobs.push_back(static_cast<const ax::RkTileJobs *>(dptr(o_tj[i])));
             plan->n_jobs.push_back(static_cast<int>(t.tile_fastest.size()));
         }
+        for (size_t i = 0; i < bal.size(); ++i) {
+            const ax::BalancedJobs &t = bal[i];
+            put(o_jb[i], t.jobs.data(), t.jobs.size() * sizeof(ax::RkJob));
+            put(o_gb[i], t.groups.data(), t.groups.size() * sizeof(int));
+            put(o_tjb[i], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
+            plan->jobs_bal.push_back(static_cast<const ax::RkJob *>(dptr(o_jb[i])));
+            plan->groups_bal.push_back(static_cast<const int *>(dptr(o_gb[i])));
+            plan->tile_jobs_bal.push_back(static_cast<const ax::RkTileJobs *>(dptr(o_tjb[i])));
+            plan->n_groups_bal.push_back(static_cast<int>(t.groups.size()) - 1);
+            plan->reduce_bal.push_back(t.jobs.size() > t.tiles.size() ? 1 : 0);
+        }
+        put(o_ident, ident.data(), ident.size() * sizeof(int));
+        plan->identity = static_cast<const int *>(dptr(o_ident));
     }
     ax::PlanHost &h = plan->host;
     h.n = n;
@@ -761,7 +803,7 @@ ax_status ax_rk_tensor_tune(ax_rk_tensor *plan, const double *P, const double *Q
 namespace {
 
 struct GemmWorkspace {
-    int64_t o_seg, o_work, o_jobs, o_tj, o_part, o_Ds, o_Ys, o_De, o_Ye, size;
+    int64_t o_seg, o_work, o_jobs, o_grp, o_tj, o_part, o_Ds, o_Ys, o_De, o_Ye, size;
 };
 
 GemmWorkspace gemm_workspace(int m, int64_t n, int precision, int n_work, int n_split) {
@@ -770,6 +812,7 @@ GemmWorkspace gemm_workspace(int m, int64_t n, int precision, int n_work, int n_
     g.o_seg = L.add(sizeof(ax::RkSeg));
     g.o_work = L.add((int64_t)n_work * sizeof(ax::RkWork));
     g.o_jobs = L.add((int64_t)n_work * n_split * sizeof(ax::RkJob));
+    g.o_grp = L.add(((int64_t)n_work * n_split + 1) * sizeof(int));
     g.o_tj = L.add((int64_t)n_work * sizeof(ax::RkTileJobs));
     g.o_part = L.add(ax::rk_gemm_partial_doubles(n_work, n_split) * 8);
     if (precision == AX_PREC_SPLIT) {
@@ -847,10 +890,15 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
         sg.ye_ld = tiles;
     }
     AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_seg, &sg, sizeof(sg), hipMemcpyHostToDevice, s));
-    const ax::JobTables jt = ax::make_jobs(work, std::vector<int>{m}, split);
+    const ax::JobTables jt = ax::make_jobs(work, std::vector<int>{m}, split,
+                                           ax::k_space(n, precision == AX_PREC_FP64));
     AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_jobs, jt.tile_fastest.data(),
                                 jt.tile_fastest.size() * sizeof(ax::RkJob), hipMemcpyHostToDevice,
                                 s));
+    std::vector<int> ident(jt.tile_fastest.size() + 1);  // one job per work-group
+    for (size_t i = 0; i < ident.size(); ++i) ident[i] = static_cast<int>(i);
+    AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_grp, ident.data(), ident.size() * sizeof(int),
+                                hipMemcpyHostToDevice, s));
     AX_CHECK_HIP(hipMemcpyAsync(ws + g.o_tj, jt.tiles.data(),
                                 jt.tiles.size() * sizeof(ax::RkTileJobs), hipMemcpyHostToDevice,
                                 s));
@@ -866,8 +914,10 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
     h.segs = reinterpret_cast<const ax::RkSeg *>(ws + g.o_seg);
     h.work = reinterpret_cast<const ax::RkWork *>(ws + g.o_work);
     h.jobs = reinterpret_cast<const ax::RkJob *>(ws + g.o_jobs);
+    h.groups = reinterpret_cast<const int *>(ws + g.o_grp);
     h.tile_jobs = reinterpret_cast<const ax::RkTileJobs *>(ws + g.o_tj);
-    h.n_jobs = static_cast<int>(jt.tile_fastest.size());
+    h.n_groups = static_cast<int>(jt.tile_fastest.size());
+    h.needs_reduce = split > 1;
     h.partial = reinterpret_cast<double *>(ws + g.o_part);
     const ax::PlanHost *hp = &h;
     ax_status st = ax::contract_batch_launch(&hp, 1, &out, s);

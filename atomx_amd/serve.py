@@ -609,15 +609,28 @@ class GpuExecutor:
         for f in futs:
             f.result()
 
-    def _results_out(self, dev, n_in, end):
-        """Pinned host buffer of a launch set's packed results (from the
-        caching host allocator), filled by one copy on the current stream.
-        Its views are handed to the results, so nothing is copied again on
-        the host and the buffer lives as long as any result of the set."""
+    _OWN_RESULTS = 1 << 17  # result values from which a launch set gets its own buffer
+
+    def _host_need(self, n_in, end):
+        """Pinned staging values a launch set needs: its inputs, and its
+        results too when they come back through the staging."""
+        return n_in if end - n_in >= self._OWN_RESULTS else end
+
+    def _results_out(self, job, host, dev, n_in, end):
+        """One device-to-host copy of a launch set's packed results on the
+        current stream.  Large results go to a pinned buffer of the set's
+        own (torch's caching host allocator) whose views become the results,
+        so they are never copied again on the host; small ones go through
+        the slot's staging and are copied out once when the set finishes."""
         torch = self._torch
-        out = torch.empty(end - n_in, dtype=torch.float64, pin_memory=True)
+        if end - n_in >= self._OWN_RESULTS:
+            out = torch.empty(end - n_in, dtype=torch.float64, pin_memory=True)
+            job["own"] = True
+        else:
+            out = host[n_in:end]
+            job["own"] = False
         out.copy_(dev[n_in:end], non_blocking=True)
-        return out
+        job["host"] = out
 
     def _group(self, atoms):
         """Cached layout of a launch set whose atoms (those of one multi-atom
@@ -680,7 +693,7 @@ class GpuExecutor:
                 chunks.append(a.Q.reshape(-1))
         n_in = int(offs[-1])
         end = int(ooff[-1])
-        host, dev = self._staging(job["sid"], n_in, end)
+        host, dev = self._staging(job["sid"], self._host_need(n_in, end), end)
         hv = host.numpy()
         self._pack(chunks, hv[:n_in])
         base = np.uint64(dev.data_ptr())
@@ -699,7 +712,7 @@ class GpuExecutor:
         dev[:n_in].copy_(host[:n_in], non_blocking=True)
         run_batch_arrays(handles, n, pp, qq, oo, stream=stream)
         job["group"] = (n_in, end, ooff, outs)
-        job["host"] = self._results_out(dev, n_in, end)
+        self._results_out(job, host, dev, n_in, end)
         if job["reqs"][0].kind == "slater_batch":
             job["states"].append(list(sts))
         else:
@@ -767,7 +780,7 @@ class GpuExecutor:
                 chunks.append(e[4].reshape(-1))
         n_in = sum(c.size for c in chunks)
         n_out = sum(e[2].size for e in entries)
-        host, dev = self._staging(job["sid"], n_in, n_in + n_out)
+        host, dev = self._staging(job["sid"], self._host_need(n_in, n_in + n_out), n_in + n_out)
         hv = host.numpy()
         self._pack(chunks, hv[:n_in])
         base = dev.data_ptr()
@@ -790,7 +803,7 @@ class GpuExecutor:
         run_batch_raw(plans, pp, qq, oo, stream=stream)
         job["results"] = results
         job["n_in"] = n_in
-        job["host"] = self._results_out(dev, n_in, out)
+        self._results_out(job, host, dev, n_in, out)
 
     def _launch_fock(self, req, job, stream):
         from .fock_yk import FockYk
@@ -835,7 +848,10 @@ class GpuExecutor:
                 results.append(err)
             elif "group" in job:
                 n_in, end, ooff, outs = job["group"]
-                vals = job["host"].numpy()  # the set's own pinned buffer: views
+                if "values" not in job:  # views of the set's own buffer, or one copy
+                    v = job["host"].numpy()
+                    job["values"] = v if job["own"] else v.copy()
+                vals = job["values"]
                 first = 0 if req.kind == "slater_batch" else i  # atom index in the set
                 got = []
                 for j, st in enumerate(sts):
@@ -844,7 +860,10 @@ class GpuExecutor:
                                             st.plan.m))
                 results.append(got if req.kind == "slater_batch" else got[0])
             elif req.kind in ("slater", "slater_batch"):
-                hv = job["host"].numpy()  # the set's own pinned buffer: views
+                if "values" not in job:  # views of the set's own buffer, or one copy
+                    v = job["host"].numpy()
+                    job["values"] = v if job["own"] else v.copy()
+                hv = job["values"]
                 n_in = job["n_in"]
                 got = []
                 for j, st in enumerate(sts):

@@ -87,10 +87,12 @@ class SlaterTensor:
         return out
 
     STAGES = ("density", "scan", "slice", "contract", "reduce")
-    # block orders of the contraction launch; the last one (FP64) keeps one
-    # round of work-groups that take jobs heaviest first from a ticket
+    # block orders of the contraction launch: three static job orders, one
+    # round of resident work-groups taking jobs heaviest first from a ticket
+    # (FP64), and one K slice per XCD with its tiles packed heaviest first
+    # onto the least-loaded work-groups
     ORDERS = ("tile-major", "xcd-contiguous", "xcd-contiguous-split-fastest",
-              "dynamic-heaviest-first")
+              "dynamic-heaviest-first", "balanced-k-ranges")
 
     def schedule(self):
         """(block order name, K split) of the contraction."""

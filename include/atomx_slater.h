@@ -224,10 +224,17 @@ enum {
 ax_status ax_rk_tensor_run_stage(ax_rk_tensor *plan, int stage, const double *P,
                                  const double *Q, double *out, void *stream);
 
-/* Contraction schedule of a plan: block order (0 = tile-major, 1 = tiles
- * dealt to the 8 XCDs in contiguous ranges, 2 = as 1 with the K split
- * fastest) and K split (1 .. 2x the default, power of two).  The split fixes
- * the reduction order, so results are deterministic per schedule.
+/* Contraction schedule of a plan: block order and level S (1 .. min(16,
+ * 4x the default K split)).  Orders 0-3 cut every tile's K range into S
+ * equal chunks, one work-group each: 0 = tile-major, 1 = work-groups dealt
+ * to the 8 XCDs in contiguous ranges, 2 = as 1 with the K split fastest,
+ * 3 = (FP64) resident work-groups take the chunks heaviest tile first from
+ * a ticket.  Order 4 runs 64 x S work-groups: the K range is cut into one
+ * slice per XCD (at most 8), and each slice's tiles, heaviest first (16x16
+ * blocks inside the triangle), go whole to the least-loaded of that XCD's
+ * 8 x S work-groups.  The schedule fixes how each
+ * tile's K range is cut and the order its pieces are summed, so results
+ * are deterministic per schedule.
  * tune() runs the plan once on P, Q, then times every (order, split) with
  * `reps` timed repetitions of the contraction and reduction, keeps the
  * fastest, and leaves its result in `out`; best_ms_host (may be NULL)

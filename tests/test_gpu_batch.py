@@ -122,7 +122,8 @@ def test_contraction_splits_match_oracle(problems):
                 assert _oracle_scaled_err(pl, got, gr, grw, P, Q, kmax) < 1e-12
         if i < 4:
             assert _oracle_scaled_err(pl, ref, gr, grw, P, Q, kmax) < 1e-12
-        pl.set_schedule("xcd-contiguous" if i % 2 else "tile-major", min(1 + i, cap))
+        pl.set_schedule(("tile-major", "xcd-contiguous", "balanced-k-ranges")[i % 3],
+                        min(1 + i, cap))
         plans.append(pl)
         ins.append((Pd, Qd))
         alone.append(dv.host(pl.run(Pd, Qd)))
@@ -135,10 +136,10 @@ def test_contraction_splits_match_oracle(problems):
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])
 def test_single_chunk_tiles_write_every_value(problems, precision):
-    """At K split 1 the contraction writes the packed values itself (no
-    reduction launch): every value of an output pre-filled with NaN is
-    written, under every block order, and agrees with the default
-    schedule's result (split partials summed by the reduction)."""
+    """At K split 1 the static orders' contraction writes the packed values
+    itself (no reduction launch; the balanced order still cuts tiles into
+    pieces): every value of an output pre-filled with NaN is written, under
+    every block order, and agrees with the default schedule's result."""
     import torch
 
     from atomx_amd import device as dv

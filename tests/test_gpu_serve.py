@@ -158,14 +158,19 @@ def test_resident_state_and_budget():
         assert np.array_equal(r.values, ref)
 
 
-def test_batch_request_equals_its_atoms_alone():
+@pytest.mark.parametrize("own_results", [False, True])
+def test_batch_request_equals_its_atoms_alone(own_results):
     """A SlaterBatchRequest (resident states per atom, mixed with single
-    requests) returns, per atom, bitwise what the atom gives alone."""
+    requests) returns, per atom, bitwise what the atom gives alone; with
+    results copied out of the staging, or (own_results) as views of each
+    launch set's own pinned buffer."""
     from atomx_amd.serve import SlaterBatchRequest, SlaterServer
     atoms = [_slater_req(24 + 8 * i, 2 + i % 4, 2 + i % 3, 40 + i, precision="fp64",
                          state=("b", i)) for i in range(5)]
     single = _slater_req(30, 3, 2, 99, precision="fp64")
     with SlaterServer(max_inflight=2, max_batch=4) as srv:
+        if own_results:
+            srv.executor._OWN_RESULTS = 1  # every launch set gets its own buffer
         for it in range(2):
             fb = srv.submit(SlaterBatchRequest(atoms))
             fs = srv.submit(single)
