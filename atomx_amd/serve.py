@@ -610,6 +610,9 @@ class GpuExecutor:
             f.result()
 
     _OWN_RESULTS = 1 << 17  # result values from which a launch set gets its own buffer
+    # R^k count from which a new resident plan's contraction schedule is tuned
+    # once, on its first request (a few milliseconds, then every later request)
+    _TUNE_MIN_RK = 50_000
 
     def _host_need(self, n_in, end):
         """Pinned staging values a launch set needs: its inputs, and its
@@ -737,6 +740,7 @@ class GpuExecutor:
                 return
         entries = []  # (request index, atom index or -1, plan, P, Q)
         f64 = np.float64
+        tune = []  # new resident plans to tune on this launch set's inputs
         for i, req in enumerate(reqs):
             batch = req.kind == "slater_batch"
             atoms = req.atoms if batch else (req,)
@@ -749,6 +753,9 @@ class GpuExecutor:
                     if plan is None:
                         plan = st.plan = SlaterTensor(a.gr, a.grw, a.ls, a.kmax, a.precision)
                         self.plans_created += 1
+                        # a resident state of a large tensor is tuned on its first request
+                        if st.key is not None and plan.size >= self._TUNE_MIN_RK:
+                            tune.append(plan)
                     P, Q = a.P, a.Q
                     if type(P) is not np.ndarray or P.dtype != f64:
                         P = np.asarray(P, dtype=f64)
@@ -800,6 +807,9 @@ class GpuExecutor:
             out += plan.size
             plans.append(plan)
         dev[:n_in].copy_(host[:n_in], non_blocking=True)
+        for t in tune:  # times every contraction schedule on the staged inputs
+            k = plans.index(t)
+            t.tune_raw(pp[k], qq[k], oo[k], stream=stream)
         run_batch_raw(plans, pp, qq, oo, stream=stream)
         job["results"] = results
         job["n_in"] = n_in

@@ -115,6 +115,15 @@ class SlaterTensor:
              dv.stream_handle(stream), ctypes.byref(ms))
         return self.schedule() + (float(ms.value),)
 
+    def tune_raw(self, P_ptr, Q_ptr, out_ptr, reps=3, stream=None):
+        """``tune`` on raw device addresses (Q_ptr None or 0 for no small
+        component) whose shapes the caller has checked; synchronises the
+        stream."""
+        ms = ctypes.c_float()
+        call("ax_rk_tensor_tune", self._h, P_ptr, Q_ptr or None, out_ptr, int(reps),
+             dv.stream_handle(stream), ctypes.byref(ms))
+        return self.schedule() + (float(ms.value),)
+
     def check_inputs(self, P, Q):
         P = dv.f64(P)
         Q = dv.f64(Q) if Q is not None else None

@@ -208,3 +208,27 @@ def test_batch_request_equals_its_atoms_alone(own_results):
     # the first results are unchanged by every launch set served after them
     for a, res in zip(atoms, first):
         assert np.array_equal(res.values, _alone_slater(a))
+
+
+def test_resident_state_is_tuned_once():
+    """A new resident state of a large enough tensor is tuned on its first
+    request; its results equal, bitwise, an alone run at the tuned schedule,
+    and later requests keep that schedule."""
+    from atomx_amd import device as dv
+    from atomx_amd.serve import SlaterServer
+    from atomx_amd.tensor import SlaterTensor
+    req = _slater_req(120, 9, 4, 7, precision="fp64", state=("tuned", 0))
+    with SlaterServer(max_inflight=2, max_batch=4) as srv:
+        srv.executor._TUNE_MIN_RK = 1
+        first = srv.submit(req).result()
+        st = srv.executor.states[("slater", ("tuned", 0))]
+        sched = st.plan.schedule()
+        again = srv.submit(req).result()
+        assert st.plan.schedule() == sched
+    alone = SlaterTensor(req.gr, req.grw, req.ls, req.kmax, precision="fp64")
+    alone.set_schedule(*sched)
+    ref = dv.host(alone.run(req.P, req.Q))
+    alone.close()
+    print("tuned schedule", sched)
+    assert np.array_equal(first.values, ref)
+    assert np.array_equal(again.values, ref)
