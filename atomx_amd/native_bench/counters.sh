@@ -27,7 +27,7 @@ pass() {  # name, precision, splits, profiler options...
 }
 MFMA="SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_VALU_MFMA_MOPS_I8 SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE"
 for prec in fp64 split; do
-    splits=6,7,8,9,10,12
+    splits=6,7,8,9,10,12,4:8  # order 1 at these K splits, then the balanced order at level 8
     pass "$prec-trace" $prec $splits --kernel-trace --stats &&
     pass "$prec-fetch" $prec $splits --pmc FETCH_SIZE &&
     pass "$prec-write" $prec $splits --pmc WRITE_SIZE &&

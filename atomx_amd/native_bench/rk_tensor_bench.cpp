@@ -6,9 +6,9 @@
 // device-to-device copy of `copy_mib` MiB that calibrates byte counters.
 //
 // usage: rk_tensor_bench WORKLOAD fp64|split ORDER SPLITS STEPS [COPY_MIB]
-//        ORDER in 0..2 (tile-major, xcd-contiguous, xcd-contiguous-split-
-//        fastest) and SPLITS a comma-separated list of K splits, each timed
-//        in turn; or ORDER = -1 to tune.
+//        ORDER in 0..4 (ax_rk_tensor_set_schedule) and SPLITS a
+//        comma-separated list of levels, each timed in turn; an entry O:S
+//        runs order O at level S instead.  ORDER = -1 tunes.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -41,8 +41,16 @@ int main(int argc, char **argv) {
     const char *path = argv[1];
     const int precision = std::string_view(argv[2]) == "split" ? AX_PREC_SPLIT : AX_PREC_FP64;
     const int order = std::atoi(argv[3]);
-    std::vector<int> splits;
+    std::vector<int> splits, orders;
     for (const char *c = argv[4]; *c;) {
+        const char *colon = c;
+        while (*colon && *colon != ',' && *colon != ':') ++colon;
+        if (*colon == ':') {
+            orders.push_back(std::atoi(c));
+            c = colon + 1;
+        } else {
+            orders.push_back(order);
+        }
         splits.push_back(std::atoi(c));
         while (*c && *c != ',') ++c;
         if (*c == ',') ++c;
@@ -99,7 +107,7 @@ int main(int argc, char **argv) {
         if (order < 0) {
             float tms = 0.0f;
             if (ax_rk_tensor_tune(plan, dP, dQ, dout, 3, s, &tms)) return fail("tune");
-        } else if (ax_rk_tensor_set_schedule(plan, order, splits[si])) {
+        } else if (ax_rk_tensor_set_schedule(plan, orders[si], splits[si])) {
             return fail("set_schedule");
         }
         int o = 0, sp = 0;

@@ -80,7 +80,11 @@ def summarise(d: str, prec: str, n_tiles: int):
         t = mean(trace[key]["duration"])
         fb = mean(fetch[key]["FETCH_SIZE"]) * 1024 * f_fetch if key in fetch else float("nan")
         wb = mean(write[key]["WRITE_SIZE"]) * 1024 * f_write if key in write else float("nan")
-        split = grid // 256 // n_tiles if name.startswith("rk_gemm") else None
+        split = None
+        if name.startswith("rk_gemm"):  # static K split, or the balanced order's level
+            groups = grid // 256
+            split = (str(groups // n_tiles) if groups % n_tiles == 0
+                     else f"bal{groups // 64}")
         row = {"kernel": name, "grid": grid, "split": split, "dispatches": len(trace[key]["duration"]),
                "us": t * 1e6, "fetch": fb, "write": wb, "gbs": (fb + wb) / t / 1e9}
         m = mfma.get(key, {})
@@ -124,7 +128,8 @@ def main(argv=None) -> None:
                     "rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of the native driver "
                     "(atomx_amd/native_bench/counters.sh),\n# scaled by the closing 256 MiB copy; "
                     "avg_us from the kernel trace of the same driver.\n"
-                    "# columns: kernel k_split fetch_bytes write_bytes traffic_bytes avg_us\n")
+                    "# columns: kernel schedule (K split of the xcd-contiguous order, or balN: the\n"
+                    "# balanced order at level N) fetch_bytes write_bytes traffic_bytes avg_us\n")
             for k, s, fb, wb, us in table:
                 f.write(f"traffic: {k} {s} {fb:.0f} {wb:.0f} {fb + wb:.0f} {us:.2f}\n")
 

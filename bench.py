@@ -249,17 +249,25 @@ def neon_leg(args):
 TRAFFIC_TABLE = "profiles/c2_hbm_traffic.txt"
 
 
-def hbm_traffic(args, kernel, k_split):
-    """HBM bytes per launch of the contraction kernel at this K split, from
+def hbm_traffic(args, kernel, order, k_split):
+    """HBM bytes per launch of the contraction kernel at this schedule, from
     the calibrated FETCH_SIZE + WRITE_SIZE table measured on the default
-    workload (None for other workloads or splits)."""
+    workload (None for other workloads or schedules the table lacks: it
+    holds the xcd-contiguous order's K splits and the balanced order's
+    levels)."""
     default = (args.nv, args.ng, args.n_orb, args.kmax, args.seed) == (400, 8, 40, 8, 0)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_TABLE)
     if not default or not os.path.exists(path):
         return None
+    if order == "xcd-contiguous":
+        key = str(int(k_split))
+    elif order == "balanced-k-ranges":
+        key = f"bal{int(k_split)}"
+    else:
+        return None
     for line in open(path):
         f = line.split()
-        if len(f) >= 6 and f[0] == "traffic:" and f[1] == kernel and int(f[2]) == int(k_split):
+        if len(f) >= 6 and f[0] == "traffic:" and f[1] == kernel and f[2] == key:
             return int(f[5])
     return None
 
@@ -543,10 +551,11 @@ def main():
     achieved = flops / (contract_ms * 1e-3) / 1e12
     kernel = "rk_gemm_f64_kernel" if prec == "fp64" else f"rk_gemm_split_kernel<{plan.slices}>"
     k_split = modes[prec]["schedule"]["k_split"]
+    order = modes[prec]["schedule"]["order"]
     roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved,
                 "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": hbm_traffic(args, kernel, k_split),
-                "traffic_source": TRAFFIC_TABLE, "k_split": k_split,
+                "traffic": hbm_traffic(args, kernel, order, k_split),
+                "traffic_source": TRAFFIC_TABLE, "order": order, "k_split": k_split,
                 "algorithmic_flops_per_launch": flops, "avg_launch_ms": contract_ms}
 
     ms = modes[prec]["ms_per_step"]
