@@ -888,10 +888,9 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
     constexpr int MAXP = MAXQ * kWgScanWaves;
     constexpr int CS = NG + 1;  // padded cell stride of the transpose region
     __shared__ double tot_o[MAXP], tot_i[MAXP];
-    // transpose regions: only 7- and 9-node cells (loads) and in-scan
-    // slicing (output) go through them; 8-node cells are read and written
-    // cell-per-lane
-    constexpr int TW = (SLICE || NG != 8) ? kWgScanWaves : 1;
+    // transpose regions: only 7- and 9-node cells go through them; 8-node
+    // cells are read and written (or sliced) cell-per-lane
+    constexpr int TW = NG != 8 ? kWgScanWaves : 1;
     __shared__ double tr[TW][2][64 * CS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int64_t local;
@@ -1045,6 +1044,63 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
                 double *yo = g.Y + (int64_t)g.item_out[local] * n + c0;
 #pragma unroll
                 for (int h = 0; h < NG / 2; ++h) *reinterpret_cast<dd2 *>(yo + 2 * h) = y2[h];
+            }
+            continue;
+        }
+        if constexpr (SLICE && NG == 8) {
+            // the lane's cell, sliced in place: eight lanes hold one 64-node
+            // K-tile (their exponent is the tile's), and each slice plane
+            // gets the cell's 8 bytes in one store (the bytes of
+            // ax::slice_value, node by node)
+            typedef double dd2 __attribute__((ext_vector_type(2)));
+            const int64_t c0 = base + (int64_t)lane * NG;
+            double y[NG];
+            if (c0 + NG <= n) {
+                dd2 a2[NG / 2], b2[NG / 2];
+#pragma unroll
+                for (int h = 0; h < NG / 2; ++h) {
+                    a2[h] = *reinterpret_cast<const dd2 *>(rkp + c0 + 2 * h);
+                    b2[h] = *reinterpret_cast<const dd2 *>(rkmp + c0 + 2 * h);
+                }
+#pragma unroll
+                for (int m = 0; m < NG; ++m) {
+                    double po = 0.0, pi = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NG; ++j) {
+                        po += mo[m * NG + j] * go[q][j];
+                        pi += mi[m * NG + j] * gi[q][j];
+                    }
+                    const double ra = (m & 1) ? a2[m >> 1].y : a2[m >> 1].x;
+                    const double rb = (m & 1) ? b2[m >> 1].y : b2[m >> 1].x;
+                    y[m] = rb * (bo + po) + ra * (bi + pi);
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < NG; ++m) y[m] = 0.0;
+            }
+            double mx = 0.0;
+#pragma unroll
+            for (int m = 0; m < NG; ++m) mx = fmax(mx, fabs(y[m]));
+#pragma unroll
+            for (int off = 1; off < 8; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+            if (c0 < g.npad) {  // the same for the tile's eight lanes
+                const int ex = mx > 0.0 ? ilogb(mx) + 1 : 0;
+                const int64_t row = g.item_out[local];
+                double rho[NG];
+#pragma unroll
+                for (int m = 0; m < NG; ++m) rho[m] = ldexp(y[m], 7 - ex);
+                int8_t *dst = g.Ys + row * g.npad + c0;
+                for (int sl = 0; sl < g.slices; ++sl) {
+                    uint64_t word = 0;
+#pragma unroll
+                    for (int m = 0; m < NG; ++m) {
+                        const double c = trunc(rho[m]);
+                        word |= (uint64_t)(uint8_t)static_cast<int8_t>(c) << (8 * m);
+                        rho[m] = (rho[m] - c) * 128.0;
+                    }
+                    *reinterpret_cast<uint64_t *>(dst + (int64_t)sl * g.ys_plane) = word;
+                }
+                if ((lane & 7) == 0) g.Ye[row * (g.npad / ax::kSplitKTile) + c0 / ax::kSplitKTile] = ex;
             }
             continue;
         }
