@@ -259,10 +259,6 @@ __global__ __launch_bounds__(256, 2) void rk_gemm_f64_kernel(const ax::BatchOf<a
     }
 }
 
-constexpr int kReduceThreads = 256;
-constexpr int kReducePer = 4;  // tile elements per thread: 4096 / (4 * 256) blocks per tile
-constexpr int kReduceBlocksPerTile = 64 * 64 / (kReducePer * kReduceThreads);
-
 // Sums the split partials of each tile in split order and writes the packed
 // upper triangle of the segment (tiles of one chunk were written by the
 // contraction).
@@ -277,23 +273,15 @@ __global__ __launch_bounds__(256) void rk_reduce_kernel(const ax::BatchOf<ax::Ge
     const int64_t m = sg.m;
     const double *src = g.partial + (int64_t)tj.first * (BM * BM);
     double *out = g.out + sg.out_off;
-    // kReducePer elements of the tile per thread, their loads interleaved;
-    // each element's slots are summed in slot order
-    const int t0 = blockIdx.y * (kReducePer * kReduceThreads) + threadIdx.x;
-    double acc[kReducePer];
-#pragma unroll
-    for (int u = 0; u < kReducePer; ++u) acc[u] = src[t0 + u * kReduceThreads];
-    for (int sp = 1; sp < tj.count; ++sp) {
-        const double *ps = src + (int64_t)sp * BM * BM + t0;
-#pragma unroll
-        for (int u = 0; u < kReducePer; ++u) acc[u] += ps[u * kReduceThreads];
-    }
-#pragma unroll
-    for (int u = 0; u < kReducePer; ++u) {
-        const int t = t0 + u * kReduceThreads;
-        const int64_t i = (int64_t)wk.ti * BM + t / BM;
-        const int64_t j = (int64_t)wk.tj * BM + t % BM;
-        if (i < m && j < m && j >= i) out[ax::packed_index(i, j, m)] = acc[u];
+    const int per = (BM * BM) / gridDim.y;
+    for (int t = blockIdx.y * per + threadIdx.x; t < (blockIdx.y + 1) * per; t += blockDim.x) {
+        const int li = t / BM, lj = t % BM;
+        const int64_t i = (int64_t)wk.ti * BM + li;
+        const int64_t j = (int64_t)wk.tj * BM + lj;
+        if (i >= m || j >= m || j < i) continue;
+        double s = src[t];
+        for (int sp = 1; sp < tj.count; ++sp) s += src[(int64_t)sp * BM * BM + t];
+        out[i * m - i * (i - 1) / 2 + (j - i)] = s;
     }
 }
 
@@ -414,9 +402,8 @@ ax_status reduce_batch_launch(const PlanHost *const *plans, int n, double *const
     BatchOf<GemmArg> A{};
     const int64_t total = fill_work(A, sub, ns, sub_out, false);
     if (total == 0) return AX_OK;
-    hipLaunchKernelGGL(rk_reduce_kernel,
-                       dim3(static_cast<unsigned>(total), kReduceBlocksPerTile),
-                       dim3(kReduceThreads), 0, stream, A);
+    hipLaunchKernelGGL(rk_reduce_kernel, dim3(static_cast<unsigned>(total), 16), dim3(256), 0,
+                       stream, A);
     AX_CHECK_LAUNCH();
     return AX_OK;
 }
