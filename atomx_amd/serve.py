@@ -809,7 +809,10 @@ class GpuExecutor:
         dev[:n_in].copy_(host[:n_in], non_blocking=True)
         for t in tune:  # times every contraction schedule on the staged inputs
             k = plans.index(t)
-            t.tune_raw(pp[k], qq[k], oo[k], stream=stream)
+            try:
+                t.tune_raw(pp[k], qq[k], oo[k], stream=stream)
+            except (RuntimeError, ValueError, MemoryError):  # tuning is optional: the
+                pass  # plan keeps the schedule it had (ax_rk_tensor_tune restores it)
         run_batch_raw(plans, pp, qq, oo, stream=stream)
         job["results"] = results
         job["n_in"] = n_in
