@@ -3,12 +3,14 @@
 //
 // Work decomposition: a list of segments (one per rank k, per request, ...),
 // each an m x m symmetric-structured output; every upper 64x64 tile of every
-// segment is one work entry, and its K range is cut into `n_split`
-// contiguous chunks (one job = one work-group = one chunk of one tile).
-// A tile of one chunk is written to the packed output by its work-group;
-// the chunks of the others are written to a partial workspace and summed in
-// a fixed order by rk_reduce_kernel, so a result depends only on its own
-// segment's data and the schedule (never on what else is in the batch).
+// segment is one work entry, and its K range is cut into contiguous pieces
+// (one job = one piece of one tile).  A work-group runs a range of jobs: one
+// job for the static and dynamic orders, a packed list of whole K-slice
+// pieces for the balanced order (make_balanced_jobs).  A tile of one piece
+// is written to the packed output by its work-group; the pieces of the
+// others are written to a partial workspace and summed in K order by
+// rk_reduce_kernel, so a result depends only on its own segment's data and
+// the schedule (never on what else is in the batch).
 // (Summing a tile's chunks in the contraction, by the work-group that
 // finishes last, needs an agent-scope release per work-group — an L2
 // write-back — and measured 30-70 us slower per launch than the separate
