@@ -38,25 +38,7 @@ def build(verbose: bool = False) -> str:
     if proc.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + proc.stdout + proc.stderr)
     os.replace(tmp, LIB_PATH)
-    build_native_bench()
     return LIB_PATH
-
-
-NATIVE_BENCH = os.path.join(LIB_DIR, "rk_tensor_bench")
-
-
-def build_native_bench() -> str:
-    """Links the native benchmark driver (hardware-counter runs) against the
-    library, next to it (rpath $ORIGIN)."""
-    src = os.path.join(HERE, "native_bench", "rk_tensor_bench.cpp")
-    tmp = NATIVE_BENCH + ".tmp"
-    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-L", LIB_DIR,
-           "-latomx_slater", "-Wl,-rpath,$ORIGIN", "-o", tmp]
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        raise RuntimeError("hipcc failed on the native benchmark:\n" + proc.stdout + proc.stderr)
-    os.replace(tmp, NATIVE_BENCH)
-    return NATIVE_BENCH
 
 
 if __name__ == "__main__":

@@ -5,6 +5,7 @@
 # profiler run of its own; every run ends with a 256 MiB device copy that
 # calibrates the byte counters.
 #
+#   python -m atomx_amd.native_bench.build        (once, on the build host)
 #   bash atomx_amd/native_bench/counters.sh OUTDIR
 #
 # Summarise with: python -m atomx_amd.native_bench.summarize OUTDIR
@@ -12,6 +13,7 @@ set -o pipefail
 ROOT=$(cd "$(dirname "$0")/../.." && pwd)
 OUT=$(mkdir -p "$1" && cd "$1" && pwd)
 BIN=$ROOT/atomx_amd/_lib/rk_tensor_bench
+[ -x "$BIN" ] || { echo "missing $BIN: run python -m atomx_amd.native_bench.build first"; exit 1; }
 (cd "$ROOT" && python3 -m atomx_amd.native_bench.write_workload "$OUT/w.bin") || exit 1
 cd /tmp && export TMPDIR=/tmp
 pass() {  # name, precision, splits, profiler options...
@@ -22,8 +24,17 @@ pass() {  # name, precision, splits, profiler options...
         "$BIN" "$OUT/w.bin" "$prec" 1 "$splits" 10 256 > "$OUT/$name.log" 2>&1
     local rc=$?
     grep -E "^workload|^copy" "$OUT/$name.log"
-    if [ $rc -ne 0 ] && [ $rc -ne 139 ]; then echo "$name: rc=$rc"; return $rc; fi
-    return 0
+    if [ $rc -eq 0 ]; then return 0; fi
+    # Any other status fails the pass.  The one exception is a SIGSEGV (139)
+    # raised after the driver printed its closing line: the program had
+    # finished and its counter files were written; the fault is the
+    # profiler's own teardown.  A crash before that line is a failure.
+    if [ $rc -eq 139 ] && [ "$(grep -v '^$' "$OUT/$name.log" | tail -1)" = "rk_tensor_bench: done" ]; then
+        echo "$name: rocprofv3 exited 139 after the driver finished (results kept)"
+        return 0
+    fi
+    echo "$name: rc=$rc"
+    return $rc
 }
 MFMA="SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_VALU_MFMA_MOPS_I8 SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE"
 for prec in fp64 split; do

@@ -146,5 +146,7 @@ int main(int argc, char **argv) {
     (void)hipFree(dP);
     (void)hipFree(dQ);
     (void)hipFree(dout);
+    std::printf("rk_tensor_bench: done\n");
+    std::fflush(stdout);
     return 0;
 }

@@ -149,8 +149,8 @@ class Scheduler:
       ready(req) -> bool         False while the request's resident state is in flight
       footprint(req) -> int      device bytes the request holds while in flight
       reserve(req)               pin the request's resident state (if any)
-      resident_bytes() -> int    bytes held by idle resident states
-      evict(nbytes) -> int       drop idle resident states (LRU), bytes freed
+      resident_bytes() -> int    bytes held outside in-flight footprints (idle states, caches)
+      evict(nbytes) -> int       drop idle resident states (LRU) and caches, bytes freed
       batch_key(req)             requests with equal keys may share a launch
       batch_weight(req)          (optional) work units toward max_batch, default 1
       launch(reqs) -> handle     start the work of a batch (asynchronous)
@@ -186,6 +186,8 @@ class Scheduler:
         fut = Future()
         t = _Ticket(req, fut, time.perf_counter(), next(self._seq))
         with self._cv:
+            if self.error is not None:
+                raise RuntimeError(f"scheduler stopped after an error: {self.error!r}")
             if self._stop:
                 raise RuntimeError("scheduler is closed")
             self._arrivals.append(t)
@@ -235,8 +237,10 @@ class Scheduler:
                             if busy or not self._stop:
                                 self._cv.wait(self.poll_s if busy else 0.05)
         except BaseException as exc:  # fail every outstanding request loudly
-            self.error = exc
             with self._cv:
+                # later submits raise instead of queueing work nobody runs
+                self.error = exc
+                self._stop = True
                 rest = list(self._arrivals) + list(self._pending)
                 self._arrivals.clear()
                 self._pending.clear()
@@ -308,7 +312,13 @@ class Scheduler:
             admitted.append(t)
         if not admitted:
             return False
-        handle = self.ex.launch([t.req for t in admitted])
+        try:
+            handle = self.ex.launch([t.req for t in admitted])
+        except BaseException as exc:  # the admitted tickets are no longer queued
+            for t in admitted:
+                if not t.future.done():
+                    t.future.set_exception(exc)
+            raise
         self._inflight.append(_Job(handle, admitted, nbytes))
         self._inflight_bytes += nbytes
         self.launches += 1
@@ -372,6 +382,8 @@ class GpuExecutor:
         # pinned staging per in-flight slot: the uploads of the slot's launch
         # set (reused once the slot's job has finished)
         self._stage = [None] * n_streams
+        # device staging bytes charged to the launch set in flight on each slot
+        self._slot_need = [0] * n_streams
         # host copies into the staging, split across threads (numpy copies
         # run without the interpreter lock)
         self._pack_threads = max(1, min(8, (os.cpu_count() or 1) // 2))
@@ -385,7 +397,8 @@ class GpuExecutor:
 
     # -- footprints ----------------------------------------------------------------
     def _info(self, req):
-        """(shape key, device bytes of a fresh state, R^k count), computed
+        """(shape key, device bytes of a fresh state's plan, R^k count,
+        device staging bytes of the request's inputs and outputs), computed
         once per request (the estimate once per request shape)."""
         info = getattr(req, "_ax_info", None)
         if info is None:
@@ -396,13 +409,14 @@ class GpuExecutor:
                 est = self._estimates.get(sk)
                 if est is None:
                     n_rk = _count_rk(req.ls, int(req.kmax))
-                    est = (self._slater_bytes(req, n_rk), n_rk)
+                    n_in = len(req.ls) * nv * ng * (2 if req.Q is not None else 1)
+                    est = (self._slater_bytes(req), n_rk, 8 * (n_in + n_rk))
                     self._estimates[sk] = est
                     if len(self._estimates) > 4096:
                         self._estimates.pop(next(iter(self._estimates)))
-                info = (self._slater_shape(req), est[0], est[1])
+                info = (self._slater_shape(req), est[0], est[1], est[2])
             else:
-                info = (None, self._fock_bytes(req), 0)
+                info = (None, self._fock_bytes(req), 0, 0)
             req._ax_info = info
         return info
 
@@ -415,7 +429,9 @@ class GpuExecutor:
         return ("slater", _grid_key(req.gr, req.grw), nv, ng, ls, int(req.kmax), req.precision,
                 req.Q is not None)
 
-    def _slater_bytes(self, req, n_rk):
+    def _slater_bytes(self, req):
+        """Device bytes of the request's plan (what a resident state holds);
+        its inputs and outputs live in the slot staging, charged per launch."""
         import ctypes
         nv, ng = np.shape(req.gr)
         ls = np.ascontiguousarray(req.ls, dtype=np.int32)
@@ -424,7 +440,7 @@ class GpuExecutor:
             self._prec[req.precision]))
         if plan < 0:
             raise ValueError("invalid Slater-integral request")
-        return plan + 8 * (2 * ls.size * nv * ng + n_rk)
+        return plan
 
     def _fock_bytes(self, req):
         from .fock_yk import FockYk
@@ -438,16 +454,29 @@ class GpuExecutor:
         return (req.kind, req.state) if req.state is not None else None
 
     def footprint(self, req):
+        """Device bytes a request holds while in flight: its plans (held by
+        the resident state when there is one) and its staged inputs and
+        outputs."""
         if req.kind == "slater_batch":
             states, total = self.states, 0
             for a in req.atoms:
                 st = states.get((a.kind, a.state)) if a.state is not None else None
-                total += st.nbytes if st is not None else self._info(a)[1]
+                info = self._info(a)
+                total += (st.nbytes if st is not None else info[1]) + info[3]
             return total
+        info = self._info(req)
         key = self._key(req)
         if key is not None and key in self.states:
-            return self.states[key].nbytes
-        return self._info(req)[1]
+            return self.states[key].nbytes + info[3]
+        return info[1] + info[3]
+
+    def _staging_need(self, reqs):
+        """Device staging bytes a launch set's requests were charged."""
+        total = 0
+        for r in reqs:
+            for a in (r.atoms if r.kind == "slater_batch" else (r,)):
+                total += self._info(a)[3]
+        return total
 
     def ready(self, req):
         if req.kind == "slater_batch":
@@ -497,10 +526,23 @@ class GpuExecutor:
             self._set_busy(st, True)
         self.states.move_to_end(key)
 
+    def _held_staging(self):
+        """Device staging bytes held by the slots beyond what their in-flight
+        launch sets were charged (all of it for a free slot)."""
+        extra = 0
+        for sid, buf in enumerate(self._stage):
+            if buf is not None:
+                extra += max(0, 8 * buf[1].numel() - self._slot_need[sid])
+        return extra
+
     def resident_bytes(self):
-        return self._idle_bytes
+        """Device bytes held outside the in-flight footprints: idle resident
+        states and the slots' device staging."""
+        return self._idle_bytes + self._held_staging()
 
     def evict(self, nbytes):
+        """Frees at least nbytes if it can: idle resident states (least
+        recently used first), then the device staging of free slots."""
         freed = 0
         for key in list(self.states):
             if freed >= nbytes:
@@ -512,6 +554,13 @@ class GpuExecutor:
             self._idle_bytes -= st.nbytes
             st.close()
             del self.states[key]
+        for sid in list(self._free_streams):
+            if freed >= nbytes:
+                break
+            buf = self._stage[sid]
+            if buf is not None:
+                freed += 8 * buf[1].numel()
+                self._stage[sid] = None
         return freed
 
     def batch_weight(self, req):
@@ -559,6 +608,7 @@ class GpuExecutor:
         stream = self.streams[sid]
         job = {"sid": sid, "reqs": reqs, "states": [], "errors": [None] * len(reqs),
                "keep": [], "event": torch.cuda.Event()}
+        self._slot_need[sid] = self._staging_need(reqs)
         with torch.cuda.stream(stream):
             if reqs[0].kind in ("slater", "slater_batch"):
                 self._launch_slater(reqs, job, stream)
@@ -574,9 +624,16 @@ class GpuExecutor:
         torch, dv = self._torch, self._dv
         buf = self._stage[sid]
         if buf is None or buf[0].numel() < n_in or buf[1].numel() < n:
-            n_in = max(n_in, 1 << 16, buf[0].numel() if buf is not None else 0)
-            n = max(n, 1 << 16, buf[1].numel() if buf is not None else 0)
-            buf = (torch.empty(n_in, dtype=torch.float64, pin_memory=True), dv.empty((n,)))
+            host = buf[0] if buf is not None and buf[0].numel() >= n_in else None
+            if host is None:
+                n_in = max(n_in, 1 << 16, buf[0].numel() if buf is not None else 0)
+                host = torch.empty(n_in, dtype=torch.float64, pin_memory=True)
+            # the device staging grows to exactly what is asked (admission
+            # charged it), never past the largest launch set it served
+            dev = buf[1] if buf is not None and buf[1].numel() >= n else None
+            if dev is None:
+                dev = dv.empty((max(n, buf[1].numel() if buf is not None else 0),))
+            buf = (host, dev)
             self._stage[sid] = buf
         return buf
 
@@ -744,7 +801,7 @@ class GpuExecutor:
         for i, req in enumerate(reqs):
             batch = req.kind == "slater_batch"
             atoms = req.atoms if batch else (req,)
-            got, sts = [], []
+            got, sts, fresh = [], [], []
             try:
                 for j, a in enumerate(atoms):
                     st = self._state_for(a)
@@ -755,7 +812,7 @@ class GpuExecutor:
                         self.plans_created += 1
                         # a resident state of a large tensor is tuned on its first request
                         if st.key is not None and plan.size >= self._TUNE_MIN_RK:
-                            tune.append(plan)
+                            fresh.append(plan)
                     P, Q = a.P, a.Q
                     if type(P) is not np.ndarray or P.dtype != f64:
                         P = np.asarray(P, dtype=f64)
@@ -766,6 +823,8 @@ class GpuExecutor:
                         raise ValueError(f"orbital samples must be {shape}")
                     got.append((i, j if batch else -1, plan, P, Q))
                 entries.extend(got)
+                # only plans whose request made it into this launch set are tuned
+                tune.extend(fresh)
             except Exception as exc:  # the request fails alone
                 job["errors"][i] = exc
                 # every state the scheduler reserved for the request is
@@ -888,6 +947,7 @@ class GpuExecutor:
                 results.append(FockResult({k: v.numpy() for k, v in job["host"].items()}))
             for st in sts:
                 self._release(st)
+        self._slot_need[job["sid"]] = 0
         self._free_streams.append(job["sid"])
         return results
 

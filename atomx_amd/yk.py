@@ -25,11 +25,13 @@ def yk_partial_matrices(n_gauss):
     return _grid.partial_matrices(int(n_gauss))
 
 
-def yk_rows(rho_dev, tables, ks, weighted=False, out=None, stream=None):
+def yk_rows(rho_dev, tables, ks, weighted=False, out=None, stream=None, mo=None, mi=None):
     """Y^k of many densities on the device.
 
     rho_dev : (n_rows, n) FP64 device tensor (w*rho rows when ``weighted``).
     ks      : one rank per row (sequence) or a single int for all rows.
+    mo, mi  : optional (ng, ng) FP64 device tensors replacing the grid's own
+              partial-interval matrices for this call.
     Returns an (n_rows, n) device tensor.
     """
     n_rows, n = rho_dev.shape
@@ -45,16 +47,32 @@ def yk_rows(rho_dev, tables, ks, weighted=False, out=None, stream=None):
     kk = dv.i32(ks)
     call("ax_yk_batch", dv.ptr(rho_dev), dv.ld(rho_dev), None if weighted else dv.ptr(tables.w),
          dv.ptr(tables.rk), dv.ptr(tables.rkm), n, tables.nv, tables.ng, tables.kmax,
-         dv.ptr(tables.mo), dv.ptr(tables.mi), n_rows, dv.ptr(rows), dv.ptr(kk), dv.ptr(rows),
+         dv.ptr(tables.mo if mo is None else mo), dv.ptr(tables.mi if mi is None else mi),
+         n_rows, dv.ptr(rows), dv.ptr(kk), dv.ptr(rows),
          dv.ptr(out), dv.ld(out), dv.stream_handle(stream))
     return out
+
+
+def _caller_matrix(m, own, ng, name):
+    """Device copy of a caller's partial-interval matrix, or None when it is
+    absent or equal to the grid's own (whose device copy is reused)."""
+    if m is None:
+        return None
+    m = np.asarray(m, dtype=np.float64)
+    if m.shape != (ng, ng):
+        raise ValueError(f"{name} must be ({ng}, {ng}), got {m.shape}")
+    if np.array_equal(m, own):
+        return None
+    return dv.f64(m)
 
 
 def yk_flat(rho_flat, r_flat, w_flat, n, k, nv, ng, mo=None, mi=None):
     """Y^k on a flattened Gauss grid (yk.py:87-154), computed on the GPU.
 
-    ``mo``/``mi`` are accepted for signature compatibility; the engine uses
-    its own device copy of the same matrices.
+    ``mo``/``mi`` are the partial-interval matrices of the straddled cell, as
+    in the reference; when omitted (or equal to ``yk_partial_matrices(ng)``)
+    the engine's resident copies are used, otherwise the caller's matrices
+    are uploaded for this call.
     """
     if k < 0 or int(k) != k:
         raise ValueError("multipole rank must be a nonnegative integer")
@@ -62,12 +80,15 @@ def yk_flat(rho_flat, r_flat, w_flat, n, k, nv, ng, mo=None, mi=None):
     if rho.size != n or int(n) != int(nv) * int(ng):
         raise ValueError(f"density has {rho.size} samples, expected n = nv*ng = {n}")
     tables = _grid.get(r_flat, w_flat, nv, ng, int(k))
+    own_mo, own_mi = yk_partial_matrices(int(ng))
+    mo_d = _caller_matrix(mo, own_mo, int(ng), "mo")
+    mi_d = _caller_matrix(mi, own_mi, int(ng), "mi")
     if np.iscomplexobj(rho):
         parts = np.stack([rho.real, rho.imag]).astype(np.float64)
-        y = dv.host(yk_rows(dv.f64(parts), tables, int(k)))
+        y = dv.host(yk_rows(dv.f64(parts), tables, int(k), mo=mo_d, mi=mi_d))
         return y[0] + 1j * y[1]
     rho = np.asarray(rho, dtype=np.result_type(rho, np.float64))
-    y = yk_rows(dv.f64(rho[None, :]), tables, int(k))
+    y = yk_rows(dv.f64(rho[None, :]), tables, int(k), mo=mo_d, mi=mi_d)
     return dv.host(y)[0]
 
 

@@ -232,3 +232,50 @@ def test_resident_state_is_tuned_once():
     print("tuned schedule", sched)
     assert np.array_equal(first.values, ref)
     assert np.array_equal(again.values, ref)
+
+
+def test_bad_new_state_fails_alone_in_a_tuned_launch_set():
+    """A launch set holding a good request and the first request of a new
+    resident state whose orbitals have the wrong shape (with tuning of new
+    states forced on): the bad request fails alone, the good one equals its
+    alone result, and the executor keeps serving afterwards."""
+    import time
+
+    from atomx_amd.serve import GpuExecutor, SlaterServer
+    good = _slater_req(60, 5, 3, 11, precision="fp64", state=("good", 0))
+    bad = _slater_req(64, 6, 3, 12, precision="fp64", state=("bad", 0))
+    bad.P = bad.P[:, :-8]  # wrong number of samples
+    ex = GpuExecutor(n_streams=2)
+    try:
+        ex._TUNE_MIN_RK = 1
+        for r in (good, bad):
+            ex.reserve(r)
+        h = ex.launch([good, bad])
+        while not ex.done(h):
+            time.sleep(1e-3)
+        res = ex.finish(h)
+        assert isinstance(res[1], ValueError)
+        assert np.array_equal(res[0].values, _alone_slater(good))
+        # the same states again: the bad state's plan was never tuned or listed
+        ok = _slater_req(64, 6, 3, 12, precision="fp64", state=("bad", 0))
+        for r in (good, ok):
+            ex.reserve(r)
+        h = ex.launch([good, ok])
+        while not ex.done(h):
+            time.sleep(1e-3)
+        res = ex.finish(h)
+        assert np.array_equal(res[0].values, _alone_slater(good))
+        assert np.array_equal(res[1].values, _alone_slater(ok))
+    finally:
+        ex.close()
+    with SlaterServer(max_inflight=2, max_batch=4) as srv:
+        srv.executor._TUNE_MIN_RK = 1
+        bad2 = _slater_req(64, 6, 3, 13, precision="fp64", state=("bad", 1))
+        bad2.P = bad2.P[:, :-8]
+        f_bad = srv.submit(bad2)
+        f_good = srv.submit(good)
+        with pytest.raises(ValueError):
+            f_bad.result(timeout=60)
+        assert np.array_equal(f_good.result(timeout=60).values, _alone_slater(good))
+        later = srv.submit(_slater_req(60, 5, 3, 14, precision="fp64"))
+        assert later.result(timeout=60).values.size > 0

@@ -63,6 +63,30 @@ def test_yk_flat_matches_oracle(c2_small):
             assert np.max(np.abs(got - ref)) <= 1e-12 * scale
 
 
+def test_yk_flat_uses_caller_partial_matrices(c2_small):
+    """Caller-supplied mo/mi are used as given (the reference computes with
+    them): a midpoint-style rule (mo = 1/2 everywhere, mi = 1 - mo) and the
+    engine's own matrices both agree with the oracle run on the same
+    matrices, and the two results differ from each other."""
+    from atomx_amd import yk
+    gr, grw, P, Q, ls = c2_small
+    nv, ng = gr.shape
+    r, w = gr.ravel(), grw.ravel()
+    rho = P[2] * P[4] + Q[2] * Q[4]
+    own = O.yk_partial_matrices(ng)
+    half = (np.full((ng, ng), 0.5), np.full((ng, ng), 0.5))
+    got = {}
+    for name, (mo, mi) in (("own", own), ("half", half)):
+        for k in (0, 3):
+            g = yk.yk_flat(rho, r, w, r.size, k, nv, ng, mo, mi)[ng:]
+            ref = O.yk_flat(rho, r, w, r.size, k, nv, ng, mo, mi)[ng:]
+            assert np.max(np.abs(g - ref)) <= 1e-12 * np.abs(ref).max(), (name, k)
+            got[(name, k)] = g
+    assert np.max(np.abs(got[("own", 0)] - got[("half", 0)])) > 1e-6
+    with pytest.raises(ValueError):
+        yk.yk_flat(rho, r, w, r.size, 0, nv, ng, np.eye(ng + 1), own[1])
+
+
 @pytest.mark.parametrize("ng,nv", [(5, 40), (12, 30), (8, 1500)])
 def test_yk_generic_and_large_grids(ng, nv):
     """Grids served by the LDS-staged scan (n_gauss outside 7..9, nv > 1024)."""

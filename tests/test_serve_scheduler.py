@@ -192,3 +192,21 @@ def test_threaded_packing_equals_concatenate():
             ex._pack([np.ones(4)], np.empty(5))
     finally:
         ex._pool.shutdown()
+
+
+def test_scheduler_failure_rejects_later_submits():
+    """An exception that escapes the executor ends the scheduler thread:
+    every outstanding future fails with it and later submits raise instead
+    of queueing work that would never run."""
+
+    class Broken(FakeExecutor):
+        def launch(self, reqs):
+            raise KeyError("executor bug")
+
+    sch = Scheduler(Broken(), budget_bytes=10**6, max_inflight=2)
+    f = sch.submit(Req(0, 10))
+    with pytest.raises(KeyError):
+        f.result(timeout=10)
+    sch._thread.join(timeout=10)
+    with pytest.raises(RuntimeError, match="stopped after an error"):
+        sch.submit(Req(1, 10))
