@@ -46,6 +46,17 @@ def _alone_slater(req):
     return out
 
 
+def _alone_at(req, schedule):
+    """The request alone on a plan set to a given contraction schedule."""
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    plan = SlaterTensor(req.gr, req.grw, req.ls, req.kmax, precision=req.precision)
+    plan.set_schedule(*schedule)
+    out = dv.host(plan.run(req.P, req.Q))
+    plan.close()
+    return out
+
+
 def _alone_fock(req):
     from atomx_amd import device as dv
     from atomx_amd.fock_yk import FockYk
@@ -255,7 +266,9 @@ def test_bad_new_state_fails_alone_in_a_tuned_launch_set():
             time.sleep(1e-3)
         res = ex.finish(h)
         assert isinstance(res[1], ValueError)
-        assert np.array_equal(res[0].values, _alone_slater(good))
+        # the good state was tuned on this launch set: compare at its schedule
+        sched = ex.states[("slater", ("good", 0))].plan.schedule()
+        assert np.array_equal(res[0].values, _alone_at(good, sched))
         # the same states again: the bad state's plan was never tuned or listed
         ok = _slater_req(64, 6, 3, 12, precision="fp64", state=("bad", 0))
         for r in (good, ok):
@@ -264,8 +277,9 @@ def test_bad_new_state_fails_alone_in_a_tuned_launch_set():
         while not ex.done(h):
             time.sleep(1e-3)
         res = ex.finish(h)
-        assert np.array_equal(res[0].values, _alone_slater(good))
-        assert np.array_equal(res[1].values, _alone_slater(ok))
+        sched_ok = ex.states[("slater", ("bad", 0))].plan.schedule()
+        assert np.array_equal(res[0].values, _alone_at(good, sched))
+        assert np.array_equal(res[1].values, _alone_at(ok, sched_ok))
     finally:
         ex.close()
     with SlaterServer(max_inflight=2, max_batch=4) as srv:
@@ -276,6 +290,8 @@ def test_bad_new_state_fails_alone_in_a_tuned_launch_set():
         f_good = srv.submit(good)
         with pytest.raises(ValueError):
             f_bad.result(timeout=60)
-        assert np.array_equal(f_good.result(timeout=60).values, _alone_slater(good))
+        got = f_good.result(timeout=60).values
+        sched = srv.executor.states[("slater", ("good", 0))].plan.schedule()
+        assert np.array_equal(got, _alone_at(good, sched))
         later = srv.submit(_slater_req(60, 5, 3, 14, precision="fp64"))
         assert later.result(timeout=60).values.size > 0
