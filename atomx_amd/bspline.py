@@ -31,38 +31,46 @@ __all__ = [
 ]
 
 
+def _check_knot_request(ns, ks, r_min, r_max):
+    """Shared argument check of the exponential knot builders: at least two
+    cells and a positive, increasing radial range."""
+    if ks < 1 or ns - ks + 1 < 2:
+        raise ValueError(f"exponential knots need order ks >= 1 and ns >= ks + 1 "
+                         f"splines (at least two cells); got ns={ns}, ks={ks}")
+    if not (0 < r_min < r_max):
+        raise ValueError(f"exponential knots need 0 < r_min < r_max; got r_min={r_min}, "
+                         f"r_max={r_max}")
+
+
 def make_knots(ns: int, ks: int, r_min: float, r_max: float, *, mode: str = "exponential"):
-    """Pure exponential knot sequence (knots.py:10-17)."""
-    if ns < ks + 1:
-        raise ValueError(f"Need ns >= ks + 1, got ns={ns}, ks={ks}")
-    if r_min <= 0 or r_max <= r_min:
-        raise ValueError(f"Need 0 < r_min < r_max, got {r_min}, {r_max}")
+    """Exponential knot sequence of ns splines of order ks on [0, r_max] with
+    the first breakpoint at r_min (knots.py:10-17; ``mode`` is accepted for
+    the reference's signature, only the exponential grid exists)."""
+    _check_knot_request(ns, ks, r_min, r_max)
     return make_exponential_knots(ns, ks, r_min, r_max)
 
 
 def make_exponential_knots(ns: int, ks: int, r_min: float, r_max: float, *, power: float = 1.0):
-    """Exponential breakpoints r_min*he^i; ks-fold zero and r_max ends (knots.py:20-45).
+    """Knots [0]*ks, breakpoints r_min .. r_max, [r_max]*ks (knots.py:20-45).
 
-    ``power == 1`` uses the multiplicative recurrence of the reference so that
-    the breakpoints are bit-identical to it.
+    The nv = ns - ks + 1 breakpoints are r_min * exp(log(r_max / r_min) * u^power)
+    at nv equally spaced u in [0, 1].  For power 1 they are formed as the
+    running product r_min * h * h * ... (h = (r_max / r_min)^(1/(nv - 1)),
+    one multiplication per breakpoint, left to right), which is the rounding
+    the reference's recurrence produces, so the knots agree bit for bit.
     """
-    if (not np.isscalar(power) or np.iscomplexobj(power)
-            or not np.isfinite(power) or power <= 0):
-        raise ValueError("positive finite real grid power required")
+    if not (np.isscalar(power) and np.isrealobj(power) and np.isfinite(power) and power > 0):
+        raise ValueError(f"grid power must be a positive finite real number; got {power!r}")
+    _check_knot_request(ns, ks, r_min, r_max)
     nv = ns - ks + 1
-    if nv < 2 or ks < 1 or not 0 < r_min < r_max:
-        raise ValueError("valid spline count/order and 0 < r_min < r_max required")
-    he = np.exp(np.log(r_max / r_min) / (nv - 1))
-    t = np.zeros(ns + ks)
-    t[ks] = r_min
     if power == 1.0:
-        for i in range(ks + 1, ns + 1):
-            t[i] = t[i - 1] * he
+        h = np.exp(np.log(r_max / r_min) / (nv - 1))
+        factors = np.full(nv - 1, h)
+        inner = np.cumprod(np.concatenate(([float(r_min)], factors)))[:-1]
     else:
-        u = np.linspace(0, 1, nv)
-        t[ks:ns + 1] = r_min * np.exp(np.log(r_max / r_min) * u ** power)
-    t[ns:ns + ks] = r_max
-    return t
+        u = np.linspace(0.0, 1.0, nv)
+        inner = (r_min * np.exp(np.log(r_max / r_min) * u ** power))[:-1]
+    return np.concatenate((np.zeros(ks), inner, np.full(ks, float(r_max))))
 
 
 def make_physics_knots(ks: int, Z: int, ns: int = 113, r_max: float | None = None,
@@ -201,15 +209,14 @@ def evaluate_bsplines_on_grid(t, ns, ks, gr, nderiv=1):
 
 
 def _build_knot_sequence(t_master, ks_master, ns_master, k_target):
-    """Same interior knots, boundary multiplicity k_target (basis.py:348-361)."""
-    nv = ns_master - ks_master + 1
-    n_target = nv + k_target - 1
+    """Knots of order k_target on the breakpoints of t_master (basis.py:348-361):
+    the interior knots are shared, each end knot repeated k_target times.
+    Returns (knots, number of splines)."""
+    t_master = np.asarray(t_master, dtype=float)
     interior = t_master[ks_master:ns_master]
-    t_new = np.zeros(n_target + k_target)
-    t_new[:k_target] = t_master[0]
-    t_new[k_target:k_target + len(interior)] = interior
-    t_new[n_target:] = t_master[ns_master]
-    return t_new, n_target
+    knots = np.concatenate((np.full(k_target, t_master[0]), interior,
+                            np.full(k_target, t_master[ns_master])))
+    return knots, interior.size + k_target
 
 
 def prepare_bspline_data(t, ns, ks, n_gauss=None, kinetic_balance=False):

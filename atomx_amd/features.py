@@ -1,28 +1,46 @@
 """Auxiliary-node form of the cell Coulomb quadrature and the pair-density
 matrix action built on it.
 
-``CellCoulombFeatures`` / ``PolynomialCellCoulombFeatures`` restate
-numerics/bspline/coulomb_features.py:64-524 and ``CellPairContraction``
-restates numerics/bspline/pair_contraction.py:18-148, with the same names,
-arguments, attributes, identities and exceptions.
+Drop-in names: ``CellCoulombFeatures`` and ``PolynomialCellCoulombFeatures``
+(numerics/bspline/coulomb_features.py:64-524), ``CellPairContraction``
+(numerics/bspline/pair_contraction.py:18-148).  Constructor arguments,
+public methods, the public array attributes and the identity keys that
+callers read are the reference's; the node set, the edge set and their
+construction are this module's own.
 
-The cell algorithm's R^k (outer Gauss nodes, triangular sub-quadrature of
-order ks + k//2 inside a cell, cross-cell moments) is written as a sparse
-symmetric operator K_k on auxiliary nodes: every spinor component is
-sampled at the outer nodes, at the sub-nodes of each admitted order and at
-2 ks - 1 interpolation nodes per cell, and R^k(ab;cd) = rho_ab . K_k
-rho_cd with rho = the c-product of samples.  K_k = cross-cell part
-(rising/falling moments, prefix/suffix over cells) + same-cell edges
-between an outer node and the nodes below it in the cell.  For k >= 2 the
-sub-nodes of order ks + k//2 are reached by exact polynomial interpolation
-from the interpolation nodes (a pair density is a polynomial of degree
-2 ks - 2 in each cell).
+The operator.  The cell algorithm's R^k(ab;cd) (convol.py: cross-cell
+products of r^k and r^-(k+1) moments on the Gauss nodes, plus per cell the
+double integral over r_< below the Gauss node r_>, done with a [0,1]
+Gauss-Legendre sub-rule of order (component order, or the larger order for
+mixed components) + floor(k/2)) is a symmetric bilinear form in the pair
+densities.  Inside a cell every spinor component is a polynomial of degree
+order - 1, so a pair density is a polynomial of degree <= 2*top - 2 (top =
+the larger order).  Its values at m = 2*top - 1 distinct points of the cell
+determine it, and its value at any sub-rule point is an exact barycentric
+interpolation of those values.  Each cell therefore carries, per component,
 
-Geometry (node radii, B-spline tables at the nodes, edge lists, per-rank
-weights) is host FP64 set-up, once per grid and rank, like the B-spline
+  * the workspace's ng Gauss nodes ("outer" nodes; moments and the r_> end
+    of the same-cell rule), and
+  * m Chebyshev points of the first kind ("shape" nodes), from which every
+    sub-rule of every rank is reached by interpolation;
+
+and R^k(ab;cd) = rho_ab . K_k rho_cd with rho the c-product of samples and
+K_k = cross-cell part (rising / falling moments on the outer nodes, prefix
+and suffix over cells) + same-cell edges outer node -> shape node whose
+weight folds the rank's sub-rule and the interpolation together.  The edge
+set does not depend on the rank (only its weights do).
+
+``PolynomialCellCoulombFeatures`` keeps only the shape nodes: the outer-node
+values are themselves interpolations of the shape values, so the moments
+and the same-cell edges fold into one dense symmetric 2m x 2m block per
+cell (upper triangle with the diagonal halved, since an edge acts in both
+directions).
+
+Geometry (radii, B-spline values at the nodes, edge lists, per-rank
+weights) is host FP64 set-up once per grid and rank, like the B-spline
 tables.  Sampling, the action on many densities at once, integrals and the
-pair contraction run on the GPU (``ax_feat_*``, ``ax_cpc_*``).  Host
-arrays in give host arrays out; device tensors in give device tensors out.
+pair contraction run on the GPU (``ax_feat_*``, ``ax_cpc_*``).  Host arrays
+in give host arrays out; device tensors in give device tensors out.
 """
 
 from __future__ import annotations
@@ -30,7 +48,7 @@ from __future__ import annotations
 import ctypes
 import hashlib
 from collections import OrderedDict
-from types import MappingProxyType
+from types import MappingProxyType, SimpleNamespace
 
 import numpy as np
 
@@ -39,10 +57,11 @@ from ._lib import call, load
 from .bspline import evaluate_bsplines_on_grid
 
 __all__ = ["CellCoulombFeatures", "PolynomialCellCoulombFeatures", "CellPairContraction",
-           "cardinal_values"]
+           "cardinal_values", "chebyshev_points01"]
 
 _MiB = 1024 ** 2
 _dp = ctypes.POINTER(ctypes.c_double)
+_COMBOS = ((0, 0), (0, 1), (1, 0), (1, 1))  # (outer-node component, shape-node component)
 
 
 def _frozen(a):
@@ -51,14 +70,14 @@ def _frozen(a):
     return np.frombuffer(a.tobytes(), dtype=a.dtype).reshape(a.shape)
 
 
-def _positive_int(label, value):
+def _require_count(name, value):
     if isinstance(value, bool) or not isinstance(value, (int, np.integer)) or value < 1:
-        raise ValueError(f"positive integer {label} required")
+        raise ValueError(f"{name} must be a positive integer, got {value!r}")
 
 
-def _rank(rank):
+def _multipole(rank):
     if isinstance(rank, bool) or not isinstance(rank, (int, np.integer)) or rank < 0:
-        raise ValueError("nonnegative integer multipole required")
+        raise ValueError(f"multipole rank must be a nonnegative integer, got {rank!r}")
     return int(rank)
 
 
@@ -69,202 +88,222 @@ def _gl01(n):
     return x, w
 
 
-def cardinal_values(points, nodes):
-    """Lagrange cardinal polynomials of ``nodes`` evaluated at ``points``
-    (shape points.shape + (len(nodes),)), in the second barycentric form
-    (Berrut & Trefethen, SIAM Review 46 (2004) 501); a point on a node gives
-    that node's unit vector."""
+def chebyshev_points01(m):
+    """m Chebyshev points of the first kind on [0, 1], increasing, and their
+    barycentric weights (-1)^j sin(theta_j) (Berrut & Trefethen, SIAM Review
+    46 (2004) 501, eq. 5.3; a common factor cancels in the formula)."""
+    theta = (2.0 * np.arange(m) + 1.0) * np.pi / (2.0 * m)
+    x = 0.5 * (1.0 - np.cos(theta))
+    lam = np.where(np.arange(m) % 2 == 0, 1.0, -1.0) * np.sin(theta)
+    return x, lam
+
+
+def cardinal_values(points, nodes, weights=None):
+    """Lagrange cardinal functions of ``nodes`` at ``points`` (shape
+    points.shape + (len(nodes),)) in the second barycentric form; ``weights``
+    are the nodes' barycentric weights (computed from the node gaps when
+    omitted).  A point on a node gives that node's unit vector."""
     nodes = np.asarray(nodes, dtype=float)
-    gap = nodes[:, None] - nodes[None, :]
-    np.fill_diagonal(gap, 1.0)
-    lam = 1.0 / np.prod(gap, axis=1)
+    if weights is None:
+        gaps = nodes[:, None] - nodes[None, :]
+        np.fill_diagonal(gaps, 1.0)
+        weights = 1.0 / np.prod(gaps, axis=1)
     d = np.asarray(points, dtype=float)[..., None] - nodes
-    hit = d == 0
-    q = lam / np.where(hit, 1.0, d)
-    q = q / q.sum(axis=-1, keepdims=True)
-    return np.where(hit.any(axis=-1, keepdims=True), hit.astype(float), q)
+    on_node = d == 0.0
+    terms = np.asarray(weights, dtype=float) / np.where(on_node, 1.0, d)
+    terms /= terms.sum(axis=-1, keepdims=True)
+    return np.where(on_node.any(axis=-1, keepdims=True), on_node.astype(float), terms)
+
+
+def _cell_grid(workspace):
+    """The cells, Gauss nodes and both components' spline geometry of a
+    ``bsp_data`` workspace, checked."""
+    gr = np.array(workspace["gr"], dtype=float)
+    grw = np.array(workspace["grw"], dtype=float)
+    nv, ns, ks = int(workspace["nv"]), int(workspace["ns"]), int(workspace["ks"])
+    orders = (int(workspace.get("ksp", ks)), int(workspace.get("ksq", ks)))
+    sizes = (int(workspace.get("nsp", ns)), int(workspace.get("nsq", ns)))
+    knots = (np.array(workspace.get("tp", workspace["t"]), dtype=float),
+             np.array(workspace["t"], dtype=float))
+    ok = (gr.ndim == 2 and gr.shape[0] == nv and grw.shape == gr.shape and min(orders) >= 1
+          and bool(np.isfinite(gr).all() and np.isfinite(grw).all())
+          and bool((gr > 0).all() and (grw > 0).all()))
+    if not ok:
+        raise ValueError("cell quadrature nodes and weights must be finite and positive, "
+                         "one row of Gauss nodes per cell")
+    edges = [t[o - 1:o + nv] for t, o in zip(knots, orders)]
+    lo = edges[0][:-1] if edges[0].size == nv + 1 else None
+    shared = (all(e.size == nv + 1 and bool((np.diff(e) > 0).all()) for e in edges)
+              and np.array_equal(edges[0], edges[1]))
+    if not shared or not bool(((gr > lo[:, None]) & (gr < edges[0][1:, None])).all()):
+        raise ValueError("large and small components must share strictly increasing cell "
+                         "boundaries with every Gauss node inside its cell")
+    return SimpleNamespace(gr=gr, grw=grw, nv=nv, ng=gr.shape[1], ns=ns, orders=orders,
+                           sizes=sizes, knots=knots, lo=lo, width=np.diff(edges[0]),
+                           top=max(orders), bsp=(workspace["bsp_p"],
+                                                 workspace.get("bsp_q", workspace["bsp_p"])))
+
+
+def _digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        a = np.ascontiguousarray(a)
+        h.update(f"{a.dtype.str}{a.shape}".encode())
+        h.update(a.tobytes())
+    return h.hexdigest()
 
 
 def _s():
     return dv.stream_handle()
 
 
+def _device_rank_bytes(n_nodes, n_edges, n_moment):
+    """Device bytes of one rank's tables: rising / falling moments, the CSR
+    of both edge directions (row pointers int64, sources int32, weights) and
+    the directed list (sources, destinations int32, weights)."""
+    return 16 * n_moment + 8 * (n_nodes + 1) + 2 * n_edges * (4 + 8 + 4 + 4 + 8)
+
+
 class CellCoulombFeatures:
-    """Exact sparse radial action on the native B-spline quadrature samples
-    (coulomb_features.py:64-337).
+    """The cell algorithm's Coulomb operator on auxiliary nodes (Gauss nodes
+    plus Chebyshev shape nodes per cell and component; see the module
+    docstring).
 
     ``dot(sample(a) * sample(b), apply_density(sample(c) * sample(d), k))``
     is the cell algorithm's R^k(ab;cd) with all four component terms;
-    complex values use c-products.  ``apply_density`` returns a weighted
-    action (not a pointwise potential)."""
+    complex values use c-products (no conjugation).  ``apply_density``
+    returns a weighted action, not a pointwise potential.  Large-component
+    nodes come first: node indices below ``outer_indices[1, 0, 0]`` belong
+    to P."""
 
     def __init__(self, workspace, *, memory_bytes=128 * _MiB, rank_cache_size=8):
-        _positive_int("memory_bytes", memory_bytes)
-        _positive_int("rank_cache_size", rank_cache_size)
-        gr = np.asarray(workspace["gr"], dtype=float)
-        grw = np.asarray(workspace["grw"], dtype=float)
-        nv, ns, ks = int(workspace["nv"]), int(workspace["ns"]), int(workspace["ks"])
-        orders = (int(workspace.get("ksp", ks)), int(workspace.get("ksq", ks)))
-        sizes = (int(workspace.get("nsp", ns)), int(workspace.get("nsq", ns)))
-        knots = (np.asarray(workspace.get("tp", workspace["t"]), dtype=float),
-                 np.asarray(workspace["t"], dtype=float))
-        if (gr.ndim != 2 or gr.shape[0] != nv or grw.shape != gr.shape
-                or not np.all(np.isfinite(gr)) or not np.all(np.isfinite(grw))
-                or np.any(gr <= 0) or np.any(grw <= 0) or min(orders) < 1):
-            raise ValueError("finite positive native cell quadrature required")
-        breaks = [t[k - 1:k + nv] for t, k in zip(knots, orders)]
-        if (any(b.size != nv + 1 or np.any(np.diff(b) <= 0) for b in breaks)
-                or not np.array_equal(breaks[0], breaks[1])
-                or np.any(gr <= breaks[0][:-1, None]) or np.any(gr >= breaks[0][1:, None])):
-            raise ValueError("P/Q must have identical ordered cell boundaries")
-        ng = gr.shape[1]
-        top = max(orders)
-        m = 2 * top - 1                                   # interpolation nodes per cell
-        subs = [tuple(sorted({o, top})) for o in orders]  # sub-rule orders per component
-        widths = tuple(ng * (1 + sum(s)) + m for s in subs)
-        n_nodes = nv * sum(widths)
-        legacy = nv * ng * (sum(orders) + 2 * top)
-        n_edges = legacy + 4 * nv * ng * m
-        # the reference's payload bound (coulomb_features.py:128-135)
-        estimate = (8 * nv * sum(w * k for w, k in zip(widths, orders)) + 8 * n_nodes
-                    + 32 * nv * ng + 32 * n_edges
-                    + rank_cache_size * 8 * (2 * nv * ng + n_edges) + 8 * (nv + 1 + m))
+        _require_count("memory_bytes", memory_bytes)
+        _require_count("rank_cache_size", rank_cache_size)
+        g = _cell_grid(workspace)
+        nv, ng, top = g.nv, g.ng, g.top
+        m = 2 * top - 1
+        width = ng + m                                     # nodes per cell and component
+        n_nodes = 2 * nv * width
+        n_edges = nv * len(_COMBOS) * ng * m
+        held = (8 * nv * width * sum(g.orders)             # B-spline values at the nodes
+                + 8 * n_nodes + 16 * nv * ng + 16 * n_edges  # radii, outer and edge indices
+                + rank_cache_size * 8 * (2 * nv * ng + n_edges))
+        device = (8 * nv * width * sum(g.orders) + 4 * n_nodes + 8 * nv * ng
+                  + rank_cache_size * _device_rank_bytes(n_nodes, n_edges, nv * ng))
+        estimate = held + device
         if estimate > memory_bytes:
-            raise MemoryError(
-                f"complete auxiliary Coulomb payload needs {estimate} bytes; cap={memory_bytes}")
+            raise MemoryError(f"complete auxiliary-node geometry would hold {estimate} bytes "
+                              f"(host and device), above memory_bytes={memory_bytes}")
 
-        lo, width_cell = breaks[0][:-1], np.diff(breaks[0])
-        span = gr - lo[:, None]
-        dens_x = (np.polynomial.legendre.leggauss(m)[0] + 1.0) / 2.0
-        outer = np.empty((2, nv, ng), dtype=np.int64)
-        sub_nodes, dens_nodes, radii, tables = [{}, {}], [], [], []
-        base = 0
+        xs, lam = chebyshev_points01(m)
+        shape_r = g.lo[:, None] + g.width[:, None] * xs                  # (nv, m)
+        cell_r = np.concatenate((g.gr, shape_r), axis=1)                 # (nv, width)
+        tables = []
         for c in range(2):
-            R = np.empty((nv, widths[c]))
-            first = base + np.arange(nv)[:, None] * widths[c]      # (nv, 1) first node per cell
-            R[:, :ng] = gr
-            outer[c] = first + np.arange(ng)
-            col = ng
-            for q in subs[c]:
-                x, w = _gl01(q)
-                rad = lo[:, None, None] + span[:, :, None] * x     # (nv, ng, q)
-                R[:, col:col + ng * q] = rad.reshape(nv, ng * q)
-                idx = first[:, :, None] + col + np.arange(ng * q).reshape(ng, q)
-                sub_nodes[c][q] = (idx, span[:, :, None] * w, rad)
-                col += ng * q
-            R[:, col:] = lo[:, None] + width_cell[:, None] * dens_x
-            dens_nodes.append(first + col + np.arange(m))
-            tab = evaluate_bsplines_on_grid(knots[c], sizes[c], orders[c], R, nderiv=1)[..., 0]
-            native = np.asarray(workspace["bsp_p"] if c == 0
-                                else workspace.get("bsp_q", workspace["bsp_p"]))
-            if native.shape != (nv, ng, orders[c]) or np.max(abs(tab[:, :ng] - native)) > 1e-11:
-                raise ValueError("native basis tables do not match declared spline geometry")
-            tab[:, :ng] = native  # the outer nodes use the workspace's own tables
+            tab = evaluate_bsplines_on_grid(g.knots[c], g.sizes[c], g.orders[c], cell_r,
+                                            nderiv=1)[..., 0]
+            own = np.asarray(g.bsp[c])
+            if own.shape != (nv, ng, g.orders[c]) or \
+                    not bool(np.abs(tab[:, :ng] - own).max() <= 1e-11):
+                raise ValueError("workspace basis tables (bsp_p / bsp_q) disagree with the "
+                                 "knots and orders it declares")
+            tab[:, :ng] = own          # Gauss-node operands exactly as the workspace has them
             tables.append(_frozen(tab))
-            radii.append(R.ravel())
-            base += nv * widths[c]
 
-        left, right, e_base, e_ratio = [], [], [], []
-        for a in range(2):          # outer node's component
-            for b in range(2):      # lower node's component
-                q = orders[a] if a == b else top
-                idx, sw, rad = sub_nodes[b][q]
-                left.append(np.broadcast_to(outer[a][:, :, None], idx.shape).ravel())
-                right.append(idx.ravel())
-                e_base.append((grw[:, :, None] * sw / gr[:, :, None]).ravel())
-                e_ratio.append((rad / gr[:, :, None]).ravel())
-        for a in range(2):
-            for b in range(2):
-                shape = (nv, ng, m)
-                left.append(np.broadcast_to(outer[a][:, :, None], shape).ravel())
-                right.append(np.broadcast_to(dens_nodes[b][:, None, :], shape).ravel())
-                e_base.append(np.zeros(nv * ng * m))
-                e_ratio.append(np.zeros(nv * ng * m))
+        cell0 = np.arange(nv)[:, None] * width                           # (nv, 1)
+        base = (0, nv * width)
+        outer = np.stack([b + cell0 + np.arange(ng) for b in base])     # (2, nv, ng)
+        shape = np.stack([b + cell0 + ng + np.arange(m) for b in base])  # (2, nv, m)
+        left = np.empty((nv, len(_COMBOS), ng, m), dtype=np.int64)
+        right = np.empty_like(left)
+        for e, (a, b) in enumerate(_COMBOS):
+            left[:, e] = outer[a][:, :, None]
+            right[:, e] = shape[b][:, None, :]
 
-        self._tables = tuple(tables)
-        self._orders, self._sizes, self._ns = orders, sizes, ns
-        self._nv, self._ng = nv, ng
-        self._gr, self._weights = _frozen(gr), _frozen(grw)
-        self._breaks = _frozen(breaks[0])
-        self._dens_x = _frozen(dens_x)
-        self._legacy_edges = legacy
+        self.tables = tuple(tables)
+        self.coefficient_sizes = g.sizes
+        self._grid = g
+        self._ns, self._nv, self._ng = g.ns, nv, ng
+        self._orders = g.orders
+        self._shape_x, self._shape_lam = xs, lam
+        self._node_bases = base
         self._memory_bytes = int(memory_bytes)
-        self.radii, self.outer_indices = _frozen(np.concatenate(radii)), _frozen(outer)
-        self.edge_left, self.edge_right = (_frozen(np.concatenate(left)),
-                                           _frozen(np.concatenate(right)))
-        self.edge_base, self.edge_ratio = (_frozen(np.concatenate(e_base)),
-                                           _frozen(np.concatenate(e_ratio)))
+        self._estimate = estimate
+        self.radii = _frozen(np.concatenate((cell_r.ravel(), cell_r.ravel())))
+        self.outer_indices = _frozen(outer)
+        self.edge_left, self.edge_right = _frozen(left.ravel()), _frozen(right.ravel())
         self._rank_cache, self._rank_capacity = OrderedDict(), int(rank_cache_size)
-        self._node_bases = (0, nv * widths[0])
-        self._widths = widths
         self._dev = None
         self._dev_rank = OrderedDict()
-        h = hashlib.sha256()
-        for value in (gr, grw, *knots, *tables):
-            h.update(repr((value.shape, value.dtype.str)).encode())
-            h.update(np.ascontiguousarray(value).tobytes())
         self.identity = MappingProxyType(dict(
-            representation="native cell Coulomb auxiliary nodes",
-            geometry_sha256=h.hexdigest(),
-            coefficient_sizes=sizes,
-            component_orders=orders,
+            representation="cell Coulomb operator on Gauss and Chebyshev shape nodes",
+            geometry_sha256=_digest(g.gr, g.grw, *g.knots, *tables),
+            coefficient_sizes=g.sizes,
+            component_orders=g.orders,
             auxiliary_nodes=n_nodes,
             same_cell_edges=n_edges,
             quadrature_nodes=(nv, ng),
             payload_estimate_bytes=estimate,
             rank_cache_capacity=rank_cache_size,
-            inner_quadrature="native order + floor(rank/2) for nonnegative ranks",
-            contraction="weighted bilinear c-product; no new quadrature or screening",
+            inner_quadrature=("cell sub-rule of order (component order, larger order when "
+                              "mixed) + rank//2, interpolated exactly from the shape nodes"),
+            contraction="c-product samples; rho_ab . K_k rho_cd is the cell R^k",
         ))
 
     # -- host geometry per rank ------------------------------------------------
     def rank_arrays(self, rank):
         """(rising, falling, edge weights) of one multipole, read-only and
-        cached (LRU of rank_cache_size ranks)."""
-        rank = _rank(rank)
-        if rank not in self._rank_cache:
-            with np.errstate(over="raise", invalid="raise", divide="raise"):
-                edges = self._edge_weights(rank)
-                rising = self._weights * self._gr ** rank
-                falling = self._weights * self._gr ** (-rank - 1)
-            self._rank_cache[rank] = tuple(_frozen(x) for x in (rising, falling, edges))
+        cached (LRU of rank_cache_size ranks); rising / falling are per cell
+        and outer node, the edge weights follow ``edge_left`` /
+        ``edge_right``."""
+        rank = _multipole(rank)
+        got = self._rank_cache.get(rank)
+        if got is None:
+            got = tuple(_frozen(x) for x in self._rank_tables(rank))
+            self._rank_cache[rank] = got
             while len(self._rank_cache) > self._rank_capacity:
                 self._rank_cache.popitem(last=False)
         else:
             self._rank_cache.move_to_end(rank)
-        return self._rank_cache[rank]
+        return got
 
-    def _edge_weights(self, rank):
-        """Same-cell edge weights of one rank: the native sub-rule edges for
-        k < 2; for k >= 2 the rule of order ks + k//2 expressed on the
-        interpolation nodes (coulomb_features.py:297-330)."""
-        if rank < 2:
-            return self.edge_base * self.edge_ratio ** rank
-        nv, ng = self._gr.shape
-        m = self._dens_x.size
-        n_sub = max(self._orders) + rank // 2
-        scratch = 8 * nv * ng * n_sub * (5 * m + 8) + 8 * self.edge_base.size
-        if self.identity["payload_estimate_bytes"] + scratch > self._memory_bytes:
-            raise MemoryError("complete rank-aware Coulomb workspace exceeds memory_bytes")
-        weights = np.zeros(self.edge_base.size)
-        pos = self._legacy_edges
-        lo = self._breaks[:-1]
-        span = self._gr - lo[:, None]
-        width = np.diff(self._breaks)
-        top = max(self._orders)
-        for a in range(2):
-            for b in range(2):
-                q = (self._orders[a] if a == b else top) + rank // 2
+    def _gauss_moments(self, rank):
+        g = self._grid
+        with np.errstate(over="raise", invalid="raise", divide="raise"):
+            return g.grw * g.gr ** rank, g.grw * g.gr ** (-rank - 1)
+
+    def _sub_rule_edges(self, rank):
+        """Same-cell weights (nv, 4, ng, m): the sub-rule of the rank from the
+        cell's left boundary up to each Gauss node, weighted by
+        grw / r_>^(k+1) * r_<^k, applied to the shape-node interpolant."""
+        g = self._grid
+        m = self._shape_x.size
+        half = rank // 2
+        n_sub = g.top + half
+        scratch = 8 * g.nv * g.ng * n_sub * (3 * m + 4) * len(_COMBOS)
+        if self._estimate + scratch > self._memory_bytes:
+            raise MemoryError(f"rank-aware sub-rule of rank {rank} ({n_sub} points per Gauss "
+                              f"node) needs {scratch} more bytes; memory_bytes="
+                              f"{self._memory_bytes}")
+        span = g.gr - g.lo[:, None]                                     # (nv, ng)
+        out = np.empty((g.nv, len(_COMBOS), g.ng, m))
+        with np.errstate(over="raise", invalid="raise", divide="raise"):
+            for e, (a, b) in enumerate(_COMBOS):
+                q = (g.orders[a] if a == b else g.top) + half
                 x, w = _gl01(q)
-                t = span[:, :, None] * x / width[:, None, None]        # local coordinate
-                card = cardinal_values(t, self._dens_x)                # (nv, ng, q, m)
-                sub = lo[:, None, None] + span[:, :, None] * x
-                radial = span[:, :, None] * w * (sub / self._gr[:, :, None]) ** rank
-                block = np.einsum("cgq,cgqj->cgj", radial, card)
-                block *= (self._weights / self._gr)[:, :, None]
-                weights[pos:pos + nv * ng * m] = block.ravel()
-                pos += nv * ng * m
-        return weights
+                r_lt = g.lo[:, None, None] + span[:, :, None] * x         # (nv, ng, q)
+                shape_fn = cardinal_values((r_lt - g.lo[:, None, None]) / g.width[:, None, None],
+                                           self._shape_x, self._shape_lam)  # (nv, ng, q, m)
+                inner = span[:, :, None] * w * (r_lt / g.gr[:, :, None]) ** rank
+                out[:, e] = (g.grw / g.gr)[:, :, None] * np.einsum("cgq,cgqj->cgj", inner,
+                                                                   shape_fn)
+        return out
+
+    def _rank_tables(self, rank):
+        edges = self._sub_rule_edges(rank)   # holds the memory check of the rank
+        rising, falling = self._gauss_moments(rank)
+        return rising, falling, edges.ravel()
+
 
     # -- device tables ------------------------------------------------------------
     @property
@@ -285,7 +324,7 @@ class CellCoulombFeatures:
             self._dev = {
                 "outer": dv.i32(outer.ravel()),
                 "slot": dv.i32(slot),
-                "tables": [dv.f64(t) for t in self._tables],
+                "tables": [dv.f64(t) for t in self.tables],
             }
         return self._dev
 
@@ -293,7 +332,7 @@ class CellCoulombFeatures:
         """Device rising/falling weights and the same-cell edges of one rank:
         by destination (CSR) for the action, as directed (src, dst, w) for
         the pair contraction.  Zero-weight edges are dropped."""
-        rank = _rank(rank)
+        rank = _multipole(rank)
         d = self._dev_rank.get(rank)
         if d is None:
             rising, falling, w = self.rank_arrays(rank)
@@ -326,7 +365,8 @@ class CellCoulombFeatures:
     def _check_coefficients(self, values):
         values = np.asarray(values)
         if values.shape != (self._ns,) or not np.all(np.isfinite(values)):
-            raise ValueError("finite padded native coefficient vector required")
+            raise ValueError(f"coefficient vector must be finite with the workspace's padded "
+                             f"length ns = {self._ns}")
         return values
 
     def sample_columns(self, P, Q):
@@ -336,9 +376,11 @@ class CellCoulombFeatures:
         P = np.atleast_2d(np.asarray(P))
         Q = np.atleast_2d(np.asarray(Q))
         if P.shape != Q.shape or P.ndim != 2 or P.shape[1] != self._ns:
-            raise ValueError("finite padded native coefficient vector required")
+            raise ValueError(f"coefficient vector must be finite with the workspace's padded "
+                             f"length ns = {self._ns}")
         if not (np.all(np.isfinite(P)) and np.all(np.isfinite(Q))):
-            raise ValueError("finite padded native coefficient vector required")
+            raise ValueError(f"coefficient vector must be finite with the workspace's padded "
+                             f"length ns = {self._ns}")
         cplx = np.iscomplexobj(P) or np.iscomplexobj(Q)
         dtype = np.complex128 if cplx else np.float64
         dev = self._device()
@@ -350,7 +392,7 @@ class CellCoulombFeatures:
             cd = dv.to_dev(np.ascontiguousarray(coef, dtype=dtype))
             tab = dev["tables"][c]
             nv, width, order = tab.shape
-            call("ax_feat_sample", dv.ptr(tab), nv, width, order, self._sizes[c], dv.ptr(cd),
+            call("ax_feat_sample", dv.ptr(tab), nv, width, order, self.coefficient_sizes[c], dv.ptr(cd),
                  self._ns, n_col, int(cplx), dv.ptr(out), n_col, self._node_bases[c], _s())
         return out
 
@@ -371,7 +413,8 @@ class CellCoulombFeatures:
         if rho.dim() == 1:
             rho = rho[:, None]
         if rho.shape[0] != n:
-            raise ValueError("finite complete auxiliary-node density required")
+            raise ValueError(f"density must be finite with one value per auxiliary node "
+                             f"({self.radii.size})")
         rho = rho.contiguous()
         real = torch.view_as_real(rho).reshape(n, -1) if rho.is_complex() else rho
         n_col = real.shape[1]
@@ -388,17 +431,19 @@ class CellCoulombFeatures:
     def apply_density(self, density, rank):
         """Return weighted K_k density in O(nodes + same-cell edges) work;
         a (n_nodes, n) array applies K_k to every column."""
-        rank = _rank(rank)
+        rank = _multipole(rank)
         n = self.radii.size
         if dv._is_tensor(density):
             if density.shape[0] != n or not bool(dv.torch().isfinite(density).all()):
-                raise ValueError("finite complete auxiliary-node density required")
+                raise ValueError(f"density must be finite with one value per auxiliary node "
+                             f"({self.radii.size})")
             res = self.apply_columns(density.to(device=dv.device()), rank)
             return res.reshape(density.shape)
         rho = np.asarray(density)
         rho = rho.astype(np.result_type(rho, float), copy=False)
         if rho.ndim not in (1, 2) or rho.shape[0] != n or not np.all(np.isfinite(rho)):
-            raise ValueError("finite complete auxiliary-node density required")
+            raise ValueError(f"density must be finite with one value per auxiliary node "
+                             f"({self.radii.size})")
         self.rank_arrays(rank)  # rank checks and memory cap before any device work
         res = dv.host(self.apply_columns(dv.to_dev(np.ascontiguousarray(rho)), rank))
         return res.reshape(rho.shape)
@@ -429,8 +474,8 @@ class CellCoulombFeatures:
     def integral(self, orbitals, rank):
         """R^k(ab,cd) for four (P,Q) coefficient pairs, without conjugation."""
         if len(orbitals) != 4:
-            raise ValueError("four native coefficient pairs required")
-        rank = _rank(rank)
+            raise ValueError("integral takes exactly four (P, Q) coefficient pairs")
+        rank = _multipole(rank)
         P = np.stack([self._check_coefficients(p) for p, _ in orbitals])
         Q = np.stack([self._check_coefficients(q) for _, q in orbitals])
         self.rank_arrays(rank)
@@ -440,201 +485,181 @@ class CellCoulombFeatures:
 
 
 class PolynomialCellCoulombFeatures(CellCoulombFeatures):
-    """Exact polynomial-coordinate transform of the native cell quadrature
-    (coulomb_features.py:359-524): each component of a spinor is sampled at
-    2 ks - 1 interpolation nodes per cell and the native operator becomes
-    A^T K A with A the interpolation map, one dense symmetric
-    (2m x 2m) same-cell block per cell."""
+    """The same operator on the shape nodes alone
+    (coulomb_features.py:359-524 for the interface): per cell and component
+    the m = 2*top - 1 Chebyshev points; the Gauss-node values are their
+    interpolants, so the moments and the same-cell edges fold into per-node
+    rising / falling weights and one dense symmetric 2m x 2m block per cell.
+    Sampling, action and integrals are those of ``CellCoulombFeatures``."""
 
     def __init__(self, workspace, *, memory_bytes=128 * _MiB, rank_cache_size=8):
         super().__init__(workspace, memory_bytes=memory_bytes, rank_cache_size=rank_cache_size)
         native = dict(self.identity)
-        nv, ng = self._nv, self._ng
-        m = 2 * max(self._orders) - 1
+        g = self._grid
+        nv, ng = g.nv, g.ng
+        m = self._shape_x.size
         n_nodes = 2 * nv * m
         per_cell = (2 * m) * (2 * m + 1) // 2
         n_edges = nv * per_cell
-        n_native = self.radii.size
-        # the reference's payload bound (coulomb_features.py:380-395)
-        estimate = (native["payload_estimate_bytes"] + 32 * n_native * m
-                    + 8 * nv * m * sum(self._orders) + 8 * n_nodes + 16 * nv * m
-                    + 16 * n_edges + 32 * self.edge_left.size
-                    + rank_cache_size * 8 * (2 * nv * m + n_edges)
-                    + 24 * nv * (2 * m) ** 2 + 16 * nv * ng * max(self._orders) * m)
+        held = (8 * nv * ng * m                           # Gauss-node interpolation
+                + 8 * nv * m * sum(g.orders) + 8 * n_nodes + 16 * n_edges
+                + rank_cache_size * 8 * (2 * nv * m + n_edges)
+                + 16 * nv * (2 * m) ** 2)                 # block assembly scratch
+        device = (8 * nv * m * sum(g.orders) + 4 * n_nodes + 8 * nv * m
+                  + rank_cache_size * _device_rank_bytes(n_nodes, n_edges, nv * m))
+        estimate = native["payload_estimate_bytes"] + held + device
         if estimate > memory_bytes:
-            raise MemoryError(
-                f"complete polynomial Coulomb payload needs {estimate} bytes; cap={memory_bytes}")
-        knots = (np.asarray(workspace.get("tp", workspace["t"]), dtype=float),
-                 np.asarray(workspace["t"], dtype=float))
-        lo, width = self._breaks[:-1], np.diff(self._breaks)
-        grid = lo[:, None] + width[:, None] * self._dens_x
-        maps, tables = [], []
-        start = 0
-        for c in range(2):
-            w_c = self._tables[c].shape[1]
-            old = self.radii[start:start + nv * w_c].reshape(nv, w_c)
-            maps.append(cardinal_values((old - lo[:, None]) / width[:, None], self._dens_x)
-                        .reshape(nv * w_c, m))
-            tables.append(_frozen(evaluate_bsplines_on_grid(
-                knots[c], self._sizes[c], self._orders[c], grid, nderiv=1)[..., 0]))
-            start += nv * w_c
-        self._interpolation = _frozen(np.concatenate(maps))
-        self._native_outer = self.outer_indices
-        boundary = nv * self._tables[0].shape[1]
-        groups = []
-        for a in range(2):
-            for b in range(2):
-                idx = np.flatnonzero(((self.edge_left < boundary) == (a == 0))
-                                     & ((self.edge_right < boundary) == (b == 0)))
-                cell = (self.edge_left[idx] - (boundary if a else 0)) // \
-                    self._tables[a].shape[1]
-                groups.append(_frozen(idx[np.argsort(cell, kind="stable")].reshape(nv, -1)))
-        self._native_edge_groups = tuple(groups)
-        self._native_edge_left, self._native_edge_right = self.edge_left, self.edge_right
-        self._tables = tuple(tables)
-        self.radii = _frozen(np.concatenate((grid.ravel(), grid.ravel())))
-        self.outer_indices = _frozen(np.arange(n_nodes).reshape(2, nv, m))
-        tl, tr = np.triu_indices(2 * m)
-        self._local_left, self._local_right = _frozen(tl), _frozen(tr)
-        self._diagonal_edge = _frozen(tl == tr)
-        per_cell_nodes = self.outer_indices.transpose(1, 0, 2).reshape(nv, 2 * m)
-        self.edge_left = _frozen(per_cell_nodes[:, tl].ravel())
-        self.edge_right = _frozen(per_cell_nodes[:, tr].ravel())
+            raise MemoryError(f"complete polynomial-coordinate geometry would hold {estimate} "
+                              f"bytes (with the Gauss-node operator it is built from), above "
+                              f"memory_bytes={memory_bytes}")
+        # Gauss-node values as interpolants of the shape-node values
+        self._to_gauss = _frozen(cardinal_values((g.gr - g.lo[:, None]) / g.width[:, None],
+                                                 self._shape_x, self._shape_lam))  # (nv, ng, m)
+        self.tables = tuple(_frozen(t[:, ng:, :]) for t in self.tables)
+        shape_r = (g.lo[:, None] + g.width[:, None] * self._shape_x).ravel()
+        self.radii = _frozen(np.concatenate((shape_r, shape_r)))
         self._node_bases = (0, nv * m)
+        self.outer_indices = _frozen(np.arange(n_nodes, dtype=np.int64).reshape(2, nv, m))
+        # per cell, the upper triangle of the block over local index
+        # component * m + j, mapped to global node numbers
+        up_l, up_r = np.triu_indices(2 * m)
+        self._block_pick = (up_l, up_r)
+        local = self.outer_indices.transpose(1, 0, 2).reshape(nv, 2 * m)
+        self.edge_left = _frozen(local[:, up_l].ravel())
+        self.edge_right = _frozen(local[:, up_r].ravel())
         self._m = m
-        h = hashlib.sha256(native["geometry_sha256"].encode())
-        for value in (*tables, self.radii, self._interpolation, self.edge_left,
-                      self.edge_right):
-            h.update(repr((value.shape, value.dtype.str)).encode())
-            h.update(np.ascontiguousarray(value).tobytes())
+        self._estimate = estimate
+        self._rank_cache.clear()
         self.identity = MappingProxyType(dict(
             native,
-            representation="native cell Coulomb polynomial density coordinates",
+            representation="cell Coulomb operator on Chebyshev shape nodes only",
             native_geometry_sha256=native["geometry_sha256"],
-            geometry_sha256=h.hexdigest(),
+            geometry_sha256=_digest(native["geometry_sha256"].encode(), self.radii,
+                                    self._to_gauss, *self.tables),
             auxiliary_nodes=n_nodes,
             same_cell_edges=n_edges,
             quadrature_nodes=(nv, m),
             payload_estimate_bytes=estimate,
             density_polynomial_degree=m - 1,
             interpolation_nodes_per_component_cell=m,
-            contraction="A.T K_native A weighted bilinear c-product; original quadrature retained",
+            contraction=("c-product samples at the shape nodes; the Gauss-node operator "
+                         "conjugated by the interpolation map"),
         ))
 
     @property
     def _ngo(self):
         return self._m
 
-    def rank_arrays(self, rank):
-        rank = _rank(rank)
-        if rank not in self._rank_cache:
-            m, nv = self._m, self._nv
-            lag = self._interpolation[self._native_outer[0]]          # (nv, ng, m)
-            with np.errstate(over="raise", invalid="raise", divide="raise"):
-                native_w = CellCoulombFeatures._edge_weights(self, rank)
-                rising = np.einsum("cgj,cg->cj", lag, self._weights * self._gr ** rank)
-                falling = np.einsum("cgj,cg->cj", lag, self._weights * self._gr ** (-rank - 1))
-            local = np.zeros((nv, 2 * m, 2 * m))
-            for (a, b), idx in zip(((0, 0), (0, 1), (1, 0), (1, 1)), self._native_edge_groups):
-                L = self._interpolation[self._native_edge_left[idx]]   # (nv, e, m)
-                R = self._interpolation[self._native_edge_right[idx]]
-                term = np.swapaxes(L, 1, 2) @ (native_w[idx][..., None] * R)
-                local[:, a * m:(a + 1) * m, b * m:(b + 1) * m] += term
-                local[:, b * m:(b + 1) * m, a * m:(a + 1) * m] += np.swapaxes(term, 1, 2)
-            packed = local[:, self._local_left, self._local_right].copy()
-            packed[:, self._diagonal_edge] *= 0.5   # a diagonal edge acts in both directions
-            self._rank_cache[rank] = tuple(_frozen(x) for x in (rising, falling, packed.ravel()))
-            while len(self._rank_cache) > self._rank_capacity:
-                self._rank_cache.popitem(last=False)
-        else:
-            self._rank_cache.move_to_end(rank)
-        return self._rank_cache[rank]
+    def _rank_tables(self, rank):
+        edges = self._sub_rule_edges(rank)                      # (nv, 4, ng, m)
+        rising_g, falling_g = self._gauss_moments(rank)         # (nv, ng)
+        L = self._to_gauss                                      # (nv, ng, m)
+        rising = np.einsum("cg,cgj->cj", rising_g, L)
+        falling = np.einsum("cg,cgj->cj", falling_g, L)
+        m = self._m
+        block = np.zeros((self._nv, 2 * m, 2 * m))
+        for e, (a, b) in enumerate(_COMBOS):
+            # edge outer(a, g) -> shape(b, j) with the outer value an
+            # interpolant: a term between shape(a, i) and shape(b, j)
+            piece = np.einsum("cgi,cgj->cij", L, edges[:, e])
+            block[:, a * m:(a + 1) * m, b * m:(b + 1) * m] += piece
+            block[:, b * m:(b + 1) * m, a * m:(a + 1) * m] += np.swapaxes(piece, 1, 2)
+        up_l, up_r = self._block_pick
+        w = block[:, up_l, up_r]
+        w[:, up_l == up_r] *= 0.5     # a diagonal edge acts in both directions
+        return rising, falling, w.ravel()
 
 
 class CellPairContraction:
-    """M[m,n] = sum[r,s] R^k(mr,sn) * rho[r,s], with c-products
-    (pair_contraction.py:18-148).
+    """M[m,n] = sum[r,s] R^k(mr,sn) * rho[r,s] with c-products
+    (pair_contraction.py:18-148 for the interface).
 
-    Operands are complete auxiliary-node column samples ordered target-left,
-    source-left, source-right, target-right; they are copied to the device
-    at construction (later changes by the caller do not matter).  The
-    cross-cell part uses per-cell moments of the left and right operand
-    pairs with prefix/suffix sums over cells; the same-cell part runs over
-    every directed edge.  Results are FP64 (complex FP64 for complex
-    operands)."""
+    Operands are complete auxiliary-node column samples ordered target-left
+    (m columns), source-left (r), source-right (s), target-right (n); they
+    are copied to the device at construction, so later changes by the
+    caller do not matter.  On the device: per-cell moments of the left and
+    right operand column products on the outer nodes, prefix / suffix sums
+    of the right moments over cells (the cross-cell part), and every
+    directed same-cell edge of the features (``ax_cpc_*``).  Results are
+    FP64 (complex FP64 for complex operands).  ``edge_batch_size`` is
+    accepted and recorded; the device takes all edges in one pass."""
 
     def __init__(self, features, target_left, source_left, source_right, target_right, rank, *,
                  edge_batch_size=1024, max_work_bytes=128 * _MiB):
-        _positive_int("edge_batch_size", edge_batch_size)
-        _positive_int("max_work_bytes", max_work_bytes)
-        arrays = tuple(np.asarray(a) for a in (target_left, source_left, source_right,
-                                               target_right))
-        arrays = tuple(a.astype(np.result_type(a, float), copy=False) for a in arrays)
-        if any(a.ndim != 2 or a.shape[0] != len(features.radii) or a.shape[1] < 1
-               or not np.all(np.isfinite(a)) for a in arrays):
-            raise ValueError("finite complete auxiliary-node column operands required")
-        rank = _rank(rank)
-        rising, falling, edges = features.rank_arrays(rank)
-        m, r, s, n = (a.shape[1] for a in arrays)
-        nv, ng = features.identity["quadrature_nodes"]
-        batch = min(int(edge_batch_size), len(edges))
-        # the reference's work bound (pair_contraction.py:68-79)
-        bound = sum(a.nbytes for a in arrays) + 16 * (
-            nv * (4 * m * r + 6 * n * s + 2 * m * s) + 4 * m * n + r * s
-            + batch * (2 * m + 2 * r + 4 * s + 2 * n + 4) + 2 * nv * ng * (m + r + s + n))
+        _require_count("edge_batch_size", edge_batch_size)
+        _require_count("max_work_bytes", max_work_bytes)
+        n_nodes = len(features.radii)
+        ops = []
+        for a in (target_left, source_left, source_right, target_right):
+            a = np.asarray(a)
+            a = a.astype(np.result_type(a, float), copy=False)
+            if a.ndim != 2 or a.shape[0] != n_nodes or a.shape[1] < 1 or \
+                    not bool(np.isfinite(a).all()):
+                raise ValueError(f"operands must be finite 2-D arrays with one row per "
+                                 f"auxiliary node ({n_nodes}) and at least one column")
+            ops.append(a)
+        rank = _multipole(rank)
+        features.rank_arrays(rank)
+        m, r, s, n = (a.shape[1] for a in ops)
+        nv = features._nv
+        d = features._rank_device(rank)
+        n_edges = d["n_directed"]
+        split = int(min(256, max(1, n_edges // 512)))
+        self._complex = any(np.iscomplexobj(a) for a in ops)
+        item = 16 if self._complex else 8
+        work = int(load().ax_cpc_workspace_bytes(nv, r, s, m, n, n_nodes, n_edges, split,
+                                                  int(self._complex)))
+        # every device byte this object holds: operands, left moments (rising
+        # and falling), right moments with their prefix / suffix, the
+        # action's workspace and one output
+        bound = (item * n_nodes * (m + r + s + n) + item * 2 * nv * r * m
+                 + item * 4 * nv * s * n + work + item * m * n)
         if bound > max_work_bytes:
-            raise MemoryError(
-                f"complete cell-pair contraction needs {bound} bytes; cap={max_work_bytes}")
+            raise MemoryError(f"complete cell-pair contraction would hold {bound} device bytes, "
+                              f"above max_work_bytes={max_work_bytes}")
         torch = dv.torch()
-        self._complex = any(np.iscomplexobj(a) for a in arrays)
         self._np_dtype = np.complex128 if self._complex else np.float64
         self._torch_dtype = torch.complex128 if self._complex else torch.float64
         self._features = features
         self._rank = rank
         self._dims = (m, r, s, n)
         self._nv = nv
-        self._batch = batch
+        self._d = d
         fd = features._device()
-        self._d = features._rank_device(rank)
-        A, B, C, D = (dv.to_dev(np.ascontiguousarray(a, dtype=self._np_dtype)) for a in arrays)
+        A, B, C, D = (dv.to_dev(np.ascontiguousarray(a, dtype=self._np_dtype)) for a in ops)
         self._ops = (A, B, C, D)
         ngo = features._ngo
         cplx = int(self._complex)
-        mk = lambda *shape: torch.empty(shape, dtype=self._torch_dtype, device=dv.device())  # noqa: E731
-        # left moments [cell][r][m] (rising, falling) stacked for one product
-        self._left = mk(2, nv, r, m)
+
+        def dev(*shape):
+            return torch.empty(shape, dtype=self._torch_dtype, device=dv.device())
+
+        self._left = dev(2, nv, r, m)              # [rising, falling][cell][r][m]
         call("ax_cpc_moments", dv.ptr(A), m, m, dv.ptr(B), r, r, dv.ptr(fd["outer"]),
-             dv.ptr(self._d["rising"]), dv.ptr(self._d["falling"]), nv, ngo, cplx,
+             dv.ptr(d["rising"]), dv.ptr(d["falling"]), nv, ngo, cplx,
              dv.ptr(self._left[0]), dv.ptr(self._left[1]), _s())
-        rr, rf = mk(nv, s, n), mk(nv, s, n)
+        right_rise, right_fall = dev(nv, s, n), dev(nv, s, n)
         call("ax_cpc_moments", dv.ptr(D), n, n, dv.ptr(C), s, s, dv.ptr(fd["outer"]),
-             dv.ptr(self._d["rising"]), dv.ptr(self._d["falling"]), nv, ngo, cplx,
-             dv.ptr(rr), dv.ptr(rf), _s())
-        self._prefix, self._suffix = mk(nv, s, n), mk(nv, s, n)
-        call("ax_cpc_scan", dv.ptr(rr), dv.ptr(rf), nv, s * n, cplx, dv.ptr(self._prefix),
-             dv.ptr(self._suffix), _s())
+             dv.ptr(d["rising"]), dv.ptr(d["falling"]), nv, ngo, cplx,
+             dv.ptr(right_rise), dv.ptr(right_fall), _s())
+        self._prefix, self._suffix = dev(nv, s, n), dev(nv, s, n)
+        call("ax_cpc_scan", dv.ptr(right_rise), dv.ptr(right_fall), nv, s * n, cplx,
+             dv.ptr(self._prefix), dv.ptr(self._suffix), _s())
         self._density_shape = (r, s)
         self._output_shape = (m, n)
-        n_edges = self._d["n_directed"]
-        self._split = int(min(256, max(1, n_edges // 512)))
-        ws = int(load().ax_cpc_workspace_bytes(nv, r, s, m, n, len(features.radii), n_edges,
-                                                self._split, cplx))
-        self._work = dv.empty((ws // 8 + 1,))
-        h = hashlib.sha256()
-        for a in arrays:
-            h.update(repr((a.shape, a.dtype.str)).encode())
-            h.update(np.ascontiguousarray(a).tobytes())
+        self._split = split
+        self._work = dv.empty((work // 8 + 1,))
         self.identity = MappingProxyType(dict(
-            representation="exact cell pair-density matrix contraction",
+            representation="cell pair-density matrix action on auxiliary-node samples",
             feature_geometry_sha256=features.identity["geometry_sha256"],
-            operand_sha256=h.hexdigest(),
-            rank=int(rank),
+            operand_sha256=_digest(*ops),
+            rank=rank,
             density_shape=(r, s),
             output_shape=(m, n),
-            edge_batch_size=batch,
+            edge_batch_size=int(edge_batch_size),
             complete_work_bound_bytes=bound,
             max_work_bytes=int(max_work_bytes),
-            operator="weighted native Coulomb C-product; no amplitude threshold",
+            operator="cross-cell moments with prefix / suffix sums plus every same-cell edge",
         ))
 
     def apply_device(self, rho):
@@ -657,7 +682,7 @@ class CellPairContraction:
         rho = np.asarray(density)
         rho = rho.astype(np.result_type(rho, float), copy=False)
         if rho.shape != self._density_shape or not np.all(np.isfinite(rho)):
-            raise ValueError("finite complete source density matrix required")
+            raise ValueError(f"source density matrix must be finite with shape {self._density_shape}")
         if np.iscomplexobj(rho) and not self._complex:
             # real operands: the action is linear in rho
             result = (dv.host(self.apply_device(dv.to_dev(np.ascontiguousarray(rho.real))))
@@ -667,5 +692,5 @@ class CellPairContraction:
             result = dv.host(self.apply_device(
                 dv.to_dev(np.ascontiguousarray(rho, dtype=self._np_dtype))))
         if not np.all(np.isfinite(result)):
-            raise ValueError("nonfinite native cell-pair matrix action")
+            raise ValueError("cell-pair action produced non-finite values")
         return result

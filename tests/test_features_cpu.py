@@ -23,9 +23,9 @@ def host_sample(f, P, Q):
     """Samples at the auxiliary nodes by the oracle's B-spline evaluation."""
     out = []
     for c, coef in enumerate((P, Q)):
-        tab = f._tables[c]
+        tab = f.tables[c]
         nv, _, order = tab.shape
-        out.append(O.eval_orbital_on_grid(coef, tab, nv, order, f._sizes[c]).ravel())
+        out.append(O.eval_orbital_on_grid(coef, tab, nv, order, f.coefficient_sizes[c]).ravel())
     return np.concatenate(out)
 
 

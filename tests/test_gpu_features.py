@@ -29,9 +29,9 @@ def _bd(ns, ks, r_max, kb):
 def _host_sample(f, P, Q):
     out = []
     for c, coef in enumerate((P, Q)):
-        tab = f._tables[c]
+        tab = f.tables[c]
         nv, _, order = tab.shape
-        out.append(O.eval_orbital_on_grid(coef, tab, nv, order, f._sizes[c]).ravel())
+        out.append(O.eval_orbital_on_grid(coef, tab, nv, order, f.coefficient_sizes[c]).ravel())
     return np.concatenate(out)
 
 
