@@ -96,6 +96,9 @@ class SlaterResult:
     offsets: list
     m: list
     latency_s: float = 0.0
+    # int8 slices per operand the contraction used (0: it ran in FP64, also
+    # when split precision was asked for and the grid needs FP64)
+    slices: int = 0
 
     def matrix(self, k):
         m = self.m[k]
@@ -929,7 +932,7 @@ class GpuExecutor:
                 for j, st in enumerate(sts):
                     o = int(ooff[first + j]) - n_in
                     got.append(SlaterResult(vals[o:o + int(outs[first + j])], st.plan.offsets,
-                                            st.plan.m))
+                                            st.plan.m, slices=st.plan.slices))
                 results.append(got if req.kind == "slater_batch" else got[0])
             elif req.kind in ("slater", "slater_batch"):
                 if "values" not in job:  # views of the set's own buffer, or one copy
@@ -941,7 +944,8 @@ class GpuExecutor:
                 for j, st in enumerate(sts):
                     o, k = job["results"][(i, j if req.kind == "slater_batch" else -1)]
                     o -= n_in
-                    got.append(SlaterResult(hv[o:o + k], st.plan.offsets, st.plan.m))
+                    got.append(SlaterResult(hv[o:o + k], st.plan.offsets, st.plan.m,
+                                            slices=st.plan.slices))
                 results.append(got if req.kind == "slater_batch" else got[0])
             else:  # the pinned result buffers are handed over (the arrays keep them alive)
                 results.append(FockResult({k: v.numpy() for k, v in job["host"].items()}))
