@@ -67,3 +67,12 @@ hipError_t staging_release(int slot, hipStream_t stream);
             return AX_ERR_VALUE;                                                  \
         }                                                                         \
     } while (0)
+
+// AX_REQUIRE with an explicit status code (AX_ERR_MEMORY for size limits)
+#define AX_REQUIRE_CODE(cond, code, ...)                                          \
+    do {                                                                          \
+        if (!(cond)) {                                                            \
+            ax::set_error(__VA_ARGS__);                                           \
+            return (code);                                                        \
+        }                                                                         \
+    } while (0)

@@ -30,6 +30,13 @@ struct RkSeg {
     int64_t ys_ld, ys_plane;
     const int32_t *Ye;
     int64_t ye_ld;
+    // split precision: K-tile 0 (nodes 0..63, from r = 0, where r^k spans
+    // more binary orders inside the tile than the slices carry) is
+    // contracted in FP64 from D rows d_rows[i] (nullptr = identity) and the
+    // FP64 first K-tile of the Y^k rows, Y0 + j*y0_ld (nullptr: no FP64 tile)
+    const double *Y0;
+    int64_t y0_ld;
+    const int32_t *d_rows;
 };
 
 struct RkWork {
@@ -86,6 +93,7 @@ struct ScanArg {             // Y^k items of one plan
     int32_t *Ye;
     int64_t npad, ys_plane;
     int slices, pad;
+    double *Y0;              // FP64 first K-tile of the sliced rows (kSplitKTile per row)
 };
 struct SliceArg {            // rows sliced by a separate pass
     const double *src;
@@ -277,6 +285,7 @@ struct PlanHost {
     const double *w, *rk, *rkm;
     const int32_t *pa, *pb, *item_row, *item_k, *item_out;
     double *D, *Y;          // Y null when the scan writes slices directly
+    double *Y0;             // then the FP64 first K-tile of each Y^k row (split)
     int8_t *Ds, *Ys;
     int32_t *De, *Ye;
     const RkSeg *segs;

@@ -137,6 +137,47 @@ __device__ inline void split_job(const ax::BatchOf<ax::GemmArg> &A, const ax::Jo
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
+    // K-tile 0 (nodes 0..63) holds the cell that reaches r = 0, where the
+    // powers of r inside one tile span more binary orders than the slices
+    // carry: it is contracted in FP64, from the FP64 densities and the FP64
+    // first tile of the Y^k rows, through the LDS region of the slices.
+    int64_t t0 = tb;
+    if (tb == 0 && te > 0 && sg.Y0 != nullptr) {
+        constexpr int LD = KT + 1;  // padded row (LDS banks)
+        double *Dt = reinterpret_cast<double *>(As);
+        double *Yt = Dt + BM * LD;
+        const int64_t nk = sg.n < KT ? sg.n : KT;
+        for (int e = tid; e < BM * KT; e += 256) {
+            const int r = e / KT, x = e - (e / KT) * KT;
+            const int i = row0 + r, j = col0 + r;
+            double dvl = 0.0, yvl = 0.0;
+            if (x < nk && i < sg.m) {
+                const int64_t dr = sg.d_rows ? (int64_t)sg.d_rows[i] : (int64_t)i;
+                dvl = sg.D[dr * sg.ld_d + x];
+            }
+            if (x < nk && j < sg.m) yvl = sg.Y0[(int64_t)j * sg.y0_ld + x];
+            Dt[r * LD + x] = dvl;
+            Yt[r * LD + x] = yvl;
+        }
+        __syncthreads();
+        for (int x = 0; x < KT; ++x) {
+            double dv[2][4], yv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) dv[a][r] = Dt[(wr + a * 16 + 4 * fg + r) * LD + x];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) yv[b] = Yt[(wc + b * 16 + fr) * LD + x];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[a][b][r] = fma(dv[a][r], yv[b], acc[a][b][r]);
+        }
+        t0 = 1;  // the slice loop's first barrier frees the LDS again
+    }
+
     bool va_ok, vb_ok;
     const int8_t *pa = stage_base(sg.Ds, sg.ds_ld, sg.rows, row0, sg.m, va_ok);
     const int8_t *pb = stage_base(sg.Ys, sg.ys_ld, nullptr, col0, sg.m, vb_ok);
@@ -145,13 +186,13 @@ __device__ inline void split_job(const ax::BatchOf<ax::GemmArg> &A, const ax::Jo
     const bool has_e = tid < 2 * BM;
     v4i ra[S], rb[S];
     int rexp = 0;
-    if (tb < te) {
-        stage_load<S>(pa, sg.ds_plane, tb, ra);
-        stage_load<S>(pb, sg.ys_plane, tb, rb);
-        if (has_e) rexp = ep[tb];
+    if (t0 < te) {
+        stage_load<S>(pa, sg.ds_plane, t0, ra);
+        stage_load<S>(pb, sg.ys_plane, t0, rb);
+        if (has_e) rexp = ep[t0];
     }
     const v4i zero = {0, 0, 0, 0};
-    for (int64_t t = tb; t < te; ++t) {
+    for (int64_t t = t0; t < te; ++t) {
         __syncthreads();  // previous tile's LDS reads are complete
         stage_store<S>(As, ra, va_ok);
         stage_store<S>(Bs, rb, vb_ok);
@@ -261,7 +302,10 @@ ax_status slice_rows_launch(const double *src, int64_t ld, const int32_t *rows, 
 
 template <int S>
 ax_status launch_split(const BatchOf<GemmArg> &A, dim3 grid, hipStream_t stream) {
-    const size_t lds = (size_t)2 * S * PLANEB;
+    // the slice tiles, or the two FP64 tiles of K-tile 0 (64 x 65 doubles each)
+    const size_t tile0 = (size_t)2 * BM * (KT + 1) * sizeof(double);
+    const size_t slices = (size_t)2 * S * PLANEB;
+    const size_t lds = slices > tile0 ? slices : tile0;
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(rk_gemm_split_kernel<S>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));

@@ -411,6 +411,9 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
         o_De = L.add((int64_t)n_pairs * tiles * 4);
         o_Ye = L.add(yr * tiles * 4);
     }
+    // FP64 first K-tile of the Y^k rows, when the scan keeps no FP64 rows
+    const int64_t o_Y0 = (precision == AX_PREC_SPLIT && !y_fp64)
+                             ? L.add(yr * ax::kSplitKTile * 8) : -1;
     if (estimate) {
         *estimate = static_cast<int64_t>(ax::pool_class(static_cast<size_t>(L.size)));
         return AX_OK;
@@ -492,6 +495,11 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     int32_t *De = precision == AX_PREC_SPLIT ? static_cast<int32_t *>(dptr(o_De)) : nullptr;
     int32_t *Ye = precision == AX_PREC_SPLIT ? static_cast<int32_t *>(dptr(o_Ye)) : nullptr;
     const int32_t *rows_dev = static_cast<const int32_t *>(dptr(o_rows));
+    // FP64 first K-tile of the Y^k rows for the split contraction: the scan's
+    // own copy, or the FP64 rows when they exist
+    double *Y0 = o_Y0 >= 0 ? static_cast<double *>(dptr(o_Y0)) : nullptr;
+    const int64_t y0_ld = Y0 ? ax::kSplitKTile : n;
+    double *Y0base = precision == AX_PREC_SPLIT ? (Y0 ? Y0 : Y) : nullptr;
     {
         std::vector<ax::RkSeg> segs;
         for (int k : seg_k) {
@@ -515,6 +523,9 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
                 sg.ys_plane = yr * npad;
                 sg.Ye = Ye + y_off[k] * tiles;
                 sg.ye_ld = tiles;
+                sg.Y0 = Y0base ? Y0base + y_off[k] * y0_ld : nullptr;
+                sg.y0_ld = y0_ld;
+                sg.d_rows = sg.rows;
             }
             segs.push_back(sg);
         }
@@ -572,6 +583,7 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     h.item_out = static_cast<const int32_t *>(dptr(o_iout));
     h.D = D;
     h.Y = Y;
+    h.Y0 = Y0;
     h.Ds = Ds;
     h.Ys = Ys;
     h.De = De;
@@ -837,22 +849,26 @@ int64_t ax_rk_gemm_workspace_bytes(int m, int64_t n, int precision) {
     return gemm_workspace(m, n, precision, nw, choose_split(nw, n)).size;
 }
 
-ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const double *Y,
-                     int64_t ld_y, int m, int64_t n, int precision, void *workspace, double *out,
-                     void *stream) {
-    AX_REQUIRE(m >= 0 && m < 0x8000 * 64 && n > 0 && ld_d >= n && ld_y >= n,
-               "ax_rk_gemm: bad sizes");
-    AX_REQUIRE(precision == AX_PREC_FP64 || precision == AX_PREC_SPLIT,
-               "ax_rk_gemm: unknown precision %d", precision);
-    if (m == 0) return AX_OK;
-    AX_REQUIRE(workspace != nullptr && out != nullptr, "ax_rk_gemm: workspace and output required");
-    hipStream_t s = ax::as_stream(stream);
+}  // extern "C"
+
+namespace {
+
+// The contraction of one block into a workspace of at least
+// gemm_workspace(...).size bytes (ws_bytes < 0: not checked).  Every
+// workspace byte the kernels read is written earlier in the same call, so
+// the workspace needs no clearing.
+ax_status rk_gemm_into(const double *D, int64_t ld_d, const int32_t *rows, const double *Y,
+                       int64_t ld_y, int m, int64_t n, int precision, char *ws,
+                       int64_t ws_bytes, const char *who, double *out, hipStream_t s) {
     std::vector<ax::RkWork> work;
     ax::upper_tiles(m, 0, work);
     const int nw = static_cast<int>(work.size());
     const int split = choose_split(nw, n);
     const GemmWorkspace g = gemm_workspace(m, n, precision, nw, split);
-    char *ws = static_cast<char *>(workspace);
+    AX_REQUIRE_CODE(ws_bytes < 0 || g.size <= ws_bytes, AX_ERR_MEMORY,
+                    "%s: m = %d, n = %lld needs %lld bytes of scratch, the maximum is %lld",
+                    who, m, static_cast<long long>(n), static_cast<long long>(g.size),
+                    static_cast<long long>(ws_bytes));
 
     ax::RkSeg sg{};
     sg.D = D;
@@ -878,6 +894,10 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
                                    tiles, s);
         if (st != AX_OK) return st;
         sg.rows = nullptr;
+        // K-tile 0 is contracted in FP64 from the caller's operands
+        sg.Y0 = Y;
+        sg.y0_ld = ld_y;
+        sg.d_rows = rows;
         sg.Ds = Ds;
         sg.ds_ld = npad;
         sg.ds_plane = (int64_t)m * npad;
@@ -927,6 +947,29 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
     // the host-side descriptors above must outlive the asynchronous copies
     AX_CHECK_HIP(hipStreamSynchronize(s));
     return AX_OK;
+}
+
+ax_status check_gemm_args(const char *who, int m, int64_t n, int64_t ld_d, int64_t ld_y,
+                          int precision) {
+    AX_REQUIRE(m >= 0 && m < 0x8000 * 64 && n > 0 && ld_d >= n && ld_y >= n, "%s: bad sizes",
+               who);
+    AX_REQUIRE(precision == AX_PREC_FP64 || precision == AX_PREC_SPLIT,
+               "%s: unknown precision %d", who, precision);
+    return AX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const double *Y,
+                     int64_t ld_y, int m, int64_t n, int precision, void *workspace, double *out,
+                     void *stream) {
+    ax_status st = check_gemm_args("ax_rk_gemm", m, n, ld_d, ld_y, precision);
+    if (st != AX_OK || m == 0) return st;
+    AX_REQUIRE(workspace != nullptr && out != nullptr, "ax_rk_gemm: workspace and output required");
+    return rk_gemm_into(D, ld_d, rows, Y, ld_y, m, n, precision, static_cast<char *>(workspace),
+                        -1, "ax_rk_gemm", out, ax::as_stream(stream));
 }
 
 }  // extern "C"

@@ -332,7 +332,8 @@ __device__ inline void scan_item(double *__restrict__ sm, ScanLds<NG> &L,
                                  const double *__restrict__ rkp, const double *__restrict__ rkmp,
                                  int64_t n, int nv, double *__restrict__ dst,
                                  int8_t *__restrict__ sdst = nullptr, int64_t splane = 0,
-                                 int32_t *__restrict__ edst = nullptr, int slices = 0) {
+                                 int32_t *__restrict__ edst = nullptr, int slices = 0,
+                                 double *__restrict__ y0 = nullptr) {
     static_assert(!SLICE || NG * (64 / NG) == 64, "in-scan slicing needs whole K-tiles per wave");
     constexpr int P = NG + 1;
     double *gos = sm;                          // [nv * P] g_out
@@ -432,6 +433,8 @@ __device__ inline void scan_item(double *__restrict__ sm, ScanLds<NG> &L,
                 }
                 const int64_t x = (int64_t)c * NG + node;
                 ax::slice_value(v, x, sdst, splane, edst + x / ax::kSplitKTile, slices);
+                // the first K-tile also in FP64 (it is contracted in FP64)
+                if (y0 != nullptr && x < ax::kSplitKTile) y0[x] = v;
             }
         } else if (lane_used && c < nv) {
             const double *a = gos + (int64_t)c * P;
@@ -509,7 +512,8 @@ __global__ __launch_bounds__(kScanThreads) void yk_scan_batch_kernel(
         if constexpr (SLICE)
             scan_item<NG, true>(sm, L, src, nullptr, g.rk + (int64_t)k * n,
                                 g.rkm + (int64_t)k * n, n, g.nv, nullptr, g.Ys + orow * g.npad,
-                                g.ys_plane, g.Ye + orow * (g.npad / ax::kSplitKTile), g.slices);
+                                g.ys_plane, g.Ye + orow * (g.npad / ax::kSplitKTile), g.slices,
+                                g.Y0 ? g.Y0 + orow * ax::kSplitKTile : nullptr);
         else
             scan_item<NG>(sm, L, src, nullptr, g.rk + (int64_t)k * n, g.rkm + (int64_t)k * n, n,
                           g.nv, g.Y + orow * n);
@@ -1078,6 +1082,11 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
 #pragma unroll
                 for (int m = 0; m < NG; ++m) y[m] = 0.0;
             }
+            if (g.Y0 != nullptr && c0 < ax::kSplitKTile) {  // cells of K-tile 0, also in FP64
+                double *y0 = g.Y0 + g.item_out[local] * ax::kSplitKTile + c0;
+#pragma unroll
+                for (int m = 0; m < NG; ++m) y0[m] = y[m];
+            }
             double mx = 0.0;
 #pragma unroll
             for (int m = 0; m < NG; ++m) mx = fmax(mx, fabs(y[m]));
@@ -1263,7 +1272,7 @@ ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t str
                 const int64_t yr = h->n_yrows > 0 ? h->n_yrows : 1;
                 A.a[A.n] = ScanArg{h->D, h->Y, h->rk, h->rkm, h->item_row, h->item_k,
                                    h->item_out, h->n, h->nv, h->kmax, h->Ys, h->Ye, h->npad,
-                                   yr * h->npad, h->slices, 0};
+                                   yr * h->npad, h->slices, 0, h->Y0};
                 ++A.n;
                 if (h->nv > nv_max) nv_max = h->nv;
                 mo = h->mo;
@@ -1299,7 +1308,7 @@ ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t str
                 const int64_t yr = h->n_yrows > 0 ? h->n_yrows : 1;
                 A.a[A.n] = ScanArg{h->D, h->Y, h->rk, h->rkm, h->item_row, h->item_k,
                                    h->item_out, h->n, h->nv, h->kmax, h->Ys, h->Ye, h->npad,
-                                   yr * h->npad, h->slices, 0};
+                                   yr * h->npad, h->slices, 0, h->Y0};
                 ++A.n;
                 if (h->nv > nv_max) nv_max = h->nv;
                 mo = h->mo;
