@@ -57,6 +57,7 @@ SIGNATURES = {
                                    _vp, ctypes.c_double, _vp, _vp, _vp]),
     "ax_rk_gemm_workspace_bytes": (_i64, [_int, _i64, _int]),
     "ax_rk_gemm": (_int, [_vp, _i64, _vp, _vp, _i64, _int, _i64, _int, _vp, _vp, _vp]),
+    "ax_rk_gemm_noalloc": (_int, [_vp, _i64, _vp, _vp, _i64, _int, _i64, _int, _vp, _vp]),
     "ax_rk_tensor_create": (_int, [_dp, _dp, _int, _int, _int, _ip, _int, _int,
                                    ctypes.POINTER(_vp)]),
     "ax_rk_tensor_create_pairs": (_int, [_dp, _dp, _int, _int, _int, _int, _ip, _ip, _int,

@@ -854,9 +854,10 @@ int64_t ax_rk_gemm_workspace_bytes(int m, int64_t n, int precision) {
 namespace {
 
 // The contraction of one block into a workspace of at least
-// gemm_workspace(...).size bytes (ws_bytes < 0: not checked).  Every
-// workspace byte the kernels read is written earlier in the same call, so
-// the workspace needs no clearing.
+// gemm_workspace(...).size bytes (from the caller, or the library's static
+// scratch; ws_bytes < 0: not checked).  Every workspace byte the kernels
+// read is written earlier in the same call, so the workspace needs no
+// clearing.
 ax_status rk_gemm_into(const double *D, int64_t ld_d, const int32_t *rows, const double *Y,
                        int64_t ld_y, int m, int64_t n, int precision, char *ws,
                        int64_t ws_bytes, const char *who, double *out, hipStream_t s) {
@@ -958,6 +959,31 @@ ax_status check_gemm_args(const char *who, int m, int64_t n, int64_t ld_d, int64
     return AX_OK;
 }
 
+// Scratch of ax_rk_gemm_noalloc: one copy per device, held until the process
+// ends.  Host code uses the per-device address from hipGetSymbolAddress,
+// never the array's name.
+static __device__ __align__(256) char scratch[AX_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+ax_status scratch_for_current_device(char **ptr) {
+    int dev = -1;
+    AX_CHECK_HIP(hipGetDevice(&dev));
+    AX_REQUIRE(dev >= 0 && dev < kMaxDevices, "ax_rk_gemm_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        AX_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        AX_REQUIRE_CODE(p != nullptr, AX_ERR_HIP,
+                        "ax_rk_gemm_noalloc: no scratch address on device %d", dev);
+        g_scratch_ptr[dev] = static_cast<char *>(p);
+    }
+    *ptr = g_scratch_ptr[dev];
+    return AX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -970,6 +996,19 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
     AX_REQUIRE(workspace != nullptr && out != nullptr, "ax_rk_gemm: workspace and output required");
     return rk_gemm_into(D, ld_d, rows, Y, ld_y, m, n, precision, static_cast<char *>(workspace),
                         -1, "ax_rk_gemm", out, ax::as_stream(stream));
+}
+
+ax_status ax_rk_gemm_noalloc(const double *D, int64_t ld_d, const int32_t *rows, const double *Y,
+                             int64_t ld_y, int m, int64_t n, int precision, double *out,
+                             void *stream) {
+    ax_status st = check_gemm_args("ax_rk_gemm_noalloc", m, n, ld_d, ld_y, precision);
+    if (st != AX_OK || m == 0) return st;
+    AX_REQUIRE(out != nullptr, "ax_rk_gemm_noalloc: output required");
+    char *ws = nullptr;
+    st = scratch_for_current_device(&ws);
+    if (st != AX_OK) return st;
+    return rk_gemm_into(D, ld_d, rows, Y, ld_y, m, n, precision, ws, AX_NOALLOC_SCRATCH_BYTES,
+                        "ax_rk_gemm_noalloc", out, ax::as_stream(stream));
 }
 
 }  // extern "C"

@@ -153,6 +153,21 @@ ax_status ax_rk_gemm(const double *D, int64_t ld_d, const int32_t *rows, const d
                      int64_t ld_y, int m, int64_t n, int precision, void *workspace,
                      double *out, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ax_rk_gemm with its workspace taken from a static
+ * device array of AX_NOALLOC_SCRATCH_BYTES that the library holds on every
+ * device it runs on until the process ends.  The call allocates nothing;
+ * it fails with AX_ERR_MEMORY when ax_rk_gemm_workspace_bytes(m, n,
+ * precision) exceeds the array.  All calls on one device share that
+ * device's array: calls that overlap on one device (other threads or
+ * streams) race and must be serialised by the caller.  Like ax_rk_gemm it
+ * synchronises `stream` before returning. */
+#define AX_NOALLOC_SCRATCH_BYTES (32ll << 20)
+ax_status ax_rk_gemm_noalloc(const double *D, int64_t ld_d, const int32_t *rows,
+                             const double *Y, int64_t ld_y, int m, int64_t n, int precision,
+                             double *out, void *stream);
+
 /* ---- batched R^k tensor over many orbitals (the synthetic workload) ------ */
 
 /* All R^k(ab;cd), k = 0..kmax, over quartets whose pairs satisfy the
