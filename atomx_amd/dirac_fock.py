@@ -230,3 +230,112 @@ def _semi_direct():
 def _semi_exchange(block):
     from .convol import SEMISEPARABLE_EXCHANGE_NS, SEMISEPARABLE_MIXED_EXCHANGE_NS
     return SEMISEPARABLE_EXCHANGE_NS if block in ("pppp", "qqqq") else SEMISEPARABLE_MIXED_EXCHANGE_NS
+
+
+_BLOCK_NAMES = ("pppp", "pqpq", "qpqp", "qqqq")
+
+
+class FockEngine:
+    """The per-iteration Fock build of many atoms or ions, every target of
+    each, as one plan of five launches (``ax_fock_plan_*``: direct
+    densities, semiseparable chains, direct contractions, exchange and
+    assembly per output tile, symmetrisation and occupation divisor) —
+    methods/dirac_fock.py:1128-1319 for all (ion, target) pairs at once.
+
+    builders: ``FockBuilder`` per ion (grid, angular tables, banks).  The
+    coefficient stacks live in resident device buffers written by ``load``;
+    ``run`` launches on the current (or given) stream and leaves every
+    target's PP, QQ, PQ, QP blocks in resident outputs (``blocks``).  The
+    launches allocate nothing and copy nothing, so a caller can capture
+    ``run`` in a HIP graph."""
+
+    def __init__(self, builders, skip_pq_exchange=False):
+        from ._lib import load as _load
+        from .convol import SEMISEPARABLE_DIRECT_NS
+        dv.require_gpu()
+        lib = _load()
+        self.builders = list(builders)
+        self.skip = bool(skip_pq_exchange)
+        self.P, self.Q, self.H = [], [], []
+        meta, targets, direct, exchange, divisor = [], [], [], [], []
+        banks, factors = [], []
+        self._keep = []  # device tensors the plan points at
+        for fb in self.builders:
+            tab, cache = fb.tables, fb.cache
+            ns, n = cache.ns, fb.n_orb
+            K = tab.direct.shape[2]
+            self.P.append(dv.zeros((n, ns)))
+            self.Q.append(dv.zeros((n, ns)))
+            self.H.append(dv.zeros((n, 4, ns, ns)))
+            meta += [ns, cache.ksp, cache.ksq, n, n, K - 1, 1 if self.skip else 0]
+            targets += list(range(n))
+            direct.append(np.ascontiguousarray(tab.direct, dtype=np.float64).ravel())
+            exchange.append(np.ascontiguousarray(tab.exchange, dtype=np.float64).ravel())
+            divisor += [float(tab.divisor(t)) for t in range(n)]
+            semi = ns >= SEMISEPARABLE_DIRECT_NS
+            for b, block in enumerate(_BLOCK_NAMES):
+                for k in range(K):
+                    need_d = bool(np.any(tab.direct[:, :, k]))
+                    need_x = bool(np.any(tab.exchange[:, :, k])) and not (self.skip and b == 1)
+                    bank = None
+                    if need_x or (need_d and not semi):
+                        bank = cache.bank_device(k, block)[0]
+                        self._keep.append(bank)
+                    banks.append(dv.ptr(bank) if bank is not None else None)
+                    fac = cache.factors_device(k, block) if (need_d and semi) else None
+                    if fac is not None:
+                        self._keep.extend(fac)
+                    factors += [dv.ptr(x) for x in fac] if fac is not None else [None] * 5
+        import ctypes
+        vp = ctypes.c_void_p
+
+        def ptrs(seq):
+            return (vp * len(seq))(*[vp(x) if x is not None else vp(0) for x in seq])
+
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        f64 = ctypes.POINTER(ctypes.c_double)
+        self._host = [np.asarray(meta, np.int32), np.asarray(targets, np.int32),
+                      np.concatenate(direct), np.concatenate(exchange),
+                      np.asarray(divisor, np.float64)]
+        m, t, d, x, q = self._host
+        handle = vp()
+        call("ax_fock_plan_create", len(self.builders), m.ctypes.data_as(i32),
+             t.ctypes.data_as(i32), d.ctypes.data_as(f64), x.ctypes.data_as(f64),
+             q.ctypes.data_as(f64), ptrs([dv.ptr(a) for a in self.P]),
+             ptrs([dv.ptr(a) for a in self.Q]), ptrs([dv.ptr(a) for a in self.H]), ptrs(banks),
+             ptrs(factors), int(SEMISEPARABLE_DIRECT_NS), ctypes.byref(handle))
+        self._plan = handle
+        self.nbytes = int(lib.ax_fock_plan_bytes(handle))
+        self.rk_equivalents = int(lib.ax_fock_plan_rk_equivalents(handle))
+
+    def load(self, P_list, Q_list):
+        """Writes new coefficient stacks (n_orb, <= ns) into the resident
+        buffers (zero-padded to ns), on the current stream."""
+        for dst_P, dst_Q, P, Q in zip(self.P, self.Q, P_list, Q_list):
+            ns = dst_P.shape[1]
+            for dst, src in ((dst_P, P), (dst_Q, Q)):
+                src = np.asarray(src, dtype=np.float64)
+                buf = np.zeros(dst.shape)
+                m = min(src.shape[1], ns)
+                buf[:, :m] = src[:, :m]
+                dst.copy_(dv.f64(buf))
+
+    def run(self, stream=None):
+        call("ax_fock_plan_run", self._plan, dv.stream_handle(stream))
+        return self.H
+
+    def blocks(self):
+        """Per ion {target: [PP, QQ, PQ, QP]} device views of the outputs."""
+        return [{t: [H[t, 0], H[t, 1], H[t, 2], H[t, 3]] for t in range(H.shape[0])}
+                for H in self.H]
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            call("ax_fock_plan_destroy", self._plan)
+        self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001  (interpreter shutdown)
+            pass

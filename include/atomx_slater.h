@@ -394,6 +394,45 @@ ax_status ax_bilinear_stage2(const double *T, int ns, int ksr, int n_right, cons
 ax_status ax_fock_finalize(double *pp, double *qq, double *pq, double *qp, int ns, double norm,
                            int skip_pq, void *stream);
 
+/* ---- batched Fock build of many atoms or ions ----------------------------
+ * methods/dirac_fock.py:1128-1319 (_build_hf_matrix_dbsr) for every target
+ * of every ion in one plan: five launches per run (direct densities,
+ * semiseparable chains, direct contractions, exchange + assembly per output
+ * tile, symmetrisation and occupation divisor), capturable in a HIP graph.
+ * Per ion, in order, ion_meta_host holds 7 ints
+ *   ns, ksp, ksq, n_orb, n_targets, kmax, skip_pq_exchange
+ * and the other host arrays hold, ion after ion:
+ *   targets_host       (n_targets) orbital index of each target
+ *   direct_host        (n_targets, n_orb, kmax+1) angular direct weights
+ *   exchange_host      (n_targets, n_orb, kmax+1) angular exchange weights
+ *                      (both zero at ranks a target does not take)
+ *   divisor_host       (n_targets) occupation divisors (no division <= 1e-15)
+ *   P[i], Q[i]         device coefficient stacks (n_orb, ns), zero past nsp / nsq
+ *   H[i]               device output (n_targets, 4, ns, ns): PP, QQ, PQ, QP
+ *   banks_host         (4, kmax+1) device banks bankT of blocks pppp, pqpq,
+ *                      qpqp, qqqq (NULL where unused)
+ *   factors_host       (4, kmax+1, 5) device semiseparable factors (near,
+ *                      rising_l, falling_l, rising_r, falling_r) used for the
+ *                      direct term when ns >= semi_direct_ns (NULL otherwise).
+ * The plan keeps the device pointers: a run reads P, Q as they are at that
+ * moment and overwrites H.  ns >= 320 (the semiseparable exchange) and
+ * orders above 12 are refused with AX_ERR_VALUE. */
+typedef struct ax_fock_plan ax_fock_plan;
+ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
+                              const int32_t *targets_host, const double *direct_host,
+                              const double *exchange_host, const double *divisor_host,
+                              const double *const *P, const double *const *Q, double *const *H,
+                              const double *const *banks_host,
+                              const double *const *factors_host, int semi_direct_ns,
+                              ax_fock_plan **plan);
+ax_status ax_fock_plan_run(ax_fock_plan *plan, void *stream);
+/* device bytes the plan holds; (target, source, k, block) contractions with
+ * a nonzero weight, direct and exchange counted separately */
+int64_t ax_fock_plan_bytes(const ax_fock_plan *plan);
+int64_t ax_fock_plan_rk_equivalents(const ax_fock_plan *plan);
+/* waits for the device, then frees the plan */
+ax_status ax_fock_plan_destroy(ax_fock_plan *plan);
+
 /* ---- auxiliary-node Coulomb action and pair contraction ------------------
  * Sampled columns are node-major (n_nodes, ld) FP64, or complex FP64 as
  * interleaved (re, im) when is_complex (c-products, no conjugation). */
