@@ -59,7 +59,7 @@ SIGNATURES = {
     "ax_rk_gemm": (_int, [_vp, _i64, _vp, _vp, _i64, _int, _i64, _int, _vp, _vp, _vp]),
     "ax_rk_gemm_noalloc": (_int, [_vp, _i64, _vp, _vp, _i64, _int, _i64, _int, _vp, _vp]),
     "ax_fock_plan_create": (_int, [_int, _ip, _ip, _dp, _dp, _dp, _vp, _vp, _vp, _vp, _vp,
-                                   _int, ctypes.POINTER(_vp)]),
+                                   _int, _int, ctypes.POINTER(_vp)]),
     "ax_fock_plan_run": (_int, [_vp, _vp]),
     "ax_fock_plan_bytes": (_i64, [_vp]),
     "ax_fock_plan_rk_equivalents": (_i64, [_vp]),

@@ -34,7 +34,7 @@ class IonBatch:
     optionally P, Q initial coefficient stacks (n_orb, ns)).
     path: "engine" (one plan for all ions) or "launches" (per-ion builds)."""
 
-    def __init__(self, ions, skip_pq_exchange=False, path="engine"):
+    def __init__(self, ions, skip_pq_exchange=False, path="engine", target_group=4):
         import torch
         dv.require_gpu()
         if path not in ("engine", "launches"):
@@ -46,7 +46,8 @@ class IonBatch:
         self.graph = None
         self.out = None
         if path == "engine":
-            self.engine = FockEngine(self.builders, skip_pq_exchange=skip_pq_exchange)
+            self.engine = FockEngine(self.builders, skip_pq_exchange=skip_pq_exchange,
+                                     target_group=target_group)
             self.P, self.Q = self.engine.P, self.engine.Q
             self.rk_equivalents = self.engine.rk_equivalents
         else:

@@ -211,18 +211,21 @@ class ConvolCache:
 
     # -- reference-layout host views ------------------------------------------
     def get_rkb(self, k, itype="pppp"):
-        """(rkb (ns, ns, ks_left, ks_right) host array, ns, ks_left) (convol.py:1507-1570)."""
+        """(rkb (ns, ns, ks_left, ks_right) host array, ns, ks_left) (convol.py:1507-1570).
+        The array is read-only and remembers its resident device bank."""
         b, ns, ksl, _ = self.bank_device(k, itype)
-        return dv.host(b).transpose(2, 3, 0, 1), ns, ksl
+        return _resident(dv.host(b).transpose(2, 3, 0, 1), bankT=b), ns, ksl
 
     def get_rkb_t(self, k, itype="pppp"):
-        """Transposed bank rkbT[ip, jp, i, j] (convol.py:1572-1591)."""
+        """Transposed bank rkbT[ip, jp, i, j] (convol.py:1572-1591), read-only,
+        remembering its resident device bank."""
         b, ns, ksl, _ = self.bank_device(k, itype)
-        return dv.host(b), ns, ksl
+        return _resident(dv.host(b), device=b, bankT=b), ns, ksl
 
     def get_coulomb_factors(self, k, itype="pppp"):
-        """Host copies of (near, rising_l, falling_l, rising_r, falling_r) (convol.py:1613-1658)."""
-        return tuple(dv.host(x) for x in self.factors_device(k, itype))
+        """Host copies of (near, rising_l, falling_l, rising_r, falling_r)
+        (convol.py:1613-1658), read-only, remembering their device tensors."""
+        return tuple(_resident(dv.host(x), device=x) for x in self.factors_device(k, itype))
 
     def get_symmetric_exchange_factors(self, k, itype="pppp"):
         if itype not in ("pppp", "qqqq"):
@@ -502,3 +505,260 @@ def compute_slater_rk_table(M, CP_l, CQ_l, CP_r, CQ_r):
     M_PP, M_QQ, M_PQ, M_QP = M
     return (CP_l.T @ M_PP @ CP_r + CQ_l.T @ M_QQ @ CQ_r
             + CP_l.T @ M_PQ @ CQ_r + CQ_l.T @ M_QP @ CP_r)
+
+
+# ---------------------------------------------------------------------------
+# private kernels of the reference module, by name and argument order
+# (convol.py:609-1373): numpy in, numpy out (or in place, as the reference),
+# every step on the device.  Banks and factors handed out by
+# ConvolCache.get_rkb / get_rkb_t / get_coulomb_factors carry their resident
+# device tensor, so passing them back costs no upload.
+# ---------------------------------------------------------------------------
+
+class _Resident(np.ndarray):
+    """Read-only host copy of a device-resident bank or factor array that
+    remembers the device tensor it came from (views and copies do not)."""
+
+    def __array_finalize__(self, obj):
+        self._ax_device = None
+        self._ax_bankT = None
+
+
+def _resident(host, device=None, bankT=None):
+    h = np.asarray(host).view(_Resident)
+    h._ax_device = device
+    h._ax_bankT = bankT
+    h.flags.writeable = False
+    return h
+
+
+def _dev(a):
+    """Device FP64 copy of a host operand, or the tensor a resident array came from."""
+    d = getattr(a, "_ax_device", None)
+    if d is not None:
+        return d
+    return dv.f64(np.ascontiguousarray(a, dtype=np.float64))
+
+
+def _bankT_dev(bank, transposed):
+    """Device bankT (ksl, ksr, ns, ns) of a bank given as rkbT[ip, jp, i, j]
+    (transposed) or rkb[i, j, ip, jp]."""
+    d = getattr(bank, "_ax_bankT", None)
+    if d is not None:
+        return d
+    b = np.asarray(bank, dtype=np.float64)
+    if not transposed:
+        b = b.transpose(2, 3, 0, 1)
+    return dv.f64(np.ascontiguousarray(b))
+
+
+def _rows(vectors, width):
+    """Coefficient vectors as zero-padded rows of one device matrix."""
+    out = np.zeros((len(vectors), max(width, 1)))
+    for r, v in enumerate(vectors):
+        v = np.asarray(v, dtype=np.float64).ravel()
+        m = min(v.size, out.shape[1])
+        out[r, :m] = v[:m]
+    return dv.f64(out)
+
+
+def _check_bank(b, ksl, ksr, ns, name):
+    if tuple(b.shape) != (ksl, ksr, ns, ns):
+        raise ValueError(f"{name}: bank of shape {tuple(b.shape)}, expected "
+                         f"({ksl}, {ksr}, {ns}, {ns}) in the transposed layout")
+
+
+def _into(dst, dev_result):
+    """Writes a device result into the caller's array in place."""
+    dst[...] = dv.host(dev_result).reshape(dst.shape)
+
+
+def _density_sym(p1, p2, ns, ks):
+    """convol.py:712-753: banded symmetric density (ns, ks), reads past
+    either vector's length are zero."""
+    p1, p2 = np.asarray(p1).ravel(), np.asarray(p2).ravel()
+    A = _rows([p1, p2], max(p1.size, p2.size, ns))
+    return dv.host(density_sym_device(A, [(0, 1)], ns, ks, p1.size, p2.size))[0]
+
+
+def _density_full(p1, p2, ns):
+    """convol.py:756-771: d[i, j] = p1[i] p2[j] (ns, ns), zero past the lengths."""
+    A, B = _rows([p1], ns), _rows([p2], ns)
+    out = dv.empty((1, ns, ns))
+    one = dv.f64(np.ones((1, 1)))
+    call("ax_exchange_density", dv.ptr(A), dv.ptr(B), dv.ld(A), 1, ns, dv.ptr(one), 1,
+         dv.ptr(out), _s())
+    return dv.host(out)[0]
+
+
+def _accumulate_density_sym(dd, p1, p2, ns, ks, coeff):
+    """convol.py:609-630: dd += coeff * _density_sym(p1, p2) (in place)."""
+    d = dv.f64(np.ascontiguousarray(dd, dtype=np.float64))
+    axpy(d, dv.f64(_density_sym(p1, p2, ns, ks)).reshape(d.shape), float(coeff))
+    _into(dd, d)
+
+
+def _accumulate_density_full(xx, p1, p2, ns, coeff):
+    """convol.py:633-643: xx += coeff * outer(p1, p2) (in place)."""
+    x = dv.f64(np.ascontiguousarray(xx, dtype=np.float64))
+    axpy(x, dv.f64(_density_full(p1, p2, ns)).reshape(x.shape), float(coeff))
+    _into(xx, x)
+
+
+def _accumulate_direct_density_batch(dd, comp_stack, coeffs, target_i, ns, ks):
+    """convol.py:646-685: dd += sum_j c_j _density_sym(p_j, p_j), the
+    target's own coefficient doubled (in place)."""
+    comp = np.asarray(comp_stack, dtype=np.float64)
+    c = np.array(coeffs, dtype=np.float64).ravel()
+    if 0 <= int(target_i) < c.size:
+        c[int(target_i)] = c[int(target_i)] + c[int(target_i)]
+    n_src = comp.shape[0]
+    A = _rows(list(comp), max(comp.shape[1], ns))
+    dens = density_sym_device(A, [(j, j) for j in range(n_src)], ns, ks,
+                              min(comp.shape[1], ns), min(comp.shape[1], ns))
+    acc = dv.empty((1, ns, ks))
+    call("ax_combine_rows", dv.ptr(dens), ns * ks, n_src, dv.ptr(dv.f64(c[None, :])), 1,
+         dv.ptr(acc), _s())
+    d = dv.f64(np.ascontiguousarray(dd, dtype=np.float64))
+    axpy(d, acc.reshape(d.shape))
+    _into(dd, d)
+
+
+def _accumulate_exchange_density_batch(xx, comp_ii_stack, comp_jj_stack, coeffs, ns):
+    """convol.py:688-709: xx += sum_j (b_j A_j) B_j^T (in place)."""
+    A = _rows(list(np.asarray(comp_ii_stack)), ns)
+    B = _rows(list(np.asarray(comp_jj_stack)), ns)
+    c = dv.f64(np.asarray(coeffs, dtype=np.float64).ravel()[None, :])
+    out = dv.empty((1, ns, ns))
+    call("ax_exchange_density", dv.ptr(A), dv.ptr(B), dv.ld(A), A.shape[0], ns, dv.ptr(c), 1,
+         dv.ptr(out), _s())
+    x = dv.f64(np.ascontiguousarray(xx, dtype=np.float64))
+    axpy(x, out.reshape(x.shape))
+    _into(xx, x)
+
+
+def _banded_to_full_add(V, v_banded, ns, ks, coeff):
+    """convol.py:814-825: V[i, i+s] and V[i+s, i] += coeff v_banded[i, s] (in place)."""
+    if np.shape(V) != (ns, ns):
+        raise ValueError(f"_banded_to_full_add: V must be ({ns}, {ns})")
+    Vd = dv.f64(np.ascontiguousarray(V, dtype=np.float64))
+    vb = dv.f64(np.ascontiguousarray(v_banded, dtype=np.float64)[:ns, :ks])
+    call("ax_banded_to_full_add", dv.ptr(Vd), ns * ns, dv.ptr(vb), ns * ks, 1, ns, ks,
+         float(coeff), _s())
+    _into(V, Vd)
+
+
+def _convol_case1_t(rkbT, d_right, ns, ks_left, ks_right):
+    """convol.py:971-997: a[i, ip] = sum rkbT[ip, jp, i, j] d[j, jp] -> (ns, ks_left)."""
+    b = _bankT_dev(rkbT, True)
+    _check_bank(b, ks_left, ks_right, ns, "_convol_case1_t")
+    d = dv.f64(np.ascontiguousarray(d_right, dtype=np.float64))
+    out = dv.empty((1, ns, ks_left))
+    call("ax_convol_case1", dv.ptr(b), ns, ks_left, ks_right, dv.ptr(d), ns * ks_right, 1,
+         dv.ptr(out), ns * ks_left, _s())
+    return dv.host(out)[0]
+
+
+def _convol_case1(rkb, d, ns, ks):
+    """convol.py:788-811 (bank rkb[i, j, ip, jp])."""
+    return _convol_case1_t(_bankT_dev(rkb, False), d, ns, ks, ks)
+
+
+def _convol_case1_mixed(rkb, d_right, ns, ks_left, ks_right):
+    """convol.py:828-849 (bank rkb[i, j, ip, jp])."""
+    return _convol_case1_t(_bankT_dev(rkb, False), d_right, ns, ks_left, ks_right)
+
+
+def _convol_case2_mixed(rkb, d_left, ns, ks_left, ks_right):
+    """convol.py:1310-1345: a[j, jp] = sum rkb[i, j, ip, jp] d_left[i, ip] over
+    i + ip < n_left (the density's own row count) -> (ns, ks_right)."""
+    b = _bankT_dev(rkb, False)
+    _check_bank(b, ks_left, ks_right, ns, "_convol_case2_mixed")
+    dl = np.ascontiguousarray(d_left, dtype=np.float64)
+    n_left = dl.shape[0]
+    d = dv.f64(dl)
+    out = dv.empty((1, ns, ks_right))
+    call("ax_convol_case2", dv.ptr(b), ns, ks_left, ks_right, n_left, dv.ptr(d),
+         n_left * ks_left, 1, dv.ptr(out), ns * ks_right, _s())
+    return dv.host(out)[0]
+
+
+def _convol_case2(rkb, d, ns, ks):
+    """convol.py:852-874."""
+    return _convol_case2_mixed(rkb, np.asarray(d)[:ns], ns, ks, ks)
+
+
+def _convol_case4_t(rkbT, d, ns, ks, ks_right=None, symmetric_output=False):
+    """convol.py:1000-1058: exchange contraction of a full density (ns, ns)."""
+    ksr = ks if ks_right is None else ks_right
+    b = _bankT_dev(rkbT, True)
+    _check_bank(b, ks, ksr, ns, "_convol_case4_t")
+    x = dv.f64(np.ascontiguousarray(np.asarray(d, dtype=np.float64)[:ns, :ns]))
+    out = dv.empty((1, ns, ns))
+    call("ax_convol_case4", dv.ptr(b), ns, ks, ksr, dv.ptr(x), ns * ns, 1,
+         1 if symmetric_output else 0, dv.ptr(out), ns * ns, _s())
+    return dv.host(out)[0]
+
+
+def _convol_case4(rkb, d, ns, ks, ks_right=None):
+    """convol.py:877-948 (bank rkb[i, j, ip, jp])."""
+    ksr = ks if ks_right is None else ks_right
+    return _convol_case4_t(_bankT_dev(rkb, False), d, ns, ks, ksr, False)
+
+
+def _convol_case1_semiseparable_t(near, rising_left, falling_left, rising_right,
+                                  falling_right, density, ns, ks_left, ks_right):
+    """convol.py:1125-1162: the direct contraction from pair moments and the
+    overlap strip -> (ns, ks_left)."""
+    f = [_dev(x) for x in (near, rising_left, falling_left, rising_right, falling_right)]
+    d = dv.f64(np.ascontiguousarray(density, dtype=np.float64))
+    ws = dv.empty((max(1, int(load().ax_convol_semi_workspace(1, ns, ks_right))),))
+    out = dv.empty((1, ns, ks_left))
+    call("ax_convol_case1_semi", *[dv.ptr(x) for x in f], ns, ks_left, ks_right, dv.ptr(d),
+         ns * ks_right, 1, dv.ptr(ws), dv.ptr(out), ns * ks_left, _s())
+    return dv.host(out)[0]
+
+
+def _convol_case4_semi_common(factors, density, ns, ksl, ksr, symmetric):
+    f = [_dev(x) for x in factors]
+    x = dv.f64(np.ascontiguousarray(np.asarray(density, dtype=np.float64)[:ns, :ns]))
+    out = dv.empty((1, ns, ns))
+    call("ax_convol_case4_semi", *[dv.ptr(t) for t in f], ns, ksl, ksr, dv.ptr(x), ns * ns, 1,
+         1 if symmetric else 0, dv.ptr(out), ns * ns, _s())
+    return dv.host(out)[0]
+
+
+def _convol_case4_semiseparable_t(near, rising, falling, density, ns, ks):
+    """convol.py:1187-1222: symmetric exchange from pair moments and local cells."""
+    return _convol_case4_semi_common((near, rising, falling, rising, falling), density, ns, ks,
+                                     ks, True)
+
+
+def _convol_case4_semiseparable_mixed_t(near, rising_left, falling_left, rising_right,
+                                        falling_right, density, ns, ks_left, ks_right):
+    """convol.py:1250-1303: mixed-component exchange from pair moments."""
+    return _convol_case4_semi_common((near, rising_left, falling_left, rising_right,
+                                      falling_right), density, ns, ks_left, ks_right, False)
+
+
+def _sum_amb_banded(a, b, ns, ks):
+    """convol.py:1348-1356: sum_{i, s; i+s < ns} a[i, s] b[i, s]."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    ad, bd = dv.f64(a[None]), dv.f64(b[None])
+    out = dv.empty((1,))
+    call("ax_sum_amb_banded", dv.ptr(ad), a.size, a.shape[1], dv.ptr(bd), b.size, b.shape[1], 1,
+         ns, ks, dv.ptr(out), _s())
+    return float(dv.host(out)[0])
+
+
+def _sum_amb_full(a, b, ns):
+    """convol.py:1359-1373: sum_{i, j < ns} a[i, j] b[i, j]."""
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64)[:ns, :ns]).reshape(-1, 1)
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.float64)[:ns, :ns]).reshape(-1, 1)
+    ad, bd = dv.f64(a), dv.f64(b)
+    idx = dv.i32(np.zeros(1, dtype=np.int32))
+    out = dv.empty((1,))
+    call("ax_feat_dots", dv.ptr(ad), 1, dv.ptr(bd), 1, a.shape[0], 1, dv.ptr(idx), dv.ptr(idx), 0,
+         dv.ptr(out), _s())
+    return float(dv.host(out)[0])

@@ -232,13 +232,36 @@ __global__ __launch_bounds__(kThreads) void fock_direct_kernel(const DirJob *__r
 }
 
 // ---- D: assembly of the quadrants ----------------------------------------------------
-// LDS: the exchange densities of the target group on the tile plus a halo of
-// ks - 1 rows / columns on each side, xx[g][lr][lc] with stride xc.
+// LDS (doubles): the exchange densities of the target group on the tile plus
+// a halo of ks - 1 rows / columns on each side (xx[g][lr][lc], planes of
+// xx_plane), then the rows of the bra / ket coefficient stacks over the
+// halo (sa[s][lr], sb[s][lc]) and the group's coefficients (sc[g][s]).
+struct AsmLds {
+    int xx_plane, n_orb_max, xr_max, xc_max, g_max, pad;
+};
+
+__device__ inline void bank_values(const double *__restrict__ bank, int ns, int ksr, int q,
+                                   int row, int col, double &b1, double &b2, double &b3,
+                                   double &b4) {
+    const int ip = q / ksr, jp = q - (q / ksr) * ksr;
+    const double *base = bank + ((int64_t)ip * ksr + jp) * ns * ns;
+    const bool up = row >= ip, down = ip >= 1 && row + ip < ns;
+    const bool right = col + jp < ns, left = jp >= 1 && col >= jp;
+    // the four index families of the exchange contraction
+    b1 = (up && right) ? base[(int64_t)(row - ip) * ns + col] : 0.0;
+    b2 = (down && right) ? base[(int64_t)row * ns + col] : 0.0;
+    b3 = (up && left) ? base[(int64_t)(row - ip) * ns + col - jp] : 0.0;
+    b4 = (down && left) ? base[(int64_t)row * ns + col - jp] : 0.0;
+}
+
 __global__ __launch_bounds__(kThreads) void fock_assemble_kernel(
-    const AsmJob *__restrict__ jobs, const AsmEntry *__restrict__ entries) {
-    extern __shared__ double xx[];
+    const AsmJob *__restrict__ jobs, const AsmEntry *__restrict__ entries, const AsmLds geo) {
+    extern __shared__ double lds[];
     const AsmJob J = jobs[blockIdx.x];
     const int ns = J.ns, n_orb = J.n_orb, tg = J.tg, t0 = J.t0;
+    double *sa = lds + geo.g_max * geo.xx_plane;
+    double *sb = sa + geo.n_orb_max * geo.xr_max;
+    double *sc = sb + geo.n_orb_max * geo.xc_max;
     const int tid = threadIdx.x;
     const int row = J.r0 + tid / kTile, col = J.c0 + tid % kTile;
     const bool active = row < ns && col < ns && (!J.symmetric || col <= row);
@@ -260,21 +283,34 @@ __global__ __launch_bounds__(kThreads) void fock_assemble_kernel(
         const int ksl = E.ksl, ksr = E.ksr;
         const int xr = kTile + 2 * (ksl - 1), xc = kTile + 2 * (ksr - 1);
         const int plane = xr * xc;
+        const int rb = J.r0 - (ksl - 1), cb = J.c0 - (ksr - 1);
         __syncthreads();  // the previous entry's LDS reads are done
+        for (int idx = tid; idx < n_orb * xr; idx += kThreads) {
+            const int sx = idx / xr, i = idx - (idx / xr) * xr;
+            const int gi = rb + i;
+            sa[sx * geo.xr_max + i] = (gi >= 0 && gi < ns) ? E.A[(int64_t)sx * ns + gi] : 0.0;
+        }
+        for (int idx = tid; idx < n_orb * xc; idx += kThreads) {
+            const int sx = idx / xc, j = idx - (idx / xc) * xc;
+            const int gj = cb + j;
+            sb[sx * geo.xc_max + j] = (gj >= 0 && gj < ns) ? E.B[(int64_t)sx * ns + gj] : 0.0;
+        }
+        for (int idx = tid; idx < tg * n_orb; idx += kThreads) {
+            const int g = idx / n_orb, sx = idx - (idx / n_orb) * n_orb;
+            sc[g * geo.n_orb_max + sx] = E.coef[(int64_t)(t0 + g) * n_orb + sx];
+        }
+        __syncthreads();
+        // exchange densities of the group: sum_s (b(t, s) A_s[i]) B_s[j]
         for (int idx = tid; idx < tg * plane; idx += kThreads) {
             const int g = idx / plane, rem = idx - (idx / plane) * plane;
             const int i = rem / xc, jx = rem - (rem / xc) * xc;
-            const int gi = J.r0 - (ksl - 1) + i, gj = J.c0 - (ksr - 1) + jx;
             double val = 0.0;
-            if (gi >= 0 && gi < ns && gj >= 0 && gj < ns) {
-                const double *c = E.coef + (int64_t)(t0 + g) * n_orb;
-                for (int s = 0; s < n_orb; ++s) {
-                    const double cs = c[s];
-                    if (cs == 0.0) continue;
-                    val += (cs * E.A[(int64_t)s * ns + gi]) * E.B[(int64_t)s * ns + gj];
-                }
+            for (int sx = 0; sx < n_orb; ++sx) {
+                const double cs = sc[g * geo.n_orb_max + sx];
+                if (cs == 0.0) continue;
+                val += (cs * sa[sx * geo.xr_max + i]) * sb[sx * geo.xc_max + jx];
             }
-            xx[idx] = val;
+            lds[g * geo.xx_plane + i * xc + jx] = val;
         }
         __syncthreads();
         if (!active) continue;
@@ -282,26 +318,27 @@ __global__ __launch_bounds__(kThreads) void fock_assemble_kernel(
 #pragma unroll
         for (int g = 0; g < kMaxG; ++g) part[g] = 0.0;
         const int lr = row - J.r0 + (ksl - 1), lc = col - J.c0 + (ksr - 1);
-        for (int ip = 0; ip < ksl; ++ip) {
-            const bool up = row >= ip, down = ip >= 1 && row + ip < ns;
-            for (int jp = 0; jp < ksr; ++jp) {
-                const double *base = E.bank + ((int64_t)ip * ksr + jp) * ns * ns;
-                const bool right = col + jp < ns, left = jp >= 1 && col >= jp;
-                // the four index families of the exchange contraction
-                const double b1 = (up && right) ? base[(int64_t)(row - ip) * ns + col] : 0.0;
-                const double b2 = (down && right) ? base[(int64_t)row * ns + col] : 0.0;
-                const double b3 = (up && left) ? base[(int64_t)(row - ip) * ns + col - jp] : 0.0;
-                const double b4 = (down && left) ? base[(int64_t)row * ns + col - jp] : 0.0;
-                const int o1 = (lr - ip) * xc + lc + jp, o2 = (lr + ip) * xc + lc + jp;
-                const int o3 = (lr - ip) * xc + lc - jp, o4 = (lr + ip) * xc + lc - jp;
+        const int nq = ksl * ksr;
+        double b1, b2, b3, b4;
+        bank_values(E.bank, ns, ksr, 0, row, col, b1, b2, b3, b4);
+        for (int q = 0; q < nq; ++q) {
+            // the next band pair's bank values are in flight during this one's FMAs
+            double n1 = 0.0, n2 = 0.0, n3 = 0.0, n4 = 0.0;
+            if (q + 1 < nq) bank_values(E.bank, ns, ksr, q + 1, row, col, n1, n2, n3, n4);
+            const int ip = q / ksr, jp = q - (q / ksr) * ksr;
+            const int o1 = (lr - ip) * xc + lc + jp, o2 = (lr + ip) * xc + lc + jp;
+            const int o3 = (lr - ip) * xc + lc - jp, o4 = (lr + ip) * xc + lc - jp;
 #pragma unroll
-                for (int g = 0; g < kMaxG; ++g) {
-                    if (g < tg) {
-                        const double *x = xx + g * plane;
-                        part[g] += b1 * x[o1] + b2 * x[o2] + b3 * x[o3] + b4 * x[o4];
-                    }
+            for (int g = 0; g < kMaxG; ++g) {
+                if (g < tg) {
+                    const double *x = lds + g * geo.xx_plane;
+                    part[g] += b1 * x[o1] + b2 * x[o2] + b3 * x[o3] + b4 * x[o4];
                 }
             }
+            b1 = n1;
+            b2 = n2;
+            b3 = n3;
+            b4 = n4;
         }
 #pragma unroll
         for (int g = 0; g < kMaxG; ++g) acc[g] += part[g];
@@ -366,6 +403,8 @@ struct ax_fock_plan {
     int n_dd = 0, n_chains = 0, n_dir = 0, n_asm = 0, n_fin = 0;
     int max_chain = 0;
     size_t asm_lds = 0;
+    AsmLds geo{};
+    int dir_blocks = 1;
     int64_t bytes = 0;
     int64_t rk_equivalents = 0;
 };
@@ -378,11 +417,14 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
                               const double *const *P, const double *const *Q, double *const *H,
                               const double *const *banks_host,
                               const double *const *factors_host, int semi_direct_ns,
-                              ax_fock_plan **plan_out) {
+                              int max_group, ax_fock_plan **plan_out) {
     AX_REQUIRE(n_ions >= 1 && ion_meta_host && targets_host && direct_host && exchange_host &&
                    divisor_host && P && Q && H && banks_host && factors_host && plan_out,
                "ax_fock_plan_create: missing argument");
     *plan_out = nullptr;
+    if (max_group <= 0) max_group = kMaxG;
+    AX_REQUIRE(max_group <= kMaxG, "ax_fock_plan_create: target groups of at most %d, got %d",
+               kMaxG, max_group);
     // host images of every table, with offsets into one device block
     struct Buf {
         size_t off, bytes;
@@ -410,7 +452,7 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
     const double *const *bptr = banks_host;
     const double *const *fptr = factors_host;
     int64_t rk_eq = 0;
-    size_t max_lds = 0;
+    int xr_max = 1, xc_max = 1, n_orb_max = 1, g_max = 1;
     // a device-pointer placeholder: the byte offset + 1 (0 stays "none")
     auto ph = [](size_t off) { return reinterpret_cast<double *>(off + 1); };
 
@@ -446,7 +488,9 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
 
         const bool semi = ns >= semi_direct_ns;
         // quadrant entry lists per target group: PP, QQ, PQ, QP
-        const int n_groups = (nt + kMaxG - 1) / kMaxG;
+        // targets in balanced groups of at most max_group
+        const int n_groups = (nt + max_group - 1) / max_group;
+        const int g_size = (nt + n_groups - 1) / n_groups;
         std::vector<std::vector<std::vector<AsmEntry>>> qe(
             4, std::vector<std::vector<AsmEntry>>(n_groups));
         const int blocks[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};  // (bra, ket): 0 P, 1 Q
@@ -465,11 +509,11 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
                         const double a = dcoef[((size_t)t * n_orb + j) * K + k];
                         const double x = xcoef[((size_t)t * n_orb + j) * K + k];
                         if (a != 0.0) {
-                            any_d = gd[t / kMaxG] = true;
+                            any_d = gd[t / g_size] = true;
                             ++rk_eq;
                         }
                         if (x != 0.0 && !(skip && b == 1)) {
-                            any_x = gx[t / kMaxG] = true;
+                            any_x = gx[t / g_size] = true;
                             ++rk_eq;
                         }
                     }
@@ -555,9 +599,10 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
                     E.B = ket;
                     E.ksl = ks_bra;
                     E.ksr = ks_ket;
-                    const size_t lds = (size_t)kMaxG * (kTile + 2 * (ks_bra - 1)) *
-                                       (kTile + 2 * (ks_ket - 1)) * sizeof(double);
-                    max_lds = std::max(max_lds, lds);
+                    xr_max = std::max(xr_max, kTile + 2 * (ks_bra - 1));
+                    xc_max = std::max(xc_max, kTile + 2 * (ks_ket - 1));
+                    n_orb_max = std::max(n_orb_max, n_orb);
+                    g_max = std::max(g_max, g_size);
                     for (int g = 0; g < n_groups; ++g)
                         if (gx[g]) qe[exch_q][g].push_back(E);
                 }
@@ -584,8 +629,8 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
                         J.symmetric = symmetric ? 1 : 0;
                         J.r0 = tr * kTile;
                         J.c0 = tc * kTile;
-                        J.t0 = g * kMaxG;
-                        J.tg = std::min(kMaxG, nt - g * kMaxG);
+                        J.t0 = g * g_size;
+                        J.tg = std::min(g_size, nt - g * g_size);
                         J.e0 = e0;
                         J.e1 = e1;
                         asmj.push_back(J);
@@ -602,6 +647,12 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
         F.skip = skip;
         fin.push_back(F);
     }
+    AsmLds geo{xr_max * xc_max, n_orb_max, xr_max, xc_max, g_max, 0};
+    // g_max planes of the exchange densities, the two coefficient strips and
+    // the group's coefficients
+    const size_t max_lds = sizeof(double) * ((size_t)g_max * geo.xx_plane +
+                                             (size_t)n_orb_max * (xr_max + xc_max) +
+                                             (size_t)g_max * n_orb_max);
     AX_REQUIRE(max_lds <= 160 * 1024, "ax_fock_plan_create: LDS need %zu exceeds the CU", max_lds);
 
     // layout of the device block: coefficients, job tables, work buffers
@@ -688,6 +739,14 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
     plan->n_fin = static_cast<int>(fin.size());
     for (const ChainJob &B : chains) plan->max_chain = std::max(plan->max_chain, B.nt * B.ksr);
     plan->asm_lds = max_lds;
+    plan->geo = geo;
+    for (const DirJob &C : dir) {
+        // one bank row per wave (4 per work-group), or one element per thread
+        const int64_t units = C.mode == 0 ? ((int64_t)C.ns * C.ksl + 3) / 4
+                                          : ((int64_t)C.ns * C.ksl * C.nt + kThreads - 1) / kThreads;
+        plan->dir_blocks = static_cast<int>(std::min<int64_t>(std::max<int64_t>(
+            plan->dir_blocks, units), 1024));
+    }
     plan->bytes = static_cast<int64_t>(off);
     plan->rk_equivalents = rk_eq;
     if (max_lds > 64 * 1024)
@@ -714,13 +773,15 @@ ax_status ax_fock_plan_run(ax_fock_plan *plan, void *stream) {
         AX_CHECK_LAUNCH();
     }
     if (plan->n_dir > 0) {
-        hipLaunchKernelGGL(fock_direct_kernel, dim3(16, plan->n_dir), dim3(kThreads), 0, s,
-                           plan->dir);
+        // enough waves per job that each bank row (or semiseparable element)
+        // chain is short: the kernel is latency-bound, not bandwidth-bound
+        hipLaunchKernelGGL(fock_direct_kernel, dim3(plan->dir_blocks, plan->n_dir), dim3(kThreads),
+                           0, s, plan->dir);
         AX_CHECK_LAUNCH();
     }
     if (plan->n_asm > 0) {
         hipLaunchKernelGGL(fock_assemble_kernel, dim3(plan->n_asm), dim3(kThreads),
-                           plan->asm_lds, s, plan->asmj, plan->entries);
+                           plan->asm_lds, s, plan->asmj, plan->entries, plan->geo);
         AX_CHECK_LAUNCH();
     }
     if (plan->n_fin > 0) {

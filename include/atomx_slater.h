@@ -414,6 +414,8 @@ ax_status ax_fock_finalize(double *pp, double *qq, double *pq, double *qp, int n
  *   factors_host       (4, kmax+1, 5) device semiseparable factors (near,
  *                      rising_l, falling_l, rising_r, falling_r) used for the
  *                      direct term when ns >= semi_direct_ns (NULL otherwise).
+ * max_group (1..8, 0 = 8) bounds the targets one assembly work-group holds
+ * (balanced groups): fewer raise occupancy, more read each bank less often.
  * The plan keeps the device pointers: a run reads P, Q as they are at that
  * moment and overwrites H.  ns >= 320 (the semiseparable exchange) and
  * orders above 12 are refused with AX_ERR_VALUE. */
@@ -424,7 +426,7 @@ ax_status ax_fock_plan_create(int n_ions, const int32_t *ion_meta_host,
                               const double *const *P, const double *const *Q, double *const *H,
                               const double *const *banks_host,
                               const double *const *factors_host, int semi_direct_ns,
-                              ax_fock_plan **plan);
+                              int max_group, ax_fock_plan **plan);
 ax_status ax_fock_plan_run(ax_fock_plan *plan, void *stream);
 /* device bytes the plan holds; (target, source, k, block) contractions with
  * a nonzero weight, direct and exchange counted separately */
