@@ -16,12 +16,13 @@ BIN=$ROOT/atomx_amd/_lib/rk_tensor_bench
 [ -x "$BIN" ] || { echo "missing $BIN: run python -m atomx_amd.native_bench.build first"; exit 1; }
 (cd "$ROOT" && python3 -m atomx_amd.native_bench.write_workload "$OUT/w.bin") || exit 1
 cd /tmp && export TMPDIR=/tmp
-pass() {  # name, precision, splits, profiler options...
-    local name=$1 prec=$2 splits=$3
-    shift 3
+pass() {  # name, precision, profiler options...
+    local name=$1 prec=$2
+    shift 2
     echo "== $name (start)"
+    # order -2: every (order, K split) the schedule tuner times, 3 + 10 runs each
     timeout -s KILL 120 rocprofv3 "$@" --output-format csv -d "$OUT/$name" -o nb -- \
-        "$BIN" "$OUT/w.bin" "$prec" 1 "$splits" 10 256 > "$OUT/$name.log" 2>&1
+        "$BIN" "$OUT/w.bin" "$prec" -2 0 10 256 > "$OUT/$name.log" 2>&1
     local rc=$?
     grep -E "^workload|^copy" "$OUT/$name.log"
     if [ $rc -eq 0 ]; then return 0; fi
@@ -38,9 +39,8 @@ pass() {  # name, precision, splits, profiler options...
 }
 MFMA="SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_VALU_MFMA_MOPS_I8 SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE"
 for prec in fp64 split; do
-    splits=6,7,8,9,10,12,4:8  # order 1 at these K splits, then the balanced order at level 8
-    pass "$prec-trace" $prec $splits --kernel-trace --stats &&
-    pass "$prec-fetch" $prec $splits --pmc FETCH_SIZE &&
-    pass "$prec-write" $prec $splits --pmc WRITE_SIZE &&
-    pass "$prec-mfma" $prec $splits --pmc $MFMA || exit 1
+    pass "$prec-trace" $prec --kernel-trace --stats &&
+    pass "$prec-fetch" $prec --pmc FETCH_SIZE &&
+    pass "$prec-write" $prec --pmc WRITE_SIZE &&
+    pass "$prec-mfma" $prec --pmc $MFMA || exit 1
 done

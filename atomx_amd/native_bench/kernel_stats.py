@@ -1,45 +1,76 @@
-"""Per-kernel statistics from a rocprofv3 database (the ``*_results.db``
-that ``rocprofv3 --kernel-trace`` writes): calls, total and average
-duration per kernel name, largest total first, as a Markdown table.
+"""Per-kernel statistics from a rocprofv3 kernel trace (the ``*_results.db``
+database, or the ``*_kernel_trace.csv`` of ``--output-format csv``): calls,
+total and average duration per kernel name, largest total first, as a
+Markdown table.
 
-    python -m atomx_amd.native_bench.kernel_stats RESULTS.db [--match TEXT] [--top N]
+    python -m atomx_amd.native_bench.kernel_stats TRACE [--match TEXT] [--top N]
+                                                  [--last N] [--per N]
+
+--last N keeps only the last N dispatches of each kernel (a timed loop that
+follows a tuning sweep of the same kernels); --per N adds a column with the
+total divided by N (time per SCF sweep, per request, ...).
 """
 
 from __future__ import annotations
 
 import argparse
+import csv
 import re
 import sqlite3
 
 
-def kernel_stats(db_path, match=None):
-    con = sqlite3.connect(db_path)
-    rows = con.execute("select name, duration from kernels").fetchall()
+def _short(name):
+    s = name.replace("(anonymous namespace)::", "")
+    s = re.sub(r"^void ", "", s)
+    return re.sub(r"\(.*", "", s)
+
+
+def dispatches(path):
+    """[(start ns, short kernel name, duration ns)] of every kernel dispatch."""
+    if path.endswith(".csv"):
+        out = []
+        for r in csv.DictReader(open(path)):
+            t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            out.append((t0, _short(r["Kernel_Name"]), t1 - t0))
+        return out
+    con = sqlite3.connect(path)
+    rows = con.execute("select start, name, duration from kernels").fetchall()
     con.close()
-    stats = {}
-    for name, dur in rows:
-        short = name.replace("(anonymous namespace)::", "")
-        short = re.sub(r"^void ", "", short)
-        short = re.sub(r"\(.*", "", short)
-        if match and match not in short:
+    return [(int(t0), _short(name), int(d)) for t0, name, d in rows]
+
+
+def kernel_stats(path, match=None, last=0):
+    by_name = {}
+    for t0, name, dur in sorted(dispatches(path)):
+        if match and match not in name:
             continue
-        n, tot = stats.get(short, (0, 0.0))
-        stats[short] = (n + 1, tot + float(dur))
-    return sorted(((k, n, tot) for k, (n, tot) in stats.items()), key=lambda r: -r[2])
+        by_name.setdefault(name, []).append(dur)
+    rows = []
+    for name, durs in by_name.items():
+        if last:
+            durs = durs[-last:]
+        rows.append((name, len(durs), float(sum(durs))))
+    return sorted(rows, key=lambda r: -r[2])
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("db")
+    ap.add_argument("trace")
     ap.add_argument("--match", default=None)
     ap.add_argument("--top", type=int, default=25)
+    ap.add_argument("--last", type=int, default=0)
+    ap.add_argument("--per", type=float, default=0.0)
     a = ap.parse_args()
-    rows = kernel_stats(a.db, a.match)
+    rows = kernel_stats(a.trace, a.match, a.last)
     total = sum(r[2] for r in rows)
-    print("| kernel | calls | total us | avg us | share |")
-    print("|---|---:|---:|---:|---:|")
+    per = f" | us per 1/{a.per:g}" if a.per else ""
+    print(f"| kernel | calls | total us | avg us | share{per} |")
+    print("|---|---:|---:|---:|---:|" + ("---:|" if a.per else ""))
     for name, n, tot in rows[:a.top]:
-        print(f"| `{name}` | {n} | {tot / 1e3:.1f} | {tot / n / 1e3:.2f} | {100 * tot / total:.1f} % |")
+        extra = f" | {tot / 1e3 / a.per:.1f}" if a.per else ""
+        print(f"| `{name}` | {n} | {tot / 1e3:.1f} | {tot / n / 1e3:.2f} | "
+              f"{100 * tot / total:.1f} %{extra} |")
+    print(f"\ntotal kernel time {total / 1e6:.3f} ms over {sum(r[1] for r in rows)} dispatches")
 
 
 if __name__ == "__main__":

@@ -11,6 +11,7 @@
 //        runs order O at level S instead.  ORDER = -1 tunes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <string_view>
@@ -34,7 +35,10 @@ int hip_fail(const char *what, hipError_t e) {
 
 int main(int argc, char **argv) {
     if (argc < 6) {
-        std::fprintf(stderr, "usage: %s WORKLOAD fp64|split ORDER SPLIT STEPS [COPY_MIB]\n",
+        std::fprintf(stderr,
+                     "usage: %s WORKLOAD fp64|split ORDER SPLIT STEPS [COPY_MIB]\n"
+                     "  ORDER -1: the tuned schedule; ORDER -2: every schedule the tuner\n"
+                     "  considers (orders 0-4, K split 1 .. the plan's cap), SPLIT ignored\n",
                      argv[0]);
         return 2;
     }
@@ -102,9 +106,21 @@ int main(int argc, char **argv) {
     hipStream_t s = nullptr;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 1;
-    const size_t n_sched = order < 0 ? 1 : splits.size();
+    if (order == -2) {  // every (order, split) that ax_rk_tensor_tune times
+        int o0 = 0, s0 = 0;
+        if (ax_rk_tensor_get_schedule(plan, &o0, &s0)) return fail("get_schedule");
+        const int cap = std::min(16, 4 * s0);
+        orders.clear();
+        splits.clear();
+        for (int sp = 1; sp <= cap; ++sp)
+            for (int o = 0; o < 5; ++o) {
+                orders.push_back(o);
+                splits.push_back(sp);
+            }
+    }
+    const size_t n_sched = order == -1 ? 1 : splits.size();
     for (size_t si = 0; si < n_sched; ++si) {
-        if (order < 0) {
+        if (order == -1) {
             float tms = 0.0f;
             if (ax_rk_tensor_tune(plan, dP, dQ, dout, 3, s, &tms)) return fail("tune");
         } else if (ax_rk_tensor_set_schedule(plan, orders[si], splits[si])) {

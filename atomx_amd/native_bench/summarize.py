@@ -20,9 +20,6 @@ import csv
 import os
 import re
 
-import numpy as np
-
-from ..workloads import valid_pairs
 
 HBM_PEAK = 8.0e12
 CLOCK = 2.4e9
@@ -36,27 +33,52 @@ def short(name: str) -> str:
     return re.sub(r"\(.*", "", name)
 
 
-def tiles_of(path: str) -> int:
-    with open(path, "rb") as f:
-        nv, ng, n_orb, kmax = np.fromfile(f, dtype=np.int32, count=4)
-        f.seek(16 + 8 * (2 * nv * ng + 2 * n_orb * nv * ng))
-        ls = np.fromfile(f, dtype=np.int32, count=n_orb)
-    return sum(((len(valid_pairs(ls, k)) + 63) // 64) * ((len(valid_pairs(ls, k)) + 63) // 64 + 1)
-               // 2 for k in range(kmax + 1))
+ORDERS = ("tile-major", "xcd-contiguous", "xcd-contiguous-split-fastest",
+          "dynamic-heaviest-first", "balanced-k-ranges")
 
 
-def per_kernel(path: str, value: str):
-    out = collections.defaultdict(lambda: collections.defaultdict(list))
+def schedules_of(log: str):
+    """The (order, K split) of every schedule the driver ran, in run order."""
+    out = []
+    for line in open(log):
+        m = re.search(r"order (\d+), K split (\d+):", line)
+        if line.startswith("workload") and m:
+            out.append((int(m.group(1)), int(m.group(2))))
+    return out
+
+
+def dispatches(path: str, value: str):
+    """{dispatch id: (short kernel name, grid, {counter or 'duration': value})}."""
+    out = {}
     for r in csv.DictReader(open(path)):
         if "Grid_Size" in r:
             grid = int(r["Grid_Size"])
         else:  # kernel trace: work-items per dimension
             grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
-        key = (short(r["Kernel_Name"]), grid)
+        did = int(r["Dispatch_Id"])
+        name, _, vals = out.setdefault(did, (short(r["Kernel_Name"]), grid, {}))
         if value == "duration":
-            out[key]["duration"].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+            vals["duration"] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
         else:
-            out[key][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            vals[r["Counter_Name"]] = vals.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def per_key(disp, scheds):
+    """Values per key: the contraction kernel's dispatches keyed by the
+    schedule they ran under (the driver runs the same number per schedule,
+    in the order of its log), every other kernel by (name, grid)."""
+    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    gemm = sorted(d for d, (name, _, _) in disp.items() if name.startswith("rk_gemm"))
+    per = len(gemm) // len(scheds) if scheds else 0
+    if per == 0 or per * len(scheds) != len(gemm):
+        raise SystemExit(f"{len(gemm)} contraction dispatches do not split over "
+                         f"{len(scheds)} schedules")
+    sched_of = {d: scheds[i // per] for i, d in enumerate(gemm)}
+    for d, (name, grid, vals) in disp.items():
+        key = (name, "%d:%d" % sched_of[d]) if d in sched_of else (name, grid)
+        for c, v in vals.items():
+            out[key][c].append(v)
     return out
 
 
@@ -64,28 +86,28 @@ def mean(v):
     return sum(v) / len(v) if v else float("nan")
 
 
-def summarise(d: str, prec: str, n_tiles: int):
-    trace = per_kernel(os.path.join(d, f"{prec}-trace", "nb_kernel_trace.csv"), "duration")
-    fetch = per_kernel(os.path.join(d, f"{prec}-fetch", "nb_counter_collection.csv"), "counter")
-    write = per_kernel(os.path.join(d, f"{prec}-write", "nb_counter_collection.csv"), "counter")
-    mfma = per_kernel(os.path.join(d, f"{prec}-mfma", "nb_counter_collection.csv"), "counter")
+def summarise(d: str, prec: str):
+    scheds = schedules_of(os.path.join(d, f"{prec}-trace.log"))
+    load = lambda name, kind, v: per_key(dispatches(os.path.join(d, f"{prec}-{name}", kind), v),
+                                         scheds)
+    trace = load("trace", "nb_kernel_trace.csv", "duration")
+    fetch = load("fetch", "nb_counter_collection.csv", "counter")
+    write = load("write", "nb_counter_collection.csv", "counter")
+    mfma = load("mfma", "nb_counter_collection.csv", "counter")
     copy = ("__amd_rocclr_copyBuffer", COPY_BYTES // 2048)
     f_fetch = COPY_BYTES / (1024 * mean(fetch[copy]["FETCH_SIZE"]))
     f_write = COPY_BYTES / (1024 * mean(write[copy]["WRITE_SIZE"]))
     rows = []
-    for key in sorted(trace):
-        name, grid = key
+    for key in sorted(trace, key=str):
+        name, where = key
         if not any(name.startswith(k) for k in KERNELS):
             continue
         t = mean(trace[key]["duration"])
         fb = mean(fetch[key]["FETCH_SIZE"]) * 1024 * f_fetch if key in fetch else float("nan")
         wb = mean(write[key]["WRITE_SIZE"]) * 1024 * f_write if key in write else float("nan")
-        split = None
-        if name.startswith("rk_gemm"):  # static K split, or the balanced order's level
-            groups = grid // 256
-            split = (str(groups // n_tiles) if groups % n_tiles == 0
-                     else f"bal{groups // 64}")
-        row = {"kernel": name, "grid": grid, "split": split, "dispatches": len(trace[key]["duration"]),
+        sched = where if isinstance(where, str) else None
+        row = {"kernel": name, "where": where, "schedule": sched,
+               "dispatches": len(trace[key]["duration"]),
                "us": t * 1e6, "fetch": fb, "write": wb, "gbs": (fb + wb) / t / 1e9}
         m = mfma.get(key, {})
         if name.startswith("rk_gemm") and m:
@@ -103,35 +125,40 @@ def main(argv=None) -> None:
     ap.add_argument("outdir")
     ap.add_argument("--table", default=None, help="also write the traffic table here")
     a = ap.parse_args(argv)
-    n_tiles = tiles_of(os.path.join(a.outdir, "w.bin"))
     table = []
     for prec in ("fp64", "split"):
-        ff, fw, rows = summarise(a.outdir, prec, n_tiles)
-        print(f"## {prec} ({n_tiles} upper 64x64 tiles; calibration: fetch x{ff:.4f}, write x{fw:.4f})")
+        ff, fw, rows = summarise(a.outdir, prec)
+        print(f"## {prec} (calibration: fetch x{ff:.4f}, write x{fw:.4f})")
         print()
-        print("| kernel | grid | K split | dispatches | avg us | fetch MB | write MB | HBM GB/s "
-              "(% of 8 TB/s) | MFMA Gop | MFMA busy |")
-        print("|---|---|---|---|---|---|---|---|---|---|")
+        print("| kernel | grid, or schedule order:split | dispatches | avg us | fetch MB | "
+              "write MB | HBM GB/s (% of 8 TB/s) | MFMA Gop | MFMA busy |")
+        print("|---|---|---|---|---|---|---|---|---|")
         for r in rows:
             ops = f"{r['ops'] / 1e9:.3f}" if "ops" in r else ""
             busy = f"{100 * r['busy']:.1f} %" if "busy" in r else ""
-            print(f"| `{r['kernel']}` | {r['grid']} | {r['split'] or ''} | {r['dispatches']} | "
+            where = r["where"]
+            if r["schedule"]:
+                o, sp = map(int, where.split(":"))
+                where = f"{ORDERS[o]}:{sp}"
+            print(f"| `{r['kernel']}` | {where} | {r['dispatches']} | "
                   f"{r['us']:.2f} | {r['fetch'] / 1e6:.1f} | {r['write'] / 1e6:.1f} | "
                   f"{r['gbs']:.0f} ({100 * r['gbs'] * 1e9 / HBM_PEAK:.1f} %) | {ops} | {busy} |")
-            if r["split"]:
-                table.append((r["kernel"], r["split"], r["fetch"], r["write"], r["us"]))
+            if r["schedule"]:
+                table.append((r["kernel"], r["schedule"], r["fetch"], r["write"], r["us"],
+                              r.get("busy", float("nan"))))
         print()
     if a.table:
         with open(a.table, "w") as f:
             f.write("# HBM traffic per launch of the R^k contraction kernels on the benchmark tensor "
-                    "(40 orbitals, 400 x 8 nodes, k = 0..8),\n# one line per K split, from "
-                    "rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of the native driver "
-                    "(atomx_amd/native_bench/counters.sh),\n# scaled by the closing 256 MiB copy; "
-                    "avg_us from the kernel trace of the same driver.\n"
-                    "# columns: kernel schedule (K split of the xcd-contiguous order, or balN: the\n"
-                    "# balanced order at level N) fetch_bytes write_bytes traffic_bytes avg_us\n")
-            for k, s, fb, wb, us in table:
-                f.write(f"traffic: {k} {s} {fb:.0f} {wb:.0f} {fb + wb:.0f} {us:.2f}\n")
+                    "(40 orbitals, 400 x 8 nodes, k = 0..8),\n# one line per schedule the tuner "
+                    "can choose, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of the "
+                    "native driver\n# (atomx_amd/native_bench/counters.sh), scaled by the closing "
+                    "256 MiB copy; avg_us from the kernel trace of the same driver.\n"
+                    "# columns: kernel order:split (order index into tile-major, xcd-contiguous,\n"
+                    "# xcd-contiguous-split-fastest, dynamic-heaviest-first, balanced-k-ranges)\n"
+                    "# fetch_bytes write_bytes traffic_bytes avg_us mfma_busy\n")
+            for k, s_, fb, wb, us, busy in table:
+                f.write(f"traffic: {k} {s_} {fb:.0f} {wb:.0f} {fb + wb:.0f} {us:.2f} {busy:.3f}\n")
 
 
 if __name__ == "__main__":

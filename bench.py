@@ -28,6 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "R^k integrals/s at \u22641e-11 rel err vs FP64 oracle; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK = 78.6      # TFLOP/s dense
 INT8_MFMA_PEAK = 5000.0    # TOP/s dense
@@ -40,6 +41,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=("split", "fp64", "auto"), default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tensor", action="store_true",
+                    help="skip the benchmark tensor (and the CPU leg): run the other legs alone")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all available (<= 16)")
     ap.add_argument("--nv", type=int, default=400)
     ap.add_argument("--ng", type=int, default=8)
@@ -50,6 +53,8 @@ def parse():
                     help="timed steps of the B-spline product workload (0 = skip that leg)")
     ap.add_argument("--ion-steps", type=int, default=20,
                     help="timed SCF-iteration Fock builds of the 26 iron ions (0 = skip)")
+    ap.add_argument("--no-ion-scf", action="store_true",
+                    help="skip the batched SCF of the 26 iron ions")
     ap.add_argument("--no-neon", action="store_true", help="skip the neon Dirac-Fock SCF leg")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests in the served Poisson stream (0 = skip the serving leg)")
@@ -103,6 +108,14 @@ def agreement(got, ref, offsets, m):
     return worst_norm, worst_el
 
 
+_T0 = time.perf_counter()
+
+
+def progress(leg):
+    """One line on stderr per finished leg (the JSON line stays alone on stdout)."""
+    print(f"bench: {leg} done at {time.perf_counter() - _T0:.1f} s", file=sys.stderr, flush=True)
+
+
 def time_steps(fn, steps, stream):
     import torch
     start = torch.cuda.Event(enable_timing=True)
@@ -116,11 +129,12 @@ def time_steps(fn, steps, stream):
     return start.elapsed_time(stop) / steps  # ms
 
 
-def cpu_baseline(P, Q, gr, grw, ls, kmax, threads, min_seconds=6.0):
+def cpu_baseline(P, Q, gr, grw, ls, kmax, threads, min_seconds=6.0, native=True):
     """The oracle's C FP64 restatement (Y^k per (pair, k) once, then one
     sum_x w rho_p Y_q per R^k), OpenMP over pairs/quartets, full workload.
-    Returns (R^k/s, seconds, repetitions, values in the tensor's packed order);
-    the values are the checker of the GPU outputs."""
+    native: the -O2 -march=native build for this host (the timed baseline);
+    otherwise the strict checker build.  Returns (R^k/s, seconds, repetitions,
+    values in the tensor's packed order)."""
     import oracle as O
     from atomx_amd.workloads import pair_list, valid_pairs
     nv, ng = gr.shape
@@ -134,7 +148,8 @@ def cpu_baseline(P, Q, gr, grw, ls, kmax, threads, min_seconds=6.0):
     qp, qq, qk = (np.concatenate(x).astype(np.int32) for x in (qp, qq, qk))
     reps, t0 = 0, time.perf_counter()
     while True:
-        vals = O.rk_quartets(rho, gr, grw, nv, ng, qp, qq, qk, nthreads=threads)
+        vals = O.rk_quartets(rho, gr, grw, nv, ng, qp, qq, qk, nthreads=threads,
+                             native=native)
         reps += 1
         dt = time.perf_counter() - t0
         if dt >= min_seconds:
@@ -192,10 +207,11 @@ def bank_leg(args):
 
 
 def ions_leg(args):
-    """Batched uneven SCF: the direct + exchange Fock build (DBSR cell
-    algorithm, FP64) of all 26 iron charge states (ns ~ U{40..160}, order
-    7-8, 1-10 orbitals, k <= 6) per SCF iteration, as one replay of a
-    captured HIP graph; the same launches without the graph for comparison."""
+    """Batched uneven SCF iteration: the direct + exchange Fock build (DBSR
+    cell algorithm, FP64) of every target of all 26 iron charge states
+    (ns ~ U{40..160}, order 7-8, 1-10 orbitals, k <= 6), the Fock engine's
+    five launches replayed from a captured HIP graph; eager, and the older
+    per-(ion, block, k) launch path, for comparison."""
     import torch
 
     from atomx_amd.batch_scf import IonBatch
@@ -207,7 +223,10 @@ def ions_leg(args):
     for _ in range(args.warmup):
         batch.replay()
     ms_graph = time_steps(batch.replay, args.ion_steps, stream)
-    ms_eager = time_steps(batch.run_eager, max(3, args.ion_steps // 4), stream)
+    ms_eager = time_steps(batch.run_eager, max(3, args.ion_steps // 2), stream)
+    launches = IonBatch(ions, path="launches").capture()
+    launches.replay()
+    ms_launches = time_steps(launches.replay, max(3, args.ion_steps // 4), stream)
     n_eq = batch.rk_equivalents
     return {"workload": "26 Fe ions (charge 0..25), ns~U{40..160}, ks~U{7,8} (kinetic balance), "
                         "ground-configuration subshells (1-10 orbitals), k<=6, all targets",
@@ -215,7 +234,48 @@ def ions_leg(args):
             "orbitals": [len(i["kappas"]) for i in ions],
             "rk_equivalents_per_iteration": n_eq,
             "ms_per_iteration_graph": ms_graph, "ms_per_iteration_eager": ms_eager,
+            "ms_per_iteration_per_block_launch_path_graph": ms_launches,
+            "engine_plan_device_bytes": batch.engine.nbytes,
             "rk_equivalents_per_s": n_eq / (ms_graph * 1e-3), "dtype": "f64"}
+
+
+def ion_scf_leg(args):
+    """The 26 iron charge states solved: independent Gauss-Seidel Dirac-Fock
+    SCFs from the bare nucleus to an orbital change below 1e-9, sweep s of
+    every ion together (stage i = the Fock blocks of orbital i of every ion
+    in one engine run on the GPU, then the ions' FP64 eigensolves in
+    parallel host threads)."""
+    import json as _json
+
+    from atomx_amd.batch_solve import BatchDiracFock, fe_ion_subshells
+    from atomx_amd.workloads import fe_ion_workload
+
+    ions = fe_ion_workload(seed=args.seed)
+    batch = BatchDiracFock([(26, fe_ion_subshells(c), ion["bsp_data"])
+                            for c, ion in enumerate(ions)])
+    try:
+        t0 = time.perf_counter()
+        e = batch.run(tol=1e-9)
+        dt = time.perf_counter() - t0
+        sweeps = [s.sweeps for s in batch.systems]
+        res = {"workload": "26 Fe ions (charge 0..25) of the Fock-build leg, Z=26, point nucleus, "
+                           "bare-nucleus start, tol 1e-9 on the orbitals",
+               "E_total_Ha": [float(x) for x in e], "sweeps_per_ion": sweeps,
+               "sweeps": batch.sweeps, "seconds": dt,
+               "ms_per_sweep": float(np.mean(batch.sweep_ms)),
+               "gpu_fock_ms_per_sweep": float(np.mean(batch.gpu_ms)),
+               "host_copy_ms_per_sweep": float(np.mean(batch.copy_ms)),
+               "host_eigensolve_ms_per_sweep": float(np.mean(batch.solve_ms)),
+               "eigensolve_threads": batch.threads,
+               "fock_rk_equivalents_per_sweep": batch.rk_equivalents_per_sweep}
+    finally:
+        batch.close()
+    gold = os.path.join(ROOT, "tests", "golden", "fe_ion_scf_oracle.json")
+    if args.seed == 0 and os.path.exists(gold):  # the oracle SCF energies of this workload
+        ref = _json.load(open(gold))["ions"]
+        res["max_abs_dE_vs_oracle_scf_Ha"] = max(abs(x - ref[str(c)]["E_total_Ha"])
+                                                  for c, x in enumerate(res["E_total_Ha"]))
+    return res
 
 
 def neon_leg(args):
@@ -252,19 +312,14 @@ TRAFFIC_TABLE = "profiles/c2_hbm_traffic.txt"
 def hbm_traffic(args, kernel, order, k_split):
     """HBM bytes per launch of the contraction kernel at this schedule, from
     the calibrated FETCH_SIZE + WRITE_SIZE table measured on the default
-    workload (None for other workloads or schedules the table lacks: it
-    holds the xcd-contiguous order's K splits and the balanced order's
-    levels)."""
+    workload for every schedule the tuner can choose (None for other
+    workloads, or a kernel the table does not hold)."""
+    from atomx_amd.tensor import SlaterTensor
     default = (args.nv, args.ng, args.n_orb, args.kmax, args.seed) == (400, 8, 40, 8, 0)
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_TABLE)
+    path = os.path.join(ROOT, TRAFFIC_TABLE)
     if not default or not os.path.exists(path):
         return None
-    if order == "xcd-contiguous":
-        key = str(int(k_split))
-    elif order == "balanced-k-ranges":
-        key = f"bal{int(k_split)}"
-    else:
-        return None
+    key = f"{SlaterTensor.ORDERS.index(order)}:{int(k_split)}"
     for line in open(path):
         f = line.split()
         if len(f) >= 6 and f[0] == "traffic:" and f[1] == kernel and f[2] == key:
@@ -486,11 +541,8 @@ def scf_serving(args, slater, iterations=6):
     return out
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
-        sys.exit(2)
+def headline(args):
+    """The benchmark tensor in both modes; returns (line, context of the CPU leg)."""
     import torch
     from atomx_amd import device as dv
     from atomx_amd.tensor import SlaterTensor
@@ -561,7 +613,7 @@ def main():
     ms = modes[prec]["ms_per_step"]
     value = n_rk / (ms * 1e-3)
     line = {
-        "metric": "R^k integrals/s at \u22641e-11 rel err vs FP64 oracle; p50/p99 request latency",
+        "metric": METRIC,
         "value": value, "unit": "R^k/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == "fp64" else "int8-slices/f64",
@@ -572,22 +624,55 @@ def main():
         "accuracy": modes[prec].get("vs_fp64_max_rel_err", 0.0),
         "modes": modes, "stage_ms": stages, "roofline": roofline,
     }
+    ctx = {"P": P, "Q": Q, "gr": gr, "grw": grw, "ls": ls, "n_rk": n_rk, "host_out": host_out,
+           "p0": p0, "modes": modes, "prec": prec, "plans": plans}
+    return line, ctx
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        sys.exit(2)
+    if args.no_tensor:  # the other legs alone (profiling one leg)
+        line = {"metric": METRIC, "value": None, "unit": "R^k/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                "data": "synthetic", "config": {"workload": "selected legs only"}}
+        ctx = None
+    else:
+        line, ctx = headline(args)
+        progress("benchmark tensor")
     if args.bank_steps > 0:
         line["bspline_products"] = bank_leg(args)
+        progress("B-spline products")
     if args.ion_steps > 0:
         line["fe_ions"] = ions_leg(args)
+        progress("iron-ion Fock builds")
+    if not args.no_ion_scf:
+        line["fe_ions_scf"] = ion_scf_leg(args)
+        progress("iron-ion SCF")
     neon = None
     if not args.no_neon:
         neon = neon_leg(args)
         line["neon_dirac_fock"] = {k: v for k, v in neon.items() if not k.startswith("_")}
+        progress("neon SCF")
     if args.serve_requests > 0:
         serving = serving_leg(args)
+        progress("serving")
         line["serving"] = serving
         line["latency_p50_ms"] = serving["p50_ms"]
         line["latency_p99_ms"] = serving["p99_ms"]
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        v, dt, reps, ref = cpu_baseline(P, Q, gr, grw, ls, args.kmax, threads)
+    if not args.no_cpu and ctx is not None:
+        import oracle as O
+        P, Q, gr, grw, ls = (ctx[k] for k in ("P", "Q", "gr", "grw", "ls"))
+        n_rk, host_out, p0, modes, prec = (ctx[k] for k in ("n_rk", "host_out", "p0", "modes",
+                                                            "prec"))
+        host = O.host_cpu()
+        threads = args.cpu_threads or max(1, min(16, host["affinity_cpus"]))
+        # the checker: the strict oracle build, once; then the timed native build
+        _, _, _, ref = cpu_baseline(P, Q, gr, grw, ls, args.kmax, threads, min_seconds=0.0,
+                                    native=False)
+        v, dt, reps, nat = cpu_baseline(P, Q, gr, grw, ls, args.kmax, threads)
         v1, dt1, reps1, _ = cpu_baseline(P, Q, gr, grw, ls, args.kmax, 1, min_seconds=4.0)
         for m in ("fp64", "split"):  # the oracle's values check every GPU output
             norm, el = agreement(host_out[m], ref, p0.offsets, p0.m)
@@ -599,15 +684,22 @@ def main():
             e_cpu, sweeps = neon_oracle_scf(neon["_bd"])
             line["neon_dirac_fock"]["E_total_oracle_scf_Ha"] = e_cpu
             line["neon_dirac_fock"]["abs_diff_vs_oracle_scf_Ha"] = abs(neon["E_total_Ha"] - e_cpu)
-        line["cpu_baseline"] = {"value": v, "unit": "R^k/s", "cores": threads, "kind": "port",
-                                "sample": f"full workload ({n_rk} R^k) repeated {reps}x in {dt:.1f} s "
-                                          f"with {threads} OpenMP threads",
-                                "single_thread": {"value": v1, "sample": f"{reps1}x in {dt1:.1f} s"},
-                                "host_cpus": os.cpu_count()}
+        line["cpu_baseline"] = {
+            "value": v, "unit": "R^k/s", "cores": threads, "kind": "port",
+            "sample": f"full workload ({n_rk} R^k) repeated {reps}x in {dt:.1f} s with {threads} "
+                      f"OpenMP threads",
+            "build": "oracle/slater_oracle.c, gcc -O2 -march=native -fno-fast-math -fopenmp, "
+                     "built on this host",
+            "native_vs_checker_max_err_over_max": agreement(nat, ref, p0.offsets, p0.m)[0],
+            "single_thread": {"value": v1, "sample": f"{reps1}x in {dt1:.1f} s"},
+            "host": host}
+        progress("CPU baseline")
     print(json.dumps(line), flush=True)
-    for pl, _ in plans.values():
-        pl.close()
+    if ctx is not None:
+        for pl, _ in ctx["plans"].values():
+            pl.close()
     if args.copy_mib > 0:
+        import torch
         src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()

@@ -46,6 +46,48 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+_NATIVE_PATH = os.path.join(_HERE, "_build", "liboracle_native.so")
+_native = None
+
+
+def lib_native():
+    """The same C source built -O2 -march=native with OpenMP for the CPU this
+    process runs on (rebuilt on first use in each process, since a library
+    built elsewhere may name another CPU): the timed CPU baseline.  Its
+    values are not the checker's (multiply-adds may be fused)."""
+    global _native
+    if _native is None:
+        import subprocess
+        subprocess.run(["make", "-s", "-B", "-C", _HERE, "native"], check=True)
+        _native = ctypes.CDLL(_NATIVE_PATH)
+    return _native
+
+
+def host_cpu():
+    """nproc, the CPUs this process may use, and lscpu's identifying lines."""
+    import subprocess
+    info = {"nproc": int(subprocess.run(["nproc"], capture_output=True, text=True).stdout)}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity_cpus"] = os.cpu_count()
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keep = ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+            "NUMA node(s)", "CPU max MHz", "L3 cache", "Flags")
+    for line in out.splitlines():
+        key, _, val = line.partition(":")
+        key = key.strip()
+        if key in keep:
+            val = val.strip()
+            if key == "Flags":  # the vector extensions only
+                val = " ".join(f for f in val.split() if f.startswith(("avx", "fma", "amx")))
+            info["lscpu " + key] = val
+    return info
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -158,16 +200,17 @@ def slater_rk_direct(rho13, rho24, gr, grw, k):
     return float(off + diag)
 
 
-def rk_quartets(rho, r, w, nv, ng, qp, qq, qk, nthreads=0):
+def rk_quartets(rho, r, w, nv, ng, qp, qq, qk, nthreads=0, native=False):
     """All requested R^k(p;q) = sum_x w rho_p Y^k[rho_q] (slater.py:118-130 per
-    quartet, with Y^k computed once per (q, k)), OpenMP over quartets."""
+    quartet, with Y^k computed once per (q, k)), OpenMP over quartets.
+    native: run the -march=native build (the timed baseline) instead."""
     rho = _c(rho)
     r, w = _c(np.ravel(r)), _c(np.ravel(w))
     qp, qq, qk = (_c(a, np.int32) for a in (qp, qq, qk))
     mo, mi = yk_partial_matrices(ng)
     out = np.empty(qp.size)
     kmax = int(qk.max()) if qk.size else 0
-    lib().orc_rk_quartets(_p(rho), ctypes.c_long(rho.shape[0]), _p(r), _p(w),
+    (lib_native() if native else lib()).orc_rk_quartets(_p(rho), ctypes.c_long(rho.shape[0]), _p(r), _p(w),
                           ctypes.c_int(nv), ctypes.c_int(ng), _p(mo), _p(mi),
                           qp.ctypes.data_as(_I), qq.ctypes.data_as(_I), qk.ctypes.data_as(_I),
                           ctypes.c_long(qp.size), ctypes.c_int(kmax), ctypes.c_int(nthreads),
