@@ -67,9 +67,10 @@ class _NoBackend:
 class BatchDiracFock:
     """systems: sequence of (Z, subshells, bsp_data); ``nuclear`` as in
     ``DiracFockSCF``.  threads: host threads for the eigensolves (default:
-    the CPUs this process may use, at most 16)."""
+    the CPUs this process may use, at most 16).  diis: Pulay extrapolation
+    per system, as ``DiracFockSCF.sweep(diis=True)`` applies it."""
 
-    def __init__(self, systems, nuclear="point", threads=None, target_group=4):
+    def __init__(self, systems, nuclear="point", threads=None, target_group=4, diis=False):
         from .dirac_fock import FockBuilder, FockEngine
         dv.require_gpu()
         self.systems = []
@@ -97,6 +98,8 @@ class BatchDiracFock:
                 threads = os.cpu_count() or 1
             threads = max(1, min(16, threads))
         self.threads = threads
+        self.diis = diis
+        self.extrapolations = 0
         self._pool = ThreadPoolExecutor(threads) if threads > 1 else None
         self.gpu_ms = []       # GPU time of the Fock builds of each sweep (all stages)
         self.copy_ms = []      # host time of the block downloads and orbital uploads
@@ -168,6 +171,9 @@ class BatchDiracFock:
         gpu = copy = solve = 0.0
         for sysm in self.systems:
             sysm.change = 0.0
+        # DIIS as DiracFockSCF.sweep applies it: five or more orbitals, fourth sweep on
+        x_in = {a: s.scf._flat() for a, s in enumerate(self.systems)
+                if self.diis and not s.converged and len(s.scf.shells) >= 5 and s.sweeps >= 3}
         for i, (members, eng) in enumerate(self.stages):
             live = [r for r, a in enumerate(members) if not self.systems[members[r]].converged]
             if not live:
@@ -188,6 +194,11 @@ class BatchDiracFock:
                 self._upload(members[r], rows=[i])
             copy += (tb - ta) + (time.perf_counter() - tc)
             solve += tc - tb
+        for a, x in x_in.items():
+            sysm = self.systems[a]
+            if sysm.scf.diis_step(x, sysm.change):
+                self.extrapolations += 1
+                self._upload(a)
         for sysm in self.systems:
             if not sysm.converged:
                 sysm.sweeps += 1

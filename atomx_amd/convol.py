@@ -546,6 +546,8 @@ def _bankT_dev(bank, transposed):
     d = getattr(bank, "_ax_bankT", None)
     if d is not None:
         return d
+    if dv._is_tensor(bank):  # already on the device (a wrapper passing its upload on)
+        return dv.f64(bank if transposed else bank.permute(2, 3, 0, 1))
     b = np.asarray(bank, dtype=np.float64)
     if not transposed:
         b = b.transpose(2, 3, 0, 1)

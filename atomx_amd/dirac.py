@@ -287,7 +287,7 @@ class DiracFockSCF:
             from .dirac_fock import FockBuilder
             fb = FockBuilder(bsp_data, kappas, occ)
             if mode == "dbsr":
-                backend = _GpuBackend(fb)
+                backend = _EngineBackend(fb)
             elif mode == "orbital":
                 from .orbital_mode import OrbitalModeFock
                 backend = _OrbitalBackend(OrbitalModeFock(bsp_data, kappas, occ, fb.cache), fb)
@@ -355,9 +355,16 @@ class DiracFockSCF:
         H[ns:, :ns] += qp
         return H, S
 
-    def sweep(self):
+    def sweep(self, diis=False):
         """One Gauss-Seidel pass over the orbitals; returns the largest
-        change of a large-component coefficient relative to its maximum."""
+        change of a large-component coefficient relative to its maximum.
+        diis: Pulay extrapolation of the swept orbitals, as the reference
+        applies it (methods/dirac_fock.py:2061-2202): only for five or more
+        orbitals from the fourth sweep on, history of 8, extrapolated when 3
+        or more entries are held and the sweep changed the orbitals by
+        between 1e-3 and 0.3, then Lowdin-orthonormalised per kappa."""
+        use_diis = diis and len(self.shells) >= 5 and self.iterations + 1 >= 4
+        x_in = self._flat() if use_diis else None
         change = 0.0
         for i, sh in enumerate(self.shells):
             H, S = self.fock_matrix(i)
@@ -373,15 +380,52 @@ class DiracFockSCF:
                 change = max(change, float(np.max(np.abs(P - self.P[i])) / pmax))
             self.P[i], self.Q[i], self.eps[i] = P, Q, e
         self.iterations += 1
+        if use_diis:
+            self.diis_step(x_in, change)
         return change
 
-    def run(self, max_iter=200, tol=1e-8, energy_tol=1e-12):
+    # -- DIIS ----------------------------------------------------------------------
+    def _flat(self):
+        """All orbitals as one vector: [P_0, Q_0, P_1, Q_1, ...]."""
+        return np.concatenate([np.concatenate((p, q)) for p, q in zip(self.P, self.Q)])
+
+    def diis_step(self, x_in, change):
+        """Records the sweep (input x_in, output the current orbitals) and
+        replaces the orbitals by the extrapolation when it applies."""
+        from .diis import DiisHistory, lowdin_by_kappa
+        if getattr(self, "_diis", None) is None:
+            self._diis = DiisHistory(8)
+        x_out = self._flat()
+        self._diis.push(x_out, x_out - x_in)
+        if len(self._diis) < 3 or not 1e-3 < change < 0.3:
+            return False
+        x = self._diis.extrapolate()
+        if x is None:
+            return False
+        ns, n = self.ns, len(self.shells)
+        v = x.reshape(n, 2, ns)
+        try:
+            P, Q = lowdin_by_kappa(v[:, 0], v[:, 1], [s.kappa for s in self.shells],
+                                   {k: self.h0[k][1] for k in self.h0})
+        except (ValueError, np.linalg.LinAlgError):
+            self._diis.clear()  # outside the independent-orbital manifold: keep the sweep
+            return False
+        self.P[:], self.Q[:] = P, Q
+        return True
+
+    def run(self, max_iter=200, tol=1e-8, energy_tol=1e-12, diis=False):
         """Sweeps until the orbitals change by less than ``tol`` (relative to
         each large component's maximum) and the total energy by less than
-        ``energy_tol`` Ha; returns the energy."""
+        ``energy_tol`` Ha; returns the energy.  The energy is evaluated only
+        on sweeps that changed the orbitals by less than 1e3 tol (an energy
+        change is stationary, second order in the orbital change, so it can
+        only decide convergence there)."""
         e_old = None
         for _ in range(max_iter):
-            change = self.sweep()
+            change = self.sweep(diis=diis)
+            if change >= 1e3 * tol:
+                e_old = None
+                continue
             e = self.energy()[0]
             if e_old is not None and change < tol and abs(e - e_old) < energy_tol:
                 return e
@@ -403,19 +447,55 @@ class DiracFockSCF:
         return e1 + e2, e1, e2
 
 
-class _GpuBackend:
+class _EngineBackend:
+    """Fock blocks from one Fock-engine plan per target over resident
+    coefficient stacks: a call uploads only the orbitals that changed since
+    the previous call (one row after a Gauss-Seidel step), runs the target's
+    plan and brings its four blocks back in one copy."""
+
     def __init__(self, fb):
+        from . import device as dv
         self.fb = fb
-        self._steps = {}
+        self.P = dv.zeros((fb.n_orb, fb.ns))
+        self.Q = dv.zeros((fb.n_orb, fb.ns))
+        self._hP = np.full((fb.n_orb, fb.ns), np.nan)  # what the device holds
+        self._hQ = np.full((fb.n_orb, fb.ns), np.nan)
+        self._plans = {}
+        self.uploads = 0
+
+    def _sync(self, P, Q):
+        from . import device as dv
+        P, Q = np.asarray(P, dtype=np.float64), np.asarray(Q, dtype=np.float64)
+        m = min(P.shape[1], self.fb.ns)
+        for b in range(self.fb.n_orb):
+            if np.array_equal(P[b, :m], self._hP[b, :m]) and \
+                    np.array_equal(Q[b, :m], self._hQ[b, :m]):
+                continue
+            self._hP[b] = 0.0
+            self._hQ[b] = 0.0
+            self._hP[b, :m], self._hQ[b, :m] = P[b, :m], Q[b, :m]
+            self.P[b] = dv.f64(self._hP[b])
+            self.Q[b] = dv.f64(self._hQ[b])
+            self.uploads += 1
+
+    def plan(self, i):
+        from .dirac_fock import FockEngine
+        eng = self._plans.get(i)
+        if eng is None:
+            eng = FockEngine([self.fb], skip_pq_exchange=True, targets=[[i]], P=[self.P],
+                             Q=[self.Q])
+            self._plans[i] = eng
+        return eng
 
     def fock(self, i, P, Q):
         from . import device as dv
-        step = self._steps.get(i)
-        if step is None:
-            step = self.fb.prepare(targets=[i], skip_pq_exchange=True)
-            self._steps[i] = step
-        H = step.run(dv.f64(P), dv.f64(Q))
-        return tuple(dv.host(x) for x in H[i])
+        self._sync(P, Q)
+        eng = self.plan(i)
+        eng.run()
+        return tuple(dv.host(eng.H[0][0]))
+
+    def rk_equivalents(self):
+        return sum(e.rk_equivalents for e in self._plans.values())
 
     def energy(self, P, Q):
         return self.fb.energy(P, Q)

@@ -4,11 +4,12 @@ total and average duration per kernel name, largest total first, as a
 Markdown table.
 
     python -m atomx_amd.native_bench.kernel_stats TRACE [--match TEXT] [--top N]
-                                                  [--last N] [--per N]
+                                                  [--last N] [--per N] [--between K]
 
 --last N keeps only the last N dispatches of each kernel (a timed loop that
 follows a tuning sweep of the same kernels); --per N adds a column with the
-total divided by N (time per SCF sweep, per request, ...).
+total divided by N (time per SCF sweep, per request, ...); --between K keeps
+the dispatches from the first to the last dispatch of kernel K.
 """
 
 from __future__ import annotations
@@ -39,9 +40,15 @@ def dispatches(path):
     return [(int(t0), _short(name), int(d)) for t0, name, d in rows]
 
 
-def kernel_stats(path, match=None, last=0):
+def kernel_stats(path, match=None, last=0, between=None):
+    disp = sorted(dispatches(path))
+    if between:  # the window from the first to the last dispatch of that kernel
+        marks = [t0 for t0, name, _ in disp if name == between]
+        if not marks:
+            raise SystemExit(f"no dispatch of {between}")
+        disp = [x for x in disp if marks[0] <= x[0] <= marks[-1]]
     by_name = {}
-    for t0, name, dur in sorted(dispatches(path)):
+    for t0, name, dur in disp:
         if match and match not in name:
             continue
         by_name.setdefault(name, []).append(dur)
@@ -60,8 +67,10 @@ def main():
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--last", type=int, default=0)
     ap.add_argument("--per", type=float, default=0.0)
+    ap.add_argument("--between", default=None,
+                    help="only dispatches from the first to the last one of this kernel")
     a = ap.parse_args()
-    rows = kernel_stats(a.trace, a.match, a.last)
+    rows = kernel_stats(a.trace, a.match, a.last, a.between)
     total = sum(r[2] for r in rows)
     per = f" | us per 1/{a.per:g}" if a.per else ""
     print(f"| kernel | calls | total us | avg us | share{per} |")

@@ -124,6 +124,9 @@ def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("outdir")
     ap.add_argument("--table", default=None, help="also write the traffic table here")
+    ap.add_argument("--top", type=int, default=10,
+                    help="contraction schedules listed per mode, fastest first (the table "
+                         "holds all)")
     a = ap.parse_args(argv)
     table = []
     for prec in ("fp64", "split"):
@@ -133,7 +136,9 @@ def main(argv=None) -> None:
         print("| kernel | grid, or schedule order:split | dispatches | avg us | fetch MB | "
               "write MB | HBM GB/s (% of 8 TB/s) | MFMA Gop | MFMA busy |")
         print("|---|---|---|---|---|---|---|---|---|")
-        for r in rows:
+        fast = sorted((r for r in rows if r["schedule"]), key=lambda r: r["us"])[:a.top]
+        listed = [r for r in rows if not r["schedule"]] + fast
+        for r in listed:
             ops = f"{r['ops'] / 1e9:.3f}" if "ops" in r else ""
             busy = f"{100 * r['busy']:.1f} %" if "busy" in r else ""
             where = r["where"]
@@ -143,10 +148,12 @@ def main(argv=None) -> None:
             print(f"| `{r['kernel']}` | {where} | {r['dispatches']} | "
                   f"{r['us']:.2f} | {r['fetch'] / 1e6:.1f} | {r['write'] / 1e6:.1f} | "
                   f"{r['gbs']:.0f} ({100 * r['gbs'] * 1e9 / HBM_PEAK:.1f} %) | {ops} | {busy} |")
+        for r in rows:
             if r["schedule"]:
                 table.append((r["kernel"], r["schedule"], r["fetch"], r["write"], r["us"],
                               r.get("busy", float("nan"))))
-        print()
+        n_sched = sum(1 for r in rows if r["schedule"])
+        print(f"\n({len(fast)} fastest of {n_sched} contraction schedules listed)\n")
     if a.table:
         with open(a.table, "w") as f:
             f.write("# HBM traffic per launch of the R^k contraction kernels on the benchmark tensor "
