@@ -31,7 +31,7 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 from . import device as dv
-from .dirac import DiracFockSCF, Subshell, solve_dirac
+from .dirac import DiracFockSCF, Subshell, one_blas_thread, solve_dirac
 
 __all__ = ["BatchDiracFock", "fe_ion_subshells"]
 
@@ -109,14 +109,10 @@ class BatchDiracFock:
 
     # -- helpers -------------------------------------------------------------------
     def _map(self, fn, items):
-        if self._pool is None:
-            return [fn(x) for x in items]
         # one BLAS thread per eigensolve: the parallelism is across systems
-        try:
-            from threadpoolctl import threadpool_limits
-        except ImportError:  # pragma: no cover
-            return list(self._pool.map(fn, items))
-        with threadpool_limits(limits=1, user_api="blas"):
+        with one_blas_thread():
+            if self._pool is None:
+                return [fn(x) for x in items]
             return list(self._pool.map(fn, items))
 
     def _upload(self, a, rows=None):

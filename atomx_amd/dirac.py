@@ -30,6 +30,25 @@ __all__ = ["fermi_params", "nuclear_potential", "assemble_integral", "dirac_matr
 
 C = SPEED_OF_LIGHT
 C_SQ = C * C
+
+_THREADPOOLS = None
+
+
+def one_blas_thread():
+    """Context manager: BLAS/LAPACK on one thread.  The SCF's dense algebra
+    is on matrices of a few hundred rows, where a multi-threaded BLAS spends
+    more time waking threads than computing (a 120-row neon eigensolve
+    takes ~15x longer on 8 threads than on one); systems solved in parallel
+    get their parallelism from host threads instead."""
+    global _THREADPOOLS
+    import contextlib
+    try:
+        from threadpoolctl import ThreadpoolController
+    except ImportError:  # pragma: no cover - threadpoolctl ships with scikit-learn
+        return contextlib.nullcontext()
+    if _THREADPOOLS is None:
+        _THREADPOOLS = ThreadpoolController()
+    return _THREADPOOLS.limit(limits=1, user_api="blas")
 BOHR_RADIUS_FM = 5.29177210544e-11 * 1e15   # CODATA 2022 Bohr radius in fm
 
 # DBSR_HF RMS charge radii (fm) and mass numbers of the light elements
@@ -326,10 +345,11 @@ class DiracFockSCF:
 
     def bare_nucleus_start(self):
         """Orbitals of the bare nucleus (one-body Dirac operator only)."""
-        for i, sh in enumerate(self.shells):
-            H, S = self.h0[sh.kappa]
-            w, v = solve_dirac(H, S, self.ns, self.nsp, self.nsq)
-            self.P[i], self.Q[i], self.eps[i] = self._root(i, w, v, False)
+        with one_blas_thread():
+            for i, sh in enumerate(self.shells):
+                H, S = self.h0[sh.kappa]
+                w, v = solve_dirac(H, S, self.ns, self.nsp, self.nsq)
+                self.P[i], self.Q[i], self.eps[i] = self._root(i, w, v, False)
         return self
 
     def set_orbitals(self, P, Q):
@@ -363,6 +383,10 @@ class DiracFockSCF:
         orbitals from the fourth sweep on, history of 8, extrapolated when 3
         or more entries are held and the sweep changed the orbitals by
         between 1e-3 and 0.3, then Lowdin-orthonormalised per kappa."""
+        with one_blas_thread():
+            return self._sweep(diis)
+
+    def _sweep(self, diis):
         use_diis = diis and len(self.shells) >= 5 and self.iterations + 1 >= 4
         x_in = self._flat() if use_diis else None
         change = 0.0

@@ -1,9 +1,10 @@
 #!/bin/bash
 # Hardware-counter runs of the benchmark tensor through the native driver
 # (no Python in the profiled process).  Per precision mode: a kernel trace,
-# then one pass each of FETCH_SIZE, WRITE_SIZE and the MFMA counters, each a
-# profiler run of its own; every run ends with a 256 MiB device copy that
-# calibrates the byte counters.
+# then one pass each of FETCH_SIZE, WRITE_SIZE, the MFMA counters, the LDS
+# counters and the wave-cycle (stall) counters, each a profiler run of its
+# own; every run ends with a 256 MiB device copy that calibrates the byte
+# counters.
 #
 #   python -m atomx_amd.native_bench.build        (once, on the build host)
 #   bash atomx_amd/native_bench/counters.sh OUTDIR
@@ -38,9 +39,14 @@ pass() {  # name, precision, profiler options...
     return $rc
 }
 MFMA="SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_VALU_MFMA_MOPS_I8 SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE"
+# LDS activity and conflicts; where the waves' cycles go (8 SQ counters per pass)
+LDS="SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_LDS_IDX_ACTIVE SQ_WAIT_INST_LDS SQ_ACTIVE_INST_LDS SQ_LDS_DATA_FIFO_FULL SQ_LDS_ADDR_CONFLICT SQ_WAVE_CYCLES"
+STALL="SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VMEM SQ_ACTIVE_INST_MISC SQ_INSTS_VALU SQ_WAVE_CYCLES"
 for prec in fp64 split; do
     pass "$prec-trace" $prec --kernel-trace --stats &&
     pass "$prec-fetch" $prec --pmc FETCH_SIZE &&
     pass "$prec-write" $prec --pmc WRITE_SIZE &&
-    pass "$prec-mfma" $prec --pmc $MFMA || exit 1
+    pass "$prec-mfma" $prec --pmc $MFMA &&
+    pass "$prec-lds" $prec --pmc $LDS &&
+    pass "$prec-stall" $prec --pmc $STALL || exit 1
 done

@@ -94,6 +94,12 @@ def summarise(d: str, prec: str):
     fetch = load("fetch", "nb_counter_collection.csv", "counter")
     write = load("write", "nb_counter_collection.csv", "counter")
     mfma = load("mfma", "nb_counter_collection.csv", "counter")
+    extra = {}
+    for name in ("lds", "stall"):  # optional passes
+        path = os.path.join(d, f"{prec}-{name}", "nb_counter_collection.csv")
+        if os.path.exists(path):
+            for key, vals in load(name, "nb_counter_collection.csv", "counter").items():
+                extra.setdefault(key, {}).update(vals)
     copy = ("__amd_rocclr_copyBuffer", COPY_BYTES // 2048)
     f_fetch = COPY_BYTES / (1024 * mean(fetch[copy]["FETCH_SIZE"]))
     f_write = COPY_BYTES / (1024 * mean(write[copy]["WRITE_SIZE"]))
@@ -116,6 +122,22 @@ def summarise(d: str, prec: str):
             busy = mean(m.get("SQ_VALU_MFMA_BUSY_CYCLES", []))
             row["ops"] = mops * 512
             row["busy"] = busy / (SIMDS * t * CLOCK)
+        x = {c: mean(v) for c, v in extra.get(key, {}).items()}
+        wc = x.get("SQ_WAVE_CYCLES")
+        if wc:
+            row["waves"] = {
+                "lds_active_inst": x.get("SQ_ACTIVE_INST_LDS", float("nan")) / wc,
+                "wait_inst_lds": x.get("SQ_WAIT_INST_LDS", float("nan")) / wc,
+                "wait_any": x.get("SQ_WAIT_ANY", float("nan")) / wc,
+                "wait_inst_any": x.get("SQ_WAIT_INST_ANY", float("nan")) / wc,
+                "active_valu": x.get("SQ_ACTIVE_INST_VALU", float("nan")) / wc,
+                "active_vmem": x.get("SQ_ACTIVE_INST_VMEM", float("nan")) / wc,
+                "active_any": x.get("SQ_ACTIVE_INST_ANY", float("nan")) / wc,
+                "lds_bank_conflict_per_active": x.get("SQ_LDS_BANK_CONFLICT", float("nan"))
+                / max(x.get("SQ_LDS_IDX_ACTIVE", float("nan")), 1.0),
+                "lds_idx_active_per_cu_cycle": x.get("SQ_LDS_IDX_ACTIVE", float("nan"))
+                / (256 * t * CLOCK),
+            }
         rows.append(row)
     return f_fetch, f_write, rows
 
@@ -154,6 +176,17 @@ def main(argv=None) -> None:
                               r.get("busy", float("nan"))))
         n_sched = sum(1 for r in rows if r["schedule"])
         print(f"\n({len(fast)} fastest of {n_sched} contraction schedules listed)\n")
+        waved = [r for r in listed if "waves" in r]
+        if waved:
+            keys = list(waved[0]["waves"])
+            print("Wave cycles (fractions of SQ_WAVE_CYCLES; LDS bank-conflict cycles per "
+                  "LDS-active cycle; LDS-active cycles per CU cycle):\n")
+            print("| kernel | where | " + " | ".join(keys) + " |")
+            print("|---|---|" + "---|" * len(keys))
+            for r in waved:
+                print(f"| `{r['kernel']}` | {r['where']} | "
+                      + " | ".join(f"{r['waves'][k]:.3f}" for k in keys) + " |")
+            print()
     if a.table:
         with open(a.table, "w") as f:
             f.write("# HBM traffic per launch of the R^k contraction kernels on the benchmark tensor "
