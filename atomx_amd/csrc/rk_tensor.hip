@@ -36,15 +36,22 @@ struct ax_rk_tensor {
     int64_t out_size = 0;
     int64_t footprint = 0;
     ax::PlanHost host{};
-    // job lists of every K-split level S = 1..split_cap (index S - 1), on
-    // the device: tile-fastest, split-fastest, heaviest first (one job per
-    // work-group: `identity` groups) with their per-tile slot ranges, and the
-    // balanced order's jobs, work-groups and slot ranges
+    // job lists of the K-split levels S = 1..split_cap (index S - 1; null
+    // until built), on the device: tile-fastest, split-fastest, heaviest
+    // first (one job per work-group: `identity` groups) with their per-tile
+    // slot ranges, and the balanced order's jobs, work-groups and slot
+    // ranges.  Creation builds the default level's; set_schedule and tune
+    // build others when first asked (ensure_tables), from the host copies
+    // of the tile list below.
     std::vector<const ax::RkJob *> jobs_tf, jobs_sf, jobs_hv, jobs_bal;
     std::vector<const ax::RkTileJobs *> tile_jobs, tile_jobs_bal;
     std::vector<const int *> groups_bal;
     std::vector<int> n_jobs, n_groups_bal, reduce_bal;
     const int *identity = nullptr;
+    std::vector<ax::RkWork> work_host;
+    std::vector<int> seg_m_host;
+    ax::KSpace ks{};
+    int64_t partial_slots = 0;  // 64x64 partial tiles the workspace holds
     struct DoneMark *done = nullptr;  // recorded after the plan's last launch set
     hipEvent_t ready = nullptr; // recorded after the set-up upload
     bool ready_pending = true;  // launches must still wait on `ready`
@@ -120,6 +127,77 @@ void apply_schedule(ax_rk_tensor *p) {
     p->host.tile_jobs = p->tile_jobs[S - 1];
     p->host.n_groups = p->n_jobs[S - 1];
     p->host.needs_reduce = S > 1;
+}
+
+// Builds and uploads the job tables of K-split level S if the plan does not
+// hold them yet, growing the partial workspace when the level needs more
+// slots (the old workspace stays allocated until the plan is destroyed:
+// launches already queued or captured keep using it).  Synchronous.
+ax_status ensure_tables(ax_rk_tensor *p, int S) {
+    if (S < 1 || S > p->split_cap) {
+        ax::set_error("K split %d outside [1, %d]", S, p->split_cap);
+        return AX_ERR_VALUE;
+    }
+    if (p->jobs_tf[S - 1] != nullptr) return AX_OK;
+    const ax::JobTables t = ax::make_jobs(p->work_host, p->seg_m_host, S, p->ks);
+    const ax::BalancedJobs b =
+        ax::make_balanced_jobs(p->work_host, p->seg_m_host, ax::balanced_groups(S), p->ks);
+    const int64_t slots = std::max<int64_t>((int64_t)p->n_work * S, (int64_t)b.jobs.size());
+    Layout L;
+    const int64_t o_tf = L.add((int64_t)t.tile_fastest.size() * sizeof(ax::RkJob));
+    const int64_t o_sf = L.add((int64_t)t.split_fastest.size() * sizeof(ax::RkJob));
+    const int64_t o_hv = L.add((int64_t)t.heaviest.size() * sizeof(ax::RkJob));
+    const int64_t o_tj = L.add((int64_t)t.tiles.size() * sizeof(ax::RkTileJobs));
+    const int64_t o_jb = L.add((int64_t)b.jobs.size() * sizeof(ax::RkJob));
+    const int64_t o_gb = L.add((int64_t)b.groups.size() * sizeof(int));
+    const int64_t o_tjb = L.add((int64_t)b.tiles.size() * sizeof(ax::RkTileJobs));
+    const int64_t head = L.size;
+    const bool grow = slots > p->partial_slots;
+    const int64_t o_part = grow ? L.add(slots * ax::gemm_tile() * ax::gemm_tile() * 8) : -1;
+    std::vector<char> img(static_cast<size_t>(head), 0);
+    auto put = [&](int64_t off, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(img.data() + off, src, bytes);
+    };
+    put(o_tf, t.tile_fastest.data(), t.tile_fastest.size() * sizeof(ax::RkJob));
+    put(o_sf, t.split_fastest.data(), t.split_fastest.size() * sizeof(ax::RkJob));
+    put(o_hv, t.heaviest.data(), t.heaviest.size() * sizeof(ax::RkJob));
+    put(o_tj, t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
+    put(o_jb, b.jobs.data(), b.jobs.size() * sizeof(ax::RkJob));
+    put(o_gb, b.groups.data(), b.groups.size() * sizeof(int));
+    put(o_tjb, b.tiles.data(), b.tiles.size() * sizeof(ax::RkTileJobs));
+    void *blk = nullptr;
+    hipError_t e = ax::pool_alloc(&blk, static_cast<size_t>(L.size));
+    if (e != hipSuccess) {
+        ax::set_error("device allocation of %lld bytes failed: %s", (long long)L.size,
+                      hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? AX_ERR_MEMORY : AX_ERR_HIP;
+    }
+    hipStream_t ss = ax::setup_stream();
+    e = hipMemcpyAsync(blk, img.data(), static_cast<size_t>(head), hipMemcpyHostToDevice, ss);
+    if (e == hipSuccess) e = hipStreamSynchronize(ss);
+    if (e != hipSuccess) {
+        ax::pool_free(blk);
+        ax::set_error("job table upload failed: %s", hipGetErrorString(e));
+        return AX_ERR_HIP;
+    }
+    p->allocs.push_back(blk);
+    p->footprint += static_cast<int64_t>(ax::pool_class(static_cast<size_t>(L.size)));
+    char *base = static_cast<char *>(blk);
+    p->jobs_tf[S - 1] = reinterpret_cast<const ax::RkJob *>(base + o_tf);
+    p->jobs_sf[S - 1] = reinterpret_cast<const ax::RkJob *>(base + o_sf);
+    p->jobs_hv[S - 1] = reinterpret_cast<const ax::RkJob *>(base + o_hv);
+    p->tile_jobs[S - 1] = reinterpret_cast<const ax::RkTileJobs *>(base + o_tj);
+    p->n_jobs[S - 1] = static_cast<int>(t.tile_fastest.size());
+    p->jobs_bal[S - 1] = reinterpret_cast<const ax::RkJob *>(base + o_jb);
+    p->groups_bal[S - 1] = reinterpret_cast<const int *>(base + o_gb);
+    p->tile_jobs_bal[S - 1] = reinterpret_cast<const ax::RkTileJobs *>(base + o_tjb);
+    p->n_groups_bal[S - 1] = static_cast<int>(b.groups.size()) - 1;
+    p->reduce_bal[S - 1] = b.jobs.size() > b.tiles.size() ? 1 : 0;
+    if (grow) {
+        p->host.partial = reinterpret_cast<double *>(base + o_part);
+        p->partial_slots = slots;
+    }
+    return AX_OK;
 }
 
 void free_plan(ax_rk_tensor *p) {
@@ -360,14 +438,14 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     std::vector<int> seg_m;
     for (int k : seg_k) seg_m.push_back(m_k[k]);
     const ax::KSpace ks = ax::k_space(n, precision == AX_PREC_FP64);
+    // the default level's job tables only (other levels on demand: building
+    // all 16 costs a request-sized plan milliseconds it rarely needs)
     std::vector<ax::JobTables> jobs;
     std::vector<ax::BalancedJobs> bal;
-    int64_t slots = (int64_t)n_work * split_cap;  // partial tiles the schedules need
-    for (int S = 1; S <= split_cap; ++S) {
-        jobs.push_back(ax::make_jobs(work, seg_m, S, ks));
-        bal.push_back(ax::make_balanced_jobs(work, seg_m, ax::balanced_groups(S), ks));
-        slots = std::max<int64_t>(slots, (int64_t)bal.back().jobs.size());
-    }
+    jobs.push_back(ax::make_jobs(work, seg_m, n_split, ks));
+    bal.push_back(ax::make_balanced_jobs(work, seg_m, ax::balanced_groups(n_split), ks));
+    const int64_t slots = std::max<int64_t>((int64_t)n_work * n_split,
+                                            (int64_t)bal.back().jobs.size());
     std::vector<int> ident((size_t)n_work * split_cap + 1);
     for (size_t i = 0; i < ident.size(); ++i) ident[i] = static_cast<int>(i);
 
@@ -437,6 +515,18 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     plan->m_k = m_k;
     plan->out_off = out_off;
     plan->out_size = outsz;
+    plan->work_host = work;
+    plan->seg_m_host = seg_m;
+    plan->ks = ks;
+    plan->partial_slots = slots;
+    for (auto *v : {&plan->jobs_tf, &plan->jobs_sf, &plan->jobs_hv, &plan->jobs_bal})
+        v->assign(split_cap, nullptr);
+    plan->tile_jobs.assign(split_cap, nullptr);
+    plan->tile_jobs_bal.assign(split_cap, nullptr);
+    plan->groups_bal.assign(split_cap, nullptr);
+    plan->n_jobs.assign(split_cap, 0);
+    plan->n_groups_bal.assign(split_cap, 0);
+    plan->reduce_bal.assign(split_cap, 0);
 
     void *blk = nullptr;
     hipError_t he = ax::pool_alloc(&blk, static_cast<size_t>(L.size));
@@ -531,28 +621,29 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
         }
         put(o_segs, segs.data(), segs.size() * sizeof(ax::RkSeg));
         put(o_work, work.data(), work.size() * sizeof(ax::RkWork));
-        for (size_t i = 0; i < jobs.size(); ++i) {
-            const ax::JobTables &t = jobs[i];
-            put(o_jtf[i], t.tile_fastest.data(), t.tile_fastest.size() * sizeof(ax::RkJob));
-            put(o_jsf[i], t.split_fastest.data(), t.split_fastest.size() * sizeof(ax::RkJob));
-            put(o_jhv[i], t.heaviest.data(), t.heaviest.size() * sizeof(ax::RkJob));
-            put(o_tj[i], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
-            plan->jobs_tf.push_back(static_cast<const ax::RkJob *>(dptr(o_jtf[i])));
-            plan->jobs_sf.push_back(static_cast<const ax::RkJob *>(dptr(o_jsf[i])));
-            plan->jobs_hv.push_back(static_cast<const ax::RkJob *>(dptr(o_jhv[i])));
-            plan->tile_jobs.push_back(static_cast<const ax::RkTileJobs *>(dptr(o_tj[i])));
-            plan->n_jobs.push_back(static_cast<int>(t.tile_fastest.size()));
+        const int d = n_split - 1;  // the default level's tables
+        {
+            const ax::JobTables &t = jobs[0];
+            put(o_jtf[0], t.tile_fastest.data(), t.tile_fastest.size() * sizeof(ax::RkJob));
+            put(o_jsf[0], t.split_fastest.data(), t.split_fastest.size() * sizeof(ax::RkJob));
+            put(o_jhv[0], t.heaviest.data(), t.heaviest.size() * sizeof(ax::RkJob));
+            put(o_tj[0], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
+            plan->jobs_tf[d] = static_cast<const ax::RkJob *>(dptr(o_jtf[0]));
+            plan->jobs_sf[d] = static_cast<const ax::RkJob *>(dptr(o_jsf[0]));
+            plan->jobs_hv[d] = static_cast<const ax::RkJob *>(dptr(o_jhv[0]));
+            plan->tile_jobs[d] = static_cast<const ax::RkTileJobs *>(dptr(o_tj[0]));
+            plan->n_jobs[d] = static_cast<int>(t.tile_fastest.size());
         }
-        for (size_t i = 0; i < bal.size(); ++i) {
-            const ax::BalancedJobs &t = bal[i];
-            put(o_jb[i], t.jobs.data(), t.jobs.size() * sizeof(ax::RkJob));
-            put(o_gb[i], t.groups.data(), t.groups.size() * sizeof(int));
-            put(o_tjb[i], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
-            plan->jobs_bal.push_back(static_cast<const ax::RkJob *>(dptr(o_jb[i])));
-            plan->groups_bal.push_back(static_cast<const int *>(dptr(o_gb[i])));
-            plan->tile_jobs_bal.push_back(static_cast<const ax::RkTileJobs *>(dptr(o_tjb[i])));
-            plan->n_groups_bal.push_back(static_cast<int>(t.groups.size()) - 1);
-            plan->reduce_bal.push_back(t.jobs.size() > t.tiles.size() ? 1 : 0);
+        {
+            const ax::BalancedJobs &t = bal[0];
+            put(o_jb[0], t.jobs.data(), t.jobs.size() * sizeof(ax::RkJob));
+            put(o_gb[0], t.groups.data(), t.groups.size() * sizeof(int));
+            put(o_tjb[0], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
+            plan->jobs_bal[d] = static_cast<const ax::RkJob *>(dptr(o_jb[0]));
+            plan->groups_bal[d] = static_cast<const int *>(dptr(o_gb[0]));
+            plan->tile_jobs_bal[d] = static_cast<const ax::RkTileJobs *>(dptr(o_tjb[0]));
+            plan->n_groups_bal[d] = static_cast<int>(t.groups.size()) - 1;
+            plan->reduce_bal[d] = t.jobs.size() > t.tiles.size() ? 1 : 0;
         }
         put(o_ident, ident.data(), ident.size() * sizeof(int));
         plan->identity = static_cast<const int *>(dptr(o_ident));
@@ -738,6 +829,8 @@ ax_status ax_rk_tensor_set_schedule(ax_rk_tensor *plan, int order, int n_split) 
     AX_REQUIRE(n_split >= 1 && n_split <= plan->split_cap,
                "ax_rk_tensor_set_schedule: split %d outside [1, %d]", n_split, plan->split_cap);
     if (plan->done) AX_CHECK_HIP(hipEventSynchronize(plan->done->ev));
+    const ax_status st = ensure_tables(plan, n_split);
+    if (st != AX_OK) return st;
     plan->host.order = order;
     plan->host.n_split = n_split;
     plan->n_split = n_split;
@@ -762,6 +855,8 @@ ax_status ax_rk_tensor_tune(ax_rk_tensor *plan, const double *P, const double *Q
     AX_CHECK_HIP(hipStreamIsCapturing(s, &cs));
     AX_REQUIRE(cs == hipStreamCaptureStatusNone, "ax_rk_tensor_tune: stream is capturing");
     ax_status st = ax_rk_tensor_run(plan, P, Q, out, stream);
+    for (int split = 1; split <= plan->split_cap && st == AX_OK; ++split)
+        st = ensure_tables(plan, split);  // every level's job tables, once
     if (st != AX_OK) return st;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     AX_CHECK_HIP(hipEventCreate(&e0));

@@ -388,9 +388,16 @@ class GpuExecutor:
         # device staging bytes charged to the launch set in flight on each slot
         self._slot_need = [0] * n_streams
         # host copies into the staging, split across threads (numpy copies
-        # run without the interpreter lock)
-        self._pack_threads = max(1, min(8, (os.cpu_count() or 1) // 2))
-        self._pool = ThreadPoolExecutor(self._pack_threads) if self._pack_threads > 1 else None
+        # run without the interpreter lock); the workers use this device
+        try:
+            cpus = len(os.sched_getaffinity(0))
+        except AttributeError:  # pragma: no cover
+            cpus = os.cpu_count() or 1
+        self._pack_threads = max(1, min(8, cpus // 2))
+        device = torch.cuda.current_device()
+        self._pool = (ThreadPoolExecutor(self._pack_threads,
+                                         initializer=lambda: torch.cuda.set_device(device))
+                      if self._pack_threads > 1 else None)
         self.states = collections.OrderedDict()  # resident, LRU order
         self._idle_bytes = 0                     # bytes of idle resident states
         self._estimates = {}                     # request shape -> (bytes, R^k count)
@@ -801,21 +808,34 @@ class GpuExecutor:
         entries = []  # (request index, atom index or -1, plan, P, Q)
         f64 = np.float64
         tune = []  # new resident plans to tune on this launch set's inputs
-        for i, req in enumerate(reqs):
-            batch = req.kind == "slater_batch"
-            atoms = req.atoms if batch else (req,)
-            got, sts, fresh = [], [], []
+        # the states of every request first (a request whose state cannot be
+        # had fails alone), then the plans, checks and staging request by
+        # request.  (Creating a set's plans on several threads measured
+        # slower: plan set-up is bound by HIP calls that serialise.)
+        claimed = []
+        for req in reqs:
+            atoms = req.atoms if req.kind == "slater_batch" else (req,)
+            sts, err = [], None
             try:
-                for j, a in enumerate(atoms):
-                    st = self._state_for(a)
-                    sts.append(st)
+                for a in atoms:
+                    sts.append(self._state_for(a))
+            except Exception as exc:
+                err = exc
+            claimed.append((atoms, sts, err))
+        for i, (req, (atoms, sts, err)) in enumerate(zip(reqs, claimed)):
+            batch = req.kind == "slater_batch"
+            got, fresh = [], []
+            try:
+                if err is not None:
+                    raise err
+                for j, (a, st) in enumerate(zip(atoms, sts)):
                     plan = st.plan
                     if plan is None:
                         plan = st.plan = SlaterTensor(a.gr, a.grw, a.ls, a.kmax, a.precision)
                         self.plans_created += 1
                         # a resident state of a large tensor is tuned on its first request
                         if st.key is not None and plan.size >= self._TUNE_MIN_RK:
-                            fresh.append(plan)
+                            fresh.append((plan, st))
                     P, Q = a.P, a.Q
                     if type(P) is not np.ndarray or P.dtype != f64:
                         P = np.asarray(P, dtype=f64)
@@ -869,12 +889,15 @@ class GpuExecutor:
             out += plan.size
             plans.append(plan)
         dev[:n_in].copy_(host[:n_in], non_blocking=True)
-        for t in tune:  # times every contraction schedule on the staged inputs
+        for t, st in tune:  # times every contraction schedule on the staged inputs
             k = plans.index(t)
             try:
                 t.tune_raw(pp[k], qq[k], oo[k], stream=stream)
             except (RuntimeError, ValueError, MemoryError):  # tuning is optional: the
                 pass  # plan keeps the schedule it had (ax_rk_tensor_tune restores it)
+            # the other K splits' job tables (and a larger partial workspace)
+            # now belong to the resident state
+            st.nbytes = max(st.nbytes, t.footprint)
         run_batch_raw(plans, pp, qq, oo, stream=stream)
         job["results"] = results
         job["n_in"] = n_in

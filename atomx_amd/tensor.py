@@ -62,8 +62,14 @@ class SlaterTensor:
         self.size = int(lib.ax_rk_tensor_size(h))
         self.m = [int(lib.ax_rk_tensor_pairs(h, k)) for k in range(self.kmax + 1)]
         self.offsets = [int(lib.ax_rk_tensor_offset(h, k)) for k in range(self.kmax + 1)]
-        self.footprint = int(lib.ax_rk_tensor_footprint(h))
         self.slices = int(lib.ax_rk_tensor_slices(h))
+
+    @property
+    def footprint(self):
+        """Device bytes the plan holds now (tuning or a schedule of another
+        K split adds that split's job tables, and partial workspace when it
+        needs more)."""
+        return int(self._lib.ax_rk_tensor_footprint(self._h))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

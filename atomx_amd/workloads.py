@@ -86,16 +86,23 @@ def pair_list(n_orb):
 
 
 def count_rk(ls, kmax):
-    """Number of R^k values of the all-quartet workload (pair-pairs once)."""
-    ls = np.asarray(ls, dtype=np.int64)
-    a, b = np.triu_indices(ls.size)
-    la, lb = ls[a], ls[b]
-    lo, hi, par = np.abs(la - lb), la + lb, (la + lb) % 2
-    total = 0
-    for k in range(kmax + 1):
-        m = int(np.count_nonzero((lo <= k) & (k <= hi) & (par == k % 2)))
-        total += m * (m + 1) // 2
-    return total
+    """Number of R^k values of the all-quartet workload (pair-pairs once),
+    counted by angular momentum class: the pairs of orbitals with l1 < l2
+    number c(l1) c(l2), those with l1 = l2 number c(c + 1) / 2, and a class
+    is valid at rank k when |l1 - l2| <= k <= l1 + l2 and l1 + l2 + k is
+    even (as ``valid_pairs``)."""
+    cnt = {}
+    for l in (ls.tolist() if isinstance(ls, np.ndarray) else list(ls)):
+        cnt[l] = cnt.get(l, 0) + 1
+    cls = sorted(cnt.items())
+    kmax = int(kmax)
+    m = [0] * (kmax + 1)
+    for i, (l1, c1) in enumerate(cls):
+        for l2, c2 in cls[i:]:
+            n = c1 * (c1 + 1) // 2 if l1 == l2 else c1 * c2
+            for k in range(l2 - l1, min(l1 + l2, kmax) + 1, 2):  # k = l2 - l1, ..., l1 + l2
+                m[k] += n
+    return sum(x * (x + 1) // 2 for x in m)
 
 
 # ---------------------------------------------------------------------------

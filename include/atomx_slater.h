@@ -202,7 +202,10 @@ int64_t ax_rk_tensor_size(const ax_rk_tensor *plan);
 int ax_rk_tensor_pairs(const ax_rk_tensor *plan, int k);
 /* offset of rank k's packed triangle inside the output */
 int64_t ax_rk_tensor_offset(const ax_rk_tensor *plan, int k);
-/* device bytes held by the plan (admission accounting) */
+/* device bytes held by the plan now (admission accounting): creation
+ * holds the default K split's job tables and partial workspace; a schedule
+ * of another split (set_schedule, tune) adds that split's tables, and a
+ * larger workspace when it needs one */
 int64_t ax_rk_tensor_footprint(const ax_rk_tensor *plan);
 /* int8 slices per operand of a split-precision plan, from the radius ratio
  * rho one 64-node K-tile spans (cells after the first) and
@@ -249,7 +252,8 @@ ax_status ax_rk_tensor_run_stage(ax_rk_tensor *plan, int stage, const double *P,
  * blocks inside the triangle), go whole to the least-loaded of that XCD's
  * 8 x S work-groups.  The schedule fixes how each
  * tile's K range is cut and the order its pieces are summed, so results
- * are deterministic per schedule.
+ * are deterministic per schedule.  A split's job tables are built and
+ * uploaded the first time a schedule of that split is set (synchronous).
  * tune() runs the plan once on P, Q, then times every (order, split) with
  * `reps` timed repetitions of the contraction and reduction, keeps the
  * fastest, and leaves its result in `out`; best_ms_host (may be NULL)
