@@ -112,6 +112,10 @@ SIGNATURES = {
     "ax_bilinear_stage2": (_int, [_vp, _int, _int, _int, _vp, _int, _vp, _vp]),
     "ax_fock_finalize": (_int, [_vp, _vp, _vp, _vp, _int, ctypes.c_double, _int, _vp]),
     "ax_sum_amb_ptr": (_int, [_vp, _vp, _int, _int, _int, _int, _vp, _vp]),
+    "ax_spinor_eri_packed": (_int, [_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _int, _int,
+                                    _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "ax_spinor_eri_list": (_int, [_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
+                                  _vp, _vp, _vp]),
     "ax_feat_sample": (_int, [_vp, _int, _int, _int, _int, _vp, _i64, _int, _int, _vp, _i64,
                               _i64, _vp]),
     "ax_feat_apply": (_int, [_vp, _i64, _int, _i64, _vp, _vp, _vp, _vp, _int, _int, _vp, _vp,

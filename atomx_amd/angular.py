@@ -14,7 +14,7 @@ from functools import lru_cache
 import numpy as np
 
 __all__ = ["wigner_3j", "exchange_lambda", "exchange_k_range", "l_of_kappa", "twoj_of_kappa",
-           "AngularTables"]
+           "reduced_c_jj", "AngularTables"]
 
 
 def l_of_kappa(kappa: int) -> int:
@@ -60,6 +60,25 @@ def wigner_3j(tj1, tj2, tj3, tm1, tm2, tm3) -> float:
         total += -term if t % 2 else term
     phase = -1.0 if ((tj1 - tj2 - tm3) // 2) % 2 else 1.0
     return phase * total
+
+
+@lru_cache(maxsize=None)
+def reduced_c_jj(kappa_a, kappa_b, k) -> float:
+    """<kappa_a || C^k || kappa_b> in Grant's phase convention
+    (angular/coefficients.py:29-65): (-1)^(j_a + 1/2) sqrt((2j_a+1)(2j_b+1))
+    (j_a k j_b; -1/2 0 1/2), zero unless l_a + k + l_b is even and
+    |j_a - j_b| <= k <= j_a + j_b.  At k = 0 it is -sqrt(2j + 1)."""
+    la, lb = l_of_kappa(kappa_a), l_of_kappa(kappa_b)
+    if (la + k + lb) % 2:
+        return 0.0
+    ta, tb = twoj_of_kappa(kappa_a), twoj_of_kappa(kappa_b)
+    if not abs(ta - tb) <= 2 * k <= ta + tb:
+        return 0.0
+    w = wigner_3j(ta, 2 * k, tb, -1, 0, 1)
+    if abs(w) < 1e-15:
+        return 0.0
+    sign = -1.0 if ((ta + 1) // 2) % 2 else 1.0
+    return sign * math.sqrt((ta + 1) * (tb + 1)) * w
 
 
 def exchange_lambda(kappa_a, kappa_b, k) -> float:

@@ -352,6 +352,29 @@ class DiracFockSCF:
                 self.P[i], self.Q[i], self.eps[i] = self._root(i, w, v, False)
         return self
 
+    def virtual_orbitals(self, kappa, count):
+        """The ``count`` lowest physical solutions of the one-body Dirac
+        operator of ``kappa`` that are orthogonal, in that kappa's metric, to
+        this SCF's orbitals of the same kappa: an orthonormal extension of
+        the occupied space (bare-nucleus virtuals) for correlation bases.
+        Returns [(P, Q, energy)]."""
+        H, S = self.h0[kappa] if kappa in self.h0 else dirac_matrices(
+            kappa, self.Z, self.bd, self.nuclear, self.A)
+        occ = [np.concatenate((self.P[j], self.Q[j])) for j, o in enumerate(self.shells)
+               if o.kappa == kappa]
+        with one_blas_thread():
+            w, v = solve_dirac(H, S, self.ns, self.nsp, self.nsq,
+                               orthogonal_to=np.column_stack(occ) if occ else None)
+        out = []
+        for j in range(min(count, v.shape[1])):
+            P = np.zeros(self.ns)
+            Q = np.zeros(self.ns)
+            P[:self.nsp] = v[:self.nsp, j]
+            Q[:self.nsq] = v[self.ns:self.ns + self.nsq, j]
+            P, Q = _phase(P, Q)
+            out.append((P, Q, float(w[j])))
+        return out
+
     def set_orbitals(self, P, Q):
         self.P[:] = 0.0
         self.Q[:] = 0.0

@@ -497,6 +497,37 @@ ax_status ax_cpc_apply(const void *rho, int nr, int ns, const void *A, int64_t l
                        const double *w, int64_t n_edges, int n_split, int is_complex, void *work,
                        void *out, void *stream);
 
+/* ---- spinor integrals for correlation engines ---------------------------
+ * physics/csf/spinor_integrals.py:258-466 (build_spinor_integrals): the
+ * signed antisymmetrised integrals g(p,q,r,s) of spinors |n kappa m>, with
+ * the angular factor of angular/csf_hamiltonian.py:874-920 split per
+ * electron: <ab|cd> = (-1)^((m_a - m_c)) sum_k T[k][a][c] T[k][b][d]
+ * R[k][pair(rad a, rad c)][pair(rad b, rad d)] when m_a + m_b = m_c + m_d
+ * (else 0), T[k][a][c] = (-1)^(j_a - m_a) (j_a k j_c; -m_a, m_a - m_c, m_c)
+ * <kappa_a||C^k||kappa_c>; g = (x + y) / 2, x = <pq|rs> - <pq|sr>,
+ * y = <rs|pq> - <rs|qp>, and max_err (one double, zeroed by the caller)
+ * receives the largest |x - y| (the Hermiticity gate of :389-399).
+ * Device pointers: rad, m2 (n, int32: radial orbital, 2m), pair_id (n_rad x
+ * n_rad, int32, symmetric), T ((kmax+1) n n), R ((kmax+1) n_rpair n_rpair,
+ * rows the target pair). */
+/* The packed layout of spinor_integrals.py:20-50 (packed_size) over the
+ * first n_packed spinors (2 <= n_packed <= n; the tables may hold further,
+ * frozen, spinors): for the (p, q) blocks b = 0..nblk-1 (p < q, block
+ * offsets blk_off), every (r, s) with p <= r < s < n_packed at
+ * out[blk_off[b] - base + m r' + s' - (r'+1)(r'+2)/2], m = n_packed - p,
+ * r' = r - p, s' = s - p. */
+ax_status ax_spinor_eri_packed(int n, int kmax, int n_rad, int n_rpair, const int32_t *rad,
+                               const int32_t *m2, const int32_t *pair_id, const double *T,
+                               const double *R, int n_packed, int nblk, const int32_t *blk_p,
+                               const int32_t *blk_q, const int64_t *blk_off, int64_t base,
+                               double *out, double *max_err, void *stream);
+/* g of an explicit list of quartets (4 int32 spinor indices each): the
+ * Fock, frozen-core and reference-energy contractions of :401-432. */
+ax_status ax_spinor_eri_list(int n, int kmax, int n_rad, int n_rpair, const int32_t *rad,
+                             const int32_t *m2, const int32_t *pair_id, const double *T,
+                             const double *R, int64_t count, const int32_t *quartets, double *out,
+                             double *max_err, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
