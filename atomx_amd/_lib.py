@@ -116,6 +116,10 @@ SIGNATURES = {
                                     _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "ax_spinor_eri_list": (_int, [_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
                                   _vp, _vp, _vp]),
+    "ax_bilinear_stage2_batch": (_int, [_int, _int, _int, _int, _vp, _vp]),
+    "ax_mbpt2_workspace_doubles": (_i64, [_int, _int]),
+    "ax_mbpt2_pair_sum": (_int, [_int, _int, _int, _int, _int, _vp, _vp, _vp, _i64, _i64, _vp,
+                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ax_feat_sample": (_int, [_vp, _int, _int, _int, _int, _vp, _i64, _int, _int, _vp, _i64,
                               _i64, _vp]),
     "ax_feat_apply": (_int, [_vp, _i64, _int, _i64, _vp, _vp, _vp, _vp, _int, _int, _vp, _vp,

@@ -13,8 +13,8 @@ from functools import lru_cache
 
 import numpy as np
 
-__all__ = ["wigner_3j", "exchange_lambda", "exchange_k_range", "l_of_kappa", "twoj_of_kappa",
-           "reduced_c_jj", "AngularTables"]
+__all__ = ["wigner_3j", "wigner_6j", "exchange_lambda", "exchange_k_range", "l_of_kappa",
+           "twoj_of_kappa", "reduced_c_jj", "AngularTables"]
 
 
 def l_of_kappa(kappa: int) -> int:
@@ -60,6 +60,33 @@ def wigner_3j(tj1, tj2, tj3, tm1, tm2, tm3) -> float:
         total += -term if t % 2 else term
     phase = -1.0 if ((tj1 - tj2 - tm3) // 2) % 2 else 1.0
     return phase * total
+
+
+def _closes(ta, tb, tc):
+    return (ta + tb + tc) % 2 == 0 and abs(ta - tb) <= tc <= ta + tb
+
+
+@lru_cache(maxsize=None)
+def wigner_6j(tj1, tj2, tj3, tj4, tj5, tj6) -> float:
+    """Wigner 6j symbol {j1 j2 j3; j4 j5 j6}, all arguments twice the angular
+    momenta: Racah's single sum over t of (-1)^t (t+1)! over the factorials
+    of (t - triad sums) and (tetrad sums - t), times the four triangle
+    coefficients of (j1 j2 j3), (j1 j5 j6), (j4 j2 j6), (j4 j5 j3), in
+    log-factorials.  Zero unless the four triads close."""
+    triads = ((tj1, tj2, tj3), (tj1, tj5, tj6), (tj4, tj2, tj6), (tj4, tj5, tj3))
+    if not all(_closes(*t) for t in triads):
+        return 0.0
+    log_tri = 0.5 * sum(_lf((a + b - c) // 2) + _lf((a - b + c) // 2) + _lf((b + c - a) // 2)
+                        - _lf((a + b + c) // 2 + 1) for a, b, c in triads)
+    lo = [sum(t) // 2 for t in triads]
+    hi = ((tj1 + tj2 + tj4 + tj5) // 2, (tj2 + tj3 + tj5 + tj6) // 2,
+          (tj3 + tj1 + tj6 + tj4) // 2)
+    total = 0.0
+    for t in range(max(lo), min(hi) + 1):
+        lg = _lf(t + 1) - sum(_lf(t - x) for x in lo) - sum(_lf(x - t) for x in hi)
+        term = math.exp(log_tri + lg)
+        total += -term if t % 2 else term
+    return total
 
 
 @lru_cache(maxsize=None)

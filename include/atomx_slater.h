@@ -528,6 +528,40 @@ ax_status ax_spinor_eri_list(int n, int kmax, int n_rad, int n_rpair, const int3
                              const double *R, int64_t count, const int32_t *quartets, double *out,
                              double *max_err, void *stream);
 
+/* ---- second-order MBPT on a closed-shell reference (methods/mbpt.py) ---- */
+/* Batched stage 2 of the bilinear R^k forms (numerics/bspline/convol.py:
+ * 1900-1978, build_slater_rk_bilinear's gather over the right band pair,
+ * as ax_bilinear_stage2): job i (5 int64 each: stage-1 contraction T
+ * (nm x ns x ksr) address, right coefficient row wb address, output
+ * address, ksr, n_right) writes an nm x ns block with row stride ld, so the
+ * four component blocks land in one 2ns x 2ns form. */
+ax_status ax_bilinear_stage2_batch(int ns, int nm, int ld, int n_jobs, const int64_t *jobs,
+                                   void *stream);
+/* Workspace (doubles) of ax_mbpt2_pair_sum for n_virt virtuals and n_pair
+ * pairs. */
+int64_t ax_mbpt2_workspace_doubles(int n_virt, int n_pair);
+/* The J-coupled MBPT(2) pair sums of _mbpt2_pair_contribution
+ * (methods/mbpt.py:602-790): for every occupied pair p and virtual pair
+ * r <= s (virtuals ordered by kappa channel, so r <= s is the reference's
+ * unordered channel sum), amp_J = sum_k cd[c][J][k] T^k[r][s] -
+ * ce[c][J][k] T^k[s][r] with T^k the Slater table R^k(a r, b s) of item
+ * pair_item[p][k] (-1 absent; entry (item, r, s) at tables[item *
+ * item_stride + r * row_stride + s]) and c the channel of (pair_type[p],
+ * vkappa[r], vkappa[s]) (chan_nj[c] J values from 2J = chan_tjlo[c], step 2;
+ * 0 = no contribution); E_p = -sum (2J+1) amp_J^2 / (pair_norm[p] n_rs
+ * (veps[r] + veps[s] - pair_eps[p])), n_rs = 2 for r = s, skipped where the
+ * denominator is below 1e-10 in magnitude.  pair_energy[p] is written
+ * (fixed-order sums); jsum (nullable, n_pair x N x N) receives sum_J
+ * (2J+1) amp_J^2 of every counted r <= s entry. */
+ax_status ax_mbpt2_pair_sum(int n_virt, int n_kappa, int n_rank, int n_pair, int max_j,
+                            const int32_t *vkappa, const double *veps, const double *tables,
+                            int64_t item_stride, int64_t row_stride, const int32_t *pair_item,
+                            const double *pair_eps,
+                            const double *pair_norm, const int32_t *pair_type,
+                            const int32_t *chan_nj, const int32_t *chan_tjlo, const double *cd,
+                            const double *ce, double *pair_energy, double *jsum,
+                            double *workspace, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

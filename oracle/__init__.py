@@ -686,6 +686,36 @@ def wigner_3j_exact(tj1, tj2, tj3, tm1, tm2, tm3):
     return float(phase) * math.copysign(1.0, float(s)) * math.sqrt(float(pre * s * s))
 
 
+@lru_cache(maxsize=None)
+def wigner_6j_exact(tj1, tj2, tj3, tj4, tj5, tj6):
+    """Exact 6j symbol {j1 j2 j3; j4 j5 j6} (twice-j integer arguments) by
+    Racah's single sum in rationals: the four triangle coefficients' squares
+    and the alternating sum are exact, one square root at the end.
+    Independent of the product's implementation (log-factorials)."""
+    triads = ((tj1, tj2, tj3), (tj1, tj5, tj6), (tj4, tj2, tj6), (tj4, tj5, tj3))
+    for a, b, c in triads:
+        if (a + b + c) % 2 or not abs(a - b) <= c <= a + b:
+            return 0.0
+    tri2 = Fraction(1)
+    for a, b, c in triads:
+        tri2 *= Fraction(_fact((a + b - c) // 2) * _fact((a - b + c) // 2)
+                         * _fact((b + c - a) // 2), _fact((a + b + c) // 2 + 1))
+    lo = [(a + b + c) // 2 for a, b, c in triads]
+    hi = [(tj1 + tj2 + tj4 + tj5) // 2, (tj2 + tj3 + tj5 + tj6) // 2,
+          (tj3 + tj1 + tj6 + tj4) // 2]
+    s = Fraction(0)
+    for t in range(max(lo), min(hi) + 1):
+        den = 1
+        for x in lo:
+            den *= _fact(t - x)
+        for x in hi:
+            den *= _fact(x - t)
+        s += Fraction((-1) ** t * _fact(t + 1), den)
+    if s == 0:
+        return 0.0
+    return math.copysign(1.0, float(s)) * math.sqrt(float(tri2 * s * s))
+
+
 def l_of_kappa(kappa):
     return kappa if kappa > 0 else -kappa - 1
 

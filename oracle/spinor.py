@@ -81,6 +81,37 @@ def packed_size(n):
     return sum((n - p - 1) * comb(n - p, 2) for p in range(n - 1))
 
 
+class SignedIntegrals:
+    """g(a, b, c, d) of spinor labels (n, kappa, 2m) over the radial
+    orbitals ``orbitals[(n, kappa)] = (P, Q)``: direct integrals from
+    two_body_angular and the memoised scalar cell R^k(ac, bd), antisymmetrised
+    and Hermitian-projected as spinor_integrals.py:389-399."""
+
+    def __init__(self, orbitals, bsp_data, banks=None):
+        self.orbitals, self.bsp_data = orbitals, bsp_data
+        self.banks = Banks(bsp_data) if banks is None else banks
+        self.memo = {}
+
+    def radial(self, k, a, c, b, d):
+        key = (k, a, c, b, d)
+        if key not in self.memo:
+            o = self.orbitals
+            self.memo[key] = compute_slater_rk_cell(*o[a][:2], *o[c][:2], *o[b][:2], *o[d][:2],
+                                                    self.bsp_data, k, self.banks)
+        return self.memo[key]
+
+    def direct(self, a, b, c, d):
+        return sum(v * self.radial(k, a[:2], c[:2], b[:2], d[:2])
+                   for k, v in two_body_angular(a[1], a[2], b[1], b[2], c[1], c[2], d[1], d[2]))
+
+    def g(self, a, b, c, d):
+        if a[2] + b[2] != c[2] + d[2]:
+            return 0.0
+        x = self.direct(a, b, c, d) - self.direct(a, b, d, c)
+        y = self.direct(c, d, a, b) - self.direct(c, d, b, a)
+        return 0.5 * (x + y)
+
+
 def spinor_integrals(orbitals, bsp_data, Z, ns, occupied_keys, virtual_keys, frozen_keys=(),
                      nuclear_potential=None):
     """(labels, noccupied, heff, fock, packed, frozen_energy, reference_energy)."""
@@ -109,15 +140,7 @@ def spinor_integrals(orbitals, bsp_data, Z, ns, occupied_keys, virtual_keys, fro
         return v
 
     vecs = {k: vec(k) for k in keys}
-    rk_memo = {}
-
-    def radial(k, a, c, b, d):
-        key = (k, a, c, b, d)
-        if key not in rk_memo:
-            rk_memo[key] = compute_slater_rk_cell(*orbitals[a][:2], *orbitals[c][:2],
-                                                  *orbitals[b][:2], *orbitals[d][:2], bsp_data,
-                                                  k, banks)
-        return rk_memo[key]
+    g = SignedIntegrals(orbitals, bsp_data, banks).g
 
     def h(a, b):
         if a[1:] != b[1:]:
@@ -125,17 +148,6 @@ def spinor_integrals(orbitals, bsp_data, Z, ns, occupied_keys, virtual_keys, fro
         H0, S = one[a[1]]
         ket = vecs[b[:2]]
         return (vecs[a[:2]] @ H0 @ ket) / (ket @ S @ ket)
-
-    def direct(a, b, c, d):
-        return sum(v * radial(k, a[:2], c[:2], b[:2], d[:2])
-                   for k, v in two_body_angular(a[1], a[2], b[1], b[2], c[1], c[2], d[1], d[2]))
-
-    def g(a, b, c, d):
-        if a[2] + b[2] != c[2] + d[2]:
-            return 0.0
-        x = direct(a, b, c, d) - direct(a, b, d, c)
-        y = direct(c, d, a, b) - direct(c, d, b, a)
-        return 0.5 * (x + y)
 
     frozen_energy = sum(h(a, a) for a in frozen_labels) + 0.5 * sum(
         g(a, b, a, b) for a in frozen_labels for b in frozen_labels)
