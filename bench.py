@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-ion-scf", action="store_true",
                     help="skip the batched SCF of the 26 iron ions")
     ap.add_argument("--no-neon", action="store_true", help="skip the neon Dirac-Fock SCF leg")
+    ap.add_argument("--no-correlation", action="store_true",
+                    help="skip the MBPT(2) and spinor-export leg (it runs on the neon leg's state)")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests in the served Poisson stream (0 = skip the serving leg)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -304,7 +306,65 @@ def neon_leg(args):
             "E_total_Ha": e, "sweeps": scf.iterations, "seconds": dt,
             "ms_per_sweep": dt / scf.iterations * 1e3,
             "orbital_uploads": scf.backend.uploads,
-            "fock_rk_equivalents_per_sweep": n_eq, "_bd": bd}
+            "fock_rk_equivalents_per_sweep": n_eq, "_bd": bd, "_scf": scf}
+
+
+def correlation_leg(neon):
+    """Consumers of the Slater integrals in correlation methods, on the
+    converged neon ground state of the neon leg (60 B-splines): MBPT(2) over
+    every physical V^N virtual with l <= 4 (kappa_max = 4, no caps), helium's
+    MBPT(2) on 80 B-splines against the reference's published value, and the
+    CC/FSCC signed spinor-integral export (1s..2p occupied; 3s 4s 3p 4p 3d
+    4d 4f virtuals from the bare-nucleus complement: 60 spinors, packed).
+    Each is timed on its second call (the first builds grid banks and
+    library handles)."""
+    import torch
+
+    from atomx_amd import bspline as B
+    from atomx_amd import dirac as D
+    from atomx_amd.mbpt import mbpt2_energy
+    from atomx_amd.spinor_integrals import build_spinor_integrals
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    scf, bd = neon["_scf"], neon["_bd"]
+    ne, dt_ne = timed(lambda: mbpt2_energy(scf, kappa_max=4))
+    t = B.make_knots(80, 7, 1e-5, 40.0)
+    bd_he = B.prepare_bspline_data(t, 80, 7)
+    he_scf = D.DiracFockSCF(2, [D.Subshell("1s", 1, -1, 2.0)], bd_he).bare_nucleus_start()
+    e_he = he_scf.run(tol=1e-9)
+    he, dt_he = timed(lambda: mbpt2_energy(he_scf, kappa_max=4))
+    orbitals = {(sh.n, sh.kappa): (scf.P[i], scf.Q[i]) for i, sh in enumerate(scf.shells)}
+    vir = []
+    for kappa, shells in ((-1, (3, 4)), (1, (3, 4)), (-2, (3, 4)), (2, (3, 4)), (-3, (3, 4)),
+                          (3, (4,)), (-4, (4,))):
+        for (P, Q, _), n in zip(scf.virtual_orbitals(kappa, len(shells)), shells):
+            orbitals[n, kappa] = (P, Q)
+            vir.append((n, kappa))
+    occ = [(1, -1), (2, -1), (2, 1), (2, -2)]
+    si, dt_si = timed(lambda: build_spinor_integrals(orbitals, bd, 10, bd["ns"],
+                                                     occupied_keys=occ, virtual_keys=vir))
+    return {"mbpt2_neon": {"workload": "Ne 1s2 2s2 2p6 (60 B-splines), kappa_max 4, every "
+                                       "physical V^N virtual",
+                           "E2_Ha": ne.correlation_energy, "virtuals": sum(ne.n_virtuals.values()),
+                           "occupied_pairs": len(ne.pair_contributions), "seconds": dt_ne},
+            "mbpt2_helium": {"workload": "He 1s2 (80 B-splines order 7, r 1e-5..40), kappa_max 4, "
+                                         "every physical V^N virtual",
+                             "E_DF_Ha": e_he, "E2_Ha": he.correlation_energy,
+                             "E2_reference_published_Ha": -0.036968,
+                             "virtuals": sum(he.n_virtuals.values()), "seconds": dt_he},
+            "spinor_export": {"workload": "Ne, 1s..2p occupied, 3s 4s 3p 4p 3d 4d 4f virtuals "
+                                          "(bare-nucleus complement), Dirac-Coulomb, packed",
+                              "spinors": len(si.labels), "packed_values": int(si.packed_eri_Ha.size),
+                              "E_reference_Ha": si.reference_energy_Ha,
+                              "abs_diff_vs_E_DF_Ha": abs(si.reference_energy_Ha - neon["E_total_Ha"]),
+                              "seconds": dt_si}}
 
 
 TRAFFIC_TABLE = "profiles/c2_hbm_traffic.txt"
@@ -657,6 +717,9 @@ def main():
         neon = neon_leg(args)
         line["neon_dirac_fock"] = {k: v for k, v in neon.items() if not k.startswith("_")}
         progress("neon SCF")
+        if not args.no_correlation:
+            line["correlation"] = correlation_leg(neon)
+            progress("MBPT(2) and spinor export")
     if args.serve_requests > 0:
         serving = serving_leg(args)
         progress("serving")
