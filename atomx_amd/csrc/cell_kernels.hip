@@ -383,33 +383,60 @@ __global__ __launch_bounds__(kT) void case1_kernel(const double *__restrict__ ba
 }
 
 // ---- (a15) case 2: a[j, jp] = sum_ip sum_{i<n_left, i+ip<n_left} bankT d_l[i, ip] --
-// one thread per (j, jp), the reference's (ip, i) accumulation order
-// (convol.py:852-874, :1310-1345).
-__global__ void case2_kernel(const double *__restrict__ bank, int ns, int ksl, int ksr,
-                             int n_left, const double *__restrict__ d, int64_t ld_d, int nb,
-                             double *__restrict__ out, int64_t ld_o) {
-    const int64_t total = (int64_t)ns * ksr;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int jp = static_cast<int>(t / ns);
-        const int j = static_cast<int>(t - (int64_t)jp * ns);
-        double acc[kMaxBatch];
+// One work-group per (jp, tile of 64 j, tile of 32 densities): chunks of 32
+// i of one ip are staged in LDS (the bank rows [i][j] and the densities
+// [b][i]) by all 256 threads at once, then each thread (j = lane, densities
+// 8 w .. 8 w + 7 of its wave w) accumulates its outputs from LDS.  Every
+// output is one sequential sum in the reference's (ip, i) order
+// (convol.py:852-874, :1310-1345), so results do not depend on the tiling.
+constexpr int kC2J = 64, kC2B = 32, kC2I = 32;
+
+__global__ __launch_bounds__(256) void case2_kernel(const double *__restrict__ bank, int ns,
+                                                    int ksl, int ksr, int n_left,
+                                                    const double *__restrict__ d, int64_t ld_d,
+                                                    int nb, double *__restrict__ out,
+                                                    int64_t ld_o) {
+    __shared__ double sB[kC2I][kC2J];
+    __shared__ double sD[kC2B][kC2I + 1];
+    const int jp = blockIdx.x;
+    const int j0 = blockIdx.y * kC2J;
+    const int b0 = blockIdx.z * kC2B;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = j0 + lane;
+    double acc[8];
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) acc[b] = 0.0;
-        if (j + jp < ns) {
-            for (int ip = 0; ip < ksl; ++ip) {
-                const double *col = bank + (((int64_t)ip * ksr + jp) * ns) * ns + j;
-                for (int i = 0; i + ip < n_left; ++i) {
-                    const double bv = col[(int64_t)i * ns];
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+    for (int ip = 0; ip < ksl; ++ip) {
+        const int n_i = n_left - ip;  // i < n_left - ip
+        const double *brow = bank + ((int64_t)ip * ksr + jp) * ns * ns;
+        for (int i0 = 0; i0 < n_i; i0 += kC2I) {
+            const int cnt = n_i - i0 < kC2I ? n_i - i0 : kC2I;
+            __syncthreads();
+            for (int e = threadIdx.x; e < kC2I * kC2J; e += 256) {
+                const int ii = e / kC2J, jj = e - (e / kC2J) * kC2J;
+                sB[ii][jj] = (ii < cnt && j0 + jj < ns) ? brow[(int64_t)(i0 + ii) * ns + j0 + jj]
+                                                        : 0.0;
+            }
+            for (int e = threadIdx.x; e < kC2B * kC2I; e += 256) {
+                const int bb = e / kC2I, ii = e - (e / kC2I) * kC2I;
+                sD[bb][ii] = (ii < cnt && b0 + bb < nb)
+                                 ? d[(int64_t)(b0 + bb) * ld_d + (int64_t)(i0 + ii) * ksl + ip]
+                                 : 0.0;
+            }
+            __syncthreads();
+            for (int ii = 0; ii < cnt; ++ii) {
+                const double bv = sB[ii][lane];
 #pragma unroll
-                    for (int b = 0; b < kMaxBatch; ++b)
-                        if (b < nb) acc[b] += bv * d[(int64_t)b * ld_d + (int64_t)i * ksl + ip];
-                }
+                for (int u = 0; u < 8; ++u) acc[u] += bv * sD[w * 8 + u][ii];
             }
         }
+    }
+    if (j < ns) {
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b)
-            if (b < nb) out[(int64_t)b * ld_o + (int64_t)j * ksr + jp] = acc[b];
+        for (int u = 0; u < 8; ++u) {
+            const int b = b0 + w * 8 + u;
+            if (b < nb) out[(int64_t)b * ld_o + (int64_t)j * ksr + jp] = j + jp < ns ? acc[u] : 0.0;
+        }
     }
 }
 
@@ -969,11 +996,14 @@ ax_status ax_convol_case2(const double *bankT, int ns, int ksl, int ksr, int n_l
     AX_REQUIRE(ns > 0 && ksl > 0 && ksr > 0 && nb >= 0 && n_left >= 0 && n_left <= ns &&
                    ld_d >= (int64_t)n_left * ksl && ld_o >= (int64_t)ns * ksr,
                "ax_convol_case2: bad sizes");
-    for (int b0 = 0; b0 < nb; b0 += kMaxBatch) {
-        const int cnt = nb - b0 < kMaxBatch ? nb - b0 : kMaxBatch;
-        hipLaunchKernelGGL(case2_kernel, dim3(blocks((int64_t)ns * ksr)), dim3(kT), 0,
-                           ax::as_stream(stream), bankT, ns, ksl, ksr, n_left,
-                           d + (int64_t)b0 * ld_d, ld_d, cnt, out + (int64_t)b0 * ld_o, ld_o);
+    // grid (jp, j tiles, density tiles); one launch per 65535 density tiles
+    const int jt = (ns + kC2J - 1) / kC2J;
+    constexpr int64_t kPerLaunch = (int64_t)65535 * kC2B;
+    for (int64_t b0 = 0; b0 < nb; b0 += kPerLaunch) {
+        const int cnt = static_cast<int>(nb - b0 < kPerLaunch ? nb - b0 : kPerLaunch);
+        hipLaunchKernelGGL(case2_kernel, dim3(ksr, jt, (cnt + kC2B - 1) / kC2B), dim3(256), 0,
+                           ax::as_stream(stream), bankT, ns, ksl, ksr, n_left, d + b0 * ld_d,
+                           ld_d, cnt, out + b0 * ld_o, ld_o);
         AX_CHECK_LAUNCH();
     }
     return AX_OK;
