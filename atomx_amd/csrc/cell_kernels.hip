@@ -350,6 +350,11 @@ __global__ __launch_bounds__(kT) void case1_kernel(const double *__restrict__ ba
                                                    int64_t ld_o) {
     __shared__ double red[kMaxBatch][kT / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // grid.y: groups of kMaxBatch densities
+    const int b0 = blockIdx.y * kMaxBatch;
+    d += (int64_t)b0 * ld_d;
+    out += (int64_t)b0 * ld_o;
+    nb = nb - b0 < kMaxBatch ? nb - b0 : kMaxBatch;
     for (int r = blockIdx.x; r < ns * ksl; r += gridDim.x) {
         const int ip = r / ns, i = r - (r / ns) * ns;
         double acc[kMaxBatch];
@@ -980,11 +985,14 @@ ax_status ax_convol_case1(const double *bankT, int ns, int ksl, int ksr, const d
     AX_REQUIRE(ns > 0 && ksl > 0 && ksr > 0 && nb >= 0 && ld_d >= (int64_t)ns * ksr &&
                    ld_o >= (int64_t)ns * ksl,
                "ax_convol_case1: bad sizes");
-    for (int b0 = 0; b0 < nb; b0 += kMaxBatch) {
-        const int cnt = nb - b0 < kMaxBatch ? nb - b0 : kMaxBatch;
-        hipLaunchKernelGGL(case1_kernel, dim3(ns * ksl < 8192 ? ns * ksl : 8192), dim3(kT), 0,
-                           ax::as_stream(stream), bankT, ns, ksl, ksr, d + (int64_t)b0 * ld_d, ld_d,
-                           cnt, out + (int64_t)b0 * ld_o, ld_o);
+    // rows on grid.x, groups of kMaxBatch densities on grid.y (65535 groups per launch)
+    constexpr int64_t kPerLaunch = (int64_t)65535 * kMaxBatch;
+    for (int64_t b0 = 0; b0 < nb; b0 += kPerLaunch) {
+        const int cnt = static_cast<int>(nb - b0 < kPerLaunch ? nb - b0 : kPerLaunch);
+        hipLaunchKernelGGL(case1_kernel, dim3(ns * ksl < 8192 ? ns * ksl : 8192,
+                                              (cnt + kMaxBatch - 1) / kMaxBatch),
+                           dim3(kT), 0, ax::as_stream(stream), bankT, ns, ksl, ksr,
+                           d + b0 * ld_d, ld_d, cnt, out + b0 * ld_o, ld_o);
         AX_CHECK_LAUNCH();
     }
     return AX_OK;

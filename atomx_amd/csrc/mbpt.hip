@@ -131,13 +131,17 @@ __global__ __launch_bounds__(256) void pair_sum_kernel(PairSum A) {
             ((wave_sum[0] + wave_sum[1]) + wave_sum[2]) + wave_sum[3];
 }
 
-__global__ void pair_reduce_kernel(const double *__restrict__ partial, int n_pair, int n_tiles,
-                                   double *__restrict__ out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pair) return;
+// One wave per pair: lane l sums tiles l, l + 64, ... in order, then a fixed
+// butterfly combines the lanes (the same order on every run).
+__global__ __launch_bounds__(64) void pair_reduce_kernel(const double *__restrict__ partial,
+                                                         int n_tiles, double *__restrict__ out) {
+    const int p = blockIdx.x;
+    const double *row = partial + (int64_t)p * n_tiles;
     double s = 0.0;
-    for (int t = 0; t < n_tiles; ++t) s += partial[(int64_t)p * n_tiles + t];
-    out[p] = s;
+    for (int t = threadIdx.x; t < n_tiles; t += 64) s += row[t];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x == 0) out[p] = s;
 }
 
 }  // namespace
@@ -194,8 +198,8 @@ ax_status ax_mbpt2_pair_sum(int n_virt, int n_kappa, int n_rank, int n_pair, int
     hipLaunchKernelGGL(pair_sum_kernel, dim3(static_cast<unsigned>(tiles), n_pair), dim3(256), 0,
                        st, A);
     AX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(pair_reduce_kernel, dim3((n_pair + 63) / 64), dim3(64), 0, st, workspace,
-                       n_pair, static_cast<int>(tiles), pair_energy);
+    hipLaunchKernelGGL(pair_reduce_kernel, dim3(n_pair), dim3(64), 0, st, workspace,
+                       static_cast<int>(tiles), pair_energy);
     AX_CHECK_LAUNCH();
     return AX_OK;
 }

@@ -5,11 +5,14 @@ Markdown table.
 
     python -m atomx_amd.native_bench.kernel_stats TRACE [--match TEXT] [--top N]
                                                   [--last N] [--per N] [--between K]
+                                                  [--until K] [--after K]
 
 --last N keeps only the last N dispatches of each kernel (a timed loop that
 follows a tuning sweep of the same kernels); --per N adds a column with the
 total divided by N (time per SCF sweep, per request, ...); --between K keeps
-the dispatches from the first to the last dispatch of kernel K.
+the dispatches from the first to the last dispatch of kernel K; --until K
+keeps those before the first dispatch of K, --after K those from it on
+(phases of one traced run).
 """
 
 from __future__ import annotations
@@ -40,13 +43,24 @@ def dispatches(path):
     return [(int(t0), _short(name), int(d)) for t0, name, d in rows]
 
 
-def kernel_stats(path, match=None, last=0, between=None):
+def _marks(disp, kernel):
+    marks = [t0 for t0, name, _ in disp if name == kernel]
+    if not marks:
+        raise SystemExit(f"no dispatch of {kernel}")
+    return marks
+
+
+def kernel_stats(path, match=None, last=0, between=None, until=None, after=None):
     disp = sorted(dispatches(path))
     if between:  # the window from the first to the last dispatch of that kernel
-        marks = [t0 for t0, name, _ in disp if name == between]
-        if not marks:
-            raise SystemExit(f"no dispatch of {between}")
+        marks = _marks(disp, between)
         disp = [x for x in disp if marks[0] <= x[0] <= marks[-1]]
+    if until:
+        first = _marks(disp, until)[0]
+        disp = [x for x in disp if x[0] < first]
+    if after:
+        first = _marks(disp, after)[0]
+        disp = [x for x in disp if x[0] >= first]
     by_name = {}
     for t0, name, dur in disp:
         if match and match not in name:
@@ -69,8 +83,12 @@ def main():
     ap.add_argument("--per", type=float, default=0.0)
     ap.add_argument("--between", default=None,
                     help="only dispatches from the first to the last one of this kernel")
+    ap.add_argument("--until", default=None,
+                    help="only dispatches before the first one of this kernel")
+    ap.add_argument("--after", default=None,
+                    help="only dispatches from the first one of this kernel on")
     a = ap.parse_args()
-    rows = kernel_stats(a.trace, a.match, a.last, a.between)
+    rows = kernel_stats(a.trace, a.match, a.last, a.between, a.until, a.after)
     total = sum(r[2] for r in rows)
     per = f" | us per 1/{a.per:g}" if a.per else ""
     print(f"| kernel | calls | total us | avg us | share{per} |")
