@@ -69,6 +69,11 @@ SIGNATURES = {
     "ax_rk_tensor_create_pairs": (_int, [_dp, _dp, _int, _int, _int, _int, _ip, _ip, _int,
                                          _int, ctypes.POINTER(_vp)]),
     "ax_rk_tensor_destroy": (_int, [_vp]),
+    "ax_rk_tensor_create_many": (_int, [_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _ip,
+                                        _ip, ctypes.POINTER(_vp), _ip, _ip,
+                                        ctypes.POINTER(_vp)]),
+    "ax_rk_tensor_describe": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int32),
+                                     _ip, ctypes.POINTER(_i64)]),
     "ax_rk_tensor_estimate": (_i64, [_int, _int, _int, _ip, _int, _int]),
     "ax_rk_tensor_size": (_i64, [_vp]),
     "ax_rk_tensor_pairs": (_int, [_vp, _int]),

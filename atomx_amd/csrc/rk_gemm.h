@@ -252,8 +252,10 @@ struct JobTables {
     std::vector<RkJob> tile_fastest, split_fastest, heaviest;
     std::vector<RkTileJobs> tiles;
 };
+// all = false: the tile-fastest list and the slot ranges only (the default
+// order's tables; the others are built when a schedule asks for them).
 JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m, int S,
-                    const KSpace &ks);
+                    const KSpace &ks, bool all = true);
 
 // Balanced jobs: the K range is cut into one slice per XCD (at most 8);
 // each slice's tiles, heaviest first (cost = 16x16 blocks inside the

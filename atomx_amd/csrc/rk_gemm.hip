@@ -442,7 +442,7 @@ KSpace k_space(int64_t n, bool fp64) {
 }
 
 JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg_m, int S,
-                    const KSpace &ks) {
+                    const KSpace &ks, bool all) {
     JobTables t;
     const int nw = static_cast<int>(work.size());
     // every tile in S chunks of equal length (weighting the chunk count by
@@ -456,9 +456,10 @@ JobTables make_jobs(const std::vector<RkWork> &work, const std::vector<int> &seg
         return make_job(work[e], sp, S, e * S + sp, kb, ke);
     };
     t.tile_fastest.reserve((size_t)nw * S);
-    t.split_fastest.reserve((size_t)nw * S);
+    if (all) t.split_fastest.reserve((size_t)nw * S);
     for (int sp = 0; sp < S; ++sp)
         for (int e = 0; e < nw; ++e) t.tile_fastest.push_back(job(e, sp));
+    if (!all) return t;
     for (int e = 0; e < nw; ++e)
         for (int sp = 0; sp < S; ++sp) t.split_fastest.push_back(job(e, sp));
     // heaviest first: tiles by their 16x16 blocks inside the triangle (the

@@ -53,7 +53,9 @@ struct ax_rk_tensor {
     ax::KSpace ks{};
     int64_t partial_slots = 0;  // 64x64 partial tiles the workspace holds
     struct DoneMark *done = nullptr;  // recorded after the plan's last launch set
-    hipEvent_t ready = nullptr; // recorded after the set-up upload
+    // recorded after the set-up upload; shared (reference-counted) by the
+    // plans created by one ax_rk_tensor_create_many call
+    struct DoneMark *ready = nullptr;
     bool ready_pending = true;  // launches must still wait on `ready`
     bool captured = false;      // launched into a stream capture at least once
     std::vector<void *> allocs;
@@ -138,7 +140,8 @@ ax_status ensure_tables(ax_rk_tensor *p, int S) {
         ax::set_error("K split %d outside [1, %d]", S, p->split_cap);
         return AX_ERR_VALUE;
     }
-    if (p->jobs_tf[S - 1] != nullptr) return AX_OK;
+    if (p->jobs_tf[S - 1] && p->jobs_sf[S - 1] && p->jobs_hv[S - 1] && p->jobs_bal[S - 1])
+        return AX_OK;  // a new plan holds its default level's tile-fastest list only
     const ax::JobTables t = ax::make_jobs(p->work_host, p->seg_m_host, S, p->ks);
     const ax::BalancedJobs b =
         ax::make_balanced_jobs(p->work_host, p->seg_m_host, ax::balanced_groups(S), p->ks);
@@ -204,7 +207,7 @@ void free_plan(ax_rk_tensor *p) {
     for (void *q : p->allocs) ax::pool_free(q);
     p->allocs.clear();
     mark_release(p->done);
-    if (p->ready) (void)hipEventDestroy(p->ready);
+    mark_release(p->ready);
     p->done = nullptr;
     p->ready = nullptr;
 }
@@ -218,15 +221,15 @@ ax_status await_ready(ax_rk_tensor *const *plans, int n, hipStream_t s) {
     for (int i = 0; i < n; ++i) {
         ax_rk_tensor *p = plans[i];
         if (!p->ready_pending) continue;
-        if (hipEventQuery(p->ready) == hipSuccess) {
+        if (hipEventQuery(p->ready->ev) == hipSuccess) {
             p->ready_pending = false;
             continue;
         }
         if (cs != hipStreamCaptureStatusNone) {
-            AX_CHECK_HIP(hipEventSynchronize(p->ready));
+            AX_CHECK_HIP(hipEventSynchronize(p->ready->ev));
             p->ready_pending = false;
         } else {
-            AX_CHECK_HIP(hipStreamWaitEvent(s, p->ready, 0));
+            AX_CHECK_HIP(hipStreamWaitEvent(s, p->ready->ev, 0));
         }
     }
     return AX_OK;
@@ -344,8 +347,6 @@ ax_status run_stage_batch(ax_rk_tensor *const *plans, int n, int stage, const do
 
 namespace {
 
-// Builds a plan, or (estimate != nullptr) only computes the device bytes it
-// would hold, without touching the device.
 // Explicit pair list of a pair-product plan (every pair valid at every rank);
 // null for the all-pairs plan with the angular rule.
 struct PairList {
@@ -353,10 +354,115 @@ struct PairList {
     const int32_t *pa, *pb;
 };
 
+// A built plan whose device block still has to receive its set-up image
+// (grid, index lists, segments, job tables) and its r-power tables.
+struct Setup {
+    ax_rk_tensor *plan = nullptr;
+    void *block = nullptr;
+    std::vector<char> image;  // the first image.size() bytes of the block
+    const double *r = nullptr;
+    double *rk = nullptr, *rkm = nullptr;
+};
+
+// r^k and r^-(k+1) tables of up to kMaxBatch plans in one launch (the
+// values of rpow_kernel, numerics/yk.py:113-117).
+struct RpowArg {
+    const double *r;
+    double *rk, *rkm;
+    int64_t n;
+    int kmax, pad;
+};
+__global__ void rpow_batch_kernel(const ax::BatchOf<RpowArg> A) {
+    const int64_t total = A.end[A.n - 1];
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t local;
+        const int p = ax::batch_find(A, t, local);
+        const RpowArg &g = A.a[p];
+        const int k = static_cast<int>(local / g.n);
+        const double rv = g.r[local - (int64_t)k * g.n];
+        g.rk[local] = pow(rv, static_cast<double>(k));
+        g.rkm[local] = pow(rv, static_cast<double>(-(k + 1)));
+    }
+}
+
+// Uploads the set-up images of n built plans: one pinned staging buffer and
+// one copy per plan on the creating thread's set-up stream, the r-power
+// tables in launches of up to kMaxBatch plans, and one `ready` mark that
+// every plan shares.  On failure every plan is freed.
+ax_status upload_plans(Setup *su, int n) {
+    auto fail_all = [&](ax_status st) {
+        for (int i = 0; i < n; ++i) {
+            if (!su[i].plan) continue;
+            free_plan(su[i].plan);
+            delete su[i].plan;
+            su[i].plan = nullptr;
+        }
+        return st;
+    };
+    std::vector<int64_t> off(static_cast<size_t>(n) + 1, 0);
+    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + align256((int64_t)su[i].image.size());
+    void *stage = nullptr;
+    int slot = -1;
+    hipError_t e = ax::staging_acquire(static_cast<size_t>(off[n]), &stage, &slot);
+    if (e != hipSuccess) {
+        ax::set_error("pinned staging of %lld bytes failed: %s", (long long)off[n],
+                      hipGetErrorString(e));
+        return fail_all(AX_ERR_HIP);
+    }
+    char *img = static_cast<char *>(stage);
+    for (int i = 0; i < n; ++i)
+        if (!su[i].image.empty()) std::memcpy(img + off[i], su[i].image.data(), su[i].image.size());
+    hipStream_t ss = ax::setup_stream();
+    for (int i = 0; i < n && e == hipSuccess; ++i)
+        if (!su[i].image.empty())
+            e = hipMemcpyAsync(su[i].block, img + off[i], su[i].image.size(),
+                               hipMemcpyHostToDevice, ss);
+    const hipError_t er = ax::staging_release(slot, ss);
+    if (e == hipSuccess) e = er;
+    for (int i0 = 0; i0 < n && e == hipSuccess; i0 += ax::kMaxBatch) {
+        ax::BatchOf<RpowArg> A{};
+        A.n = std::min(ax::kMaxBatch, n - i0);
+        int64_t total = 0;
+        for (int j = 0; j < A.n; ++j) {
+            const ax_rk_tensor *p = su[i0 + j].plan;
+            A.a[j] = RpowArg{su[i0 + j].r, su[i0 + j].rk, su[i0 + j].rkm, p->n, p->kmax, 0};
+            total += p->n * (p->kmax + 1);
+            A.end[j] = total;
+        }
+        hipLaunchKernelGGL(rpow_batch_kernel, dim3(ax::grid_for(total, 256, 4096)), dim3(256), 0,
+                           ss, A);
+        e = hipGetLastError();
+    }
+    DoneMark *m = e == hipSuccess ? mark_acquire() : nullptr;
+    if (e == hipSuccess && !m) e = hipErrorOutOfMemory;
+    if (e == hipSuccess) e = hipEventRecord(m->ev, ss);
+    if (e != hipSuccess) {
+        if (m) {
+            std::lock_guard<std::mutex> lk(g_mark_mu);
+            g_mark_free.push_back(m);
+        }
+        ax::set_error("R^k tensor set-up failed: %s", hipGetErrorString(e));
+        return fail_all(AX_ERR_HIP);
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_mark_mu);
+        m->refs = n;
+    }
+    for (int i = 0; i < n; ++i) su[i].plan->ready = m;
+    return AX_OK;
+}
+
+// Builds a plan (uploading it, or with setup_out handing it back for a
+// batched upload), or (estimate != nullptr) only computes the device bytes
+// it would hold, without touching the device.
+
 ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng, int n_orb,
                       const int32_t *l_host, int kmax, int precision, ax_rk_tensor **out_plan,
-                      int64_t *estimate, const PairList *explicit_pairs = nullptr) {
-    AX_REQUIRE(out_plan != nullptr || estimate != nullptr, "ax_rk_tensor_create: null plan pointer");
+                      int64_t *estimate, const PairList *explicit_pairs = nullptr,
+                      Setup *setup_out = nullptr) {
+    AX_REQUIRE(out_plan != nullptr || estimate != nullptr || setup_out != nullptr,
+               "ax_rk_tensor_create: null plan pointer");
     AX_REQUIRE((l_host || explicit_pairs) && (estimate || (r_host && w_host)),
                "ax_rk_tensor_create: null grid or l values");
     AX_REQUIRE(nv >= 1 && ng >= 1 && ng <= 16, "ax_rk_tensor_create: bad grid nv=%d ng=%d", nv, ng);
@@ -394,32 +500,47 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     }
     const int n_pairs = static_cast<int>(pa.size());
 
-    // valid pairs per rank, Y^k slots, scan items
-    std::vector<int32_t> rows_all, it_row, it_k, it_out;
+    // valid pairs per rank, Y^k slots, scan items: pair p is valid at the
+    // ranks lo[p] <= k <= hi[p] with k = lo[p] (mod 2) (every rank for an
+    // explicit pair list).  The scan items are the rank-major list of valid
+    // (pair, k): item rows = rows_all, item_out = its position.
+    std::vector<int32_t> plo(n_pairs, 0), phi(n_pairs, kmax);
+    int64_t n_valid = 0;
+    for (int p = 0; p < n_pairs; ++p) {
+        if (!explicit_pairs) {
+            const int la = l_host[pa[p]], lb = l_host[pb[p]];
+            plo[p] = la > lb ? la - lb : lb - la;
+            phi[p] = la + lb;
+            const int top = std::min(phi[p], kmax);
+            n_valid += top >= plo[p] ? (top - plo[p]) / 2 + 1 : 0;
+        } else {
+            n_valid += kmax + 1;
+        }
+    }
+    const int step = explicit_pairs ? 1 : 2;
+    std::vector<int32_t> rows_all(static_cast<size_t>(n_valid) + 1), it_k(n_valid), it_out(n_valid);
     std::vector<int64_t> row_off(kmax + 1), y_off(kmax + 1), out_off(kmax + 1);
     std::vector<int> m_k(kmax + 1, 0);
-    int64_t yrows = 0, outsz = 0;
+    int64_t yrows = 0, outsz = 0, idx = 0;
     for (int k = 0; k <= kmax; ++k) {
-        row_off[k] = static_cast<int64_t>(rows_all.size());
+        row_off[k] = idx;
         y_off[k] = yrows;
-        int m = 0;
+        const int64_t start = idx;
         for (int p = 0; p < n_pairs; ++p) {
-            if (!explicit_pairs) {
-                const int la = l_host[pa[p]], lb = l_host[pb[p]];
-                const int lo = la > lb ? la - lb : lb - la;
-                if (k < lo || k > la + lb || ((la + lb + k) & 1)) continue;
-            }
-            rows_all.push_back(p);
-            it_row.push_back(p);
-            it_k.push_back(k);
-            it_out.push_back(static_cast<int32_t>(yrows + m));
-            ++m;
+            rows_all[idx] = p;  // kept when valid (one spare slot at the end)
+            const int d = k - plo[p];
+            idx += d >= 0 && k <= phi[p] && d % step == 0;
         }
+        const int m = static_cast<int>(idx - start);
+        std::fill(it_k.begin() + start, it_k.begin() + idx, k);
+        for (int64_t i = start; i < idx; ++i) it_out[i] = static_cast<int32_t>(i);
         m_k[k] = m;
         out_off[k] = outsz;
         outsz += (int64_t)m * (m + 1) / 2;
         yrows += m;
     }
+    rows_all.resize(static_cast<size_t>(n_valid));
+    const std::vector<int32_t> &it_row = rows_all;
     AX_REQUIRE(yrows < 0x7fffffff, "ax_rk_tensor_create: too many Y^k rows");
     const int64_t yr = yrows > 0 ? yrows : 1;
 
@@ -438,21 +559,17 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     std::vector<int> seg_m;
     for (int k : seg_k) seg_m.push_back(m_k[k]);
     const ax::KSpace ks = ax::k_space(n, precision == AX_PREC_FP64);
-    // the default level's job tables only (other levels on demand: building
-    // all 16 costs a request-sized plan milliseconds it rarely needs)
-    std::vector<ax::JobTables> jobs;
-    std::vector<ax::BalancedJobs> bal;
-    jobs.push_back(ax::make_jobs(work, seg_m, n_split, ks));
-    bal.push_back(ax::make_balanced_jobs(work, seg_m, ax::balanced_groups(n_split), ks));
-    const int64_t slots = std::max<int64_t>((int64_t)n_work * n_split,
-                                            (int64_t)bal.back().jobs.size());
+    // the default schedule's job list only (tile-fastest at the default K
+    // split); the other orders and levels are built when a schedule or the
+    // tuner asks for them (ensure_tables): a one-shot served plan never does
+    const ax::JobTables jobs0 = ax::make_jobs(work, seg_m, n_split, ks, false);
+    const int64_t slots = (int64_t)n_work * n_split;
     std::vector<int> ident((size_t)n_work * split_cap + 1);
     for (size_t i = 0; i < ident.size(); ++i) ident[i] = static_cast<int>(i);
 
     // one device block for everything the plan owns
     Layout L;
     const int64_t o_r = L.add(n * 8), o_w = L.add(n * 8);
-    const int64_t o_rk = L.add(n * (kmax + 1) * 8), o_rkm = L.add(n * (kmax + 1) * 8);
     const int64_t o_mo = L.add((int64_t)ng * ng * 8), o_mi = L.add((int64_t)ng * ng * 8);
     const int64_t o_pa = L.add(n_pairs * 4), o_pb = L.add(n_pairs * 4);
     const int64_t o_rows = L.add((int64_t)rows_all.size() * 4);
@@ -461,21 +578,13 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     const int64_t o_iout = L.add((int64_t)it_out.size() * 4);
     const int64_t o_segs = L.add((int64_t)seg_k.size() * sizeof(ax::RkSeg));
     const int64_t o_work = L.add((int64_t)n_work * sizeof(ax::RkWork));
-    std::vector<int64_t> o_jtf, o_jsf, o_jhv, o_tj, o_jb, o_gb, o_tjb;
-    for (const ax::JobTables &t : jobs) {
-        o_jtf.push_back(L.add((int64_t)t.tile_fastest.size() * sizeof(ax::RkJob)));
-        o_jsf.push_back(L.add((int64_t)t.split_fastest.size() * sizeof(ax::RkJob)));
-        o_jhv.push_back(L.add((int64_t)t.heaviest.size() * sizeof(ax::RkJob)));
-        o_tj.push_back(L.add((int64_t)t.tiles.size() * sizeof(ax::RkTileJobs)));
-    }
-    for (const ax::BalancedJobs &t : bal) {
-        o_jb.push_back(L.add((int64_t)t.jobs.size() * sizeof(ax::RkJob)));
-        o_gb.push_back(L.add((int64_t)t.groups.size() * sizeof(int)));
-        o_tjb.push_back(L.add((int64_t)t.tiles.size() * sizeof(ax::RkTileJobs)));
-    }
+    const int64_t o_jtf = L.add((int64_t)jobs0.tile_fastest.size() * sizeof(ax::RkJob));
+    const int64_t o_tj = L.add((int64_t)jobs0.tiles.size() * sizeof(ax::RkTileJobs));
     const int64_t o_ident = L.add((int64_t)ident.size() * sizeof(int));
     const int64_t o_ctr = L.add(2 * sizeof(int));  // dynamic-order ticket (zero)
     const int64_t head = L.size;  // everything above is uploaded from the host
+    // r^k and r^-(k+1) tables: formed on the device from the uploaded grid
+    const int64_t o_rk = L.add(n * (kmax + 1) * 8), o_rkm = L.add(n * (kmax + 1) * 8);
     // split precision on 8-node cells: the scan writes Y^k slices only
     const bool y_fp64 = !(precision == AX_PREC_SPLIT && ax::scan_slices_fused(ng, nv));
     const int64_t o_D = L.add((int64_t)n_pairs * n * 8);
@@ -541,19 +650,10 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     char *base = static_cast<char *>(blk);
     auto dptr = [&](int64_t off) { return static_cast<void *>(base + off); };
 
-    // host image of the uploaded head of the block, built in pinned staging
-    void *stage = nullptr;
-    int stage_slot = -1;
-    he = ax::staging_acquire(static_cast<size_t>(head), &stage, &stage_slot);
-    if (he != hipSuccess) {
-        free_plan(plan);
-        delete plan;
-        ax::set_error("pinned staging of %lld bytes failed: %s", (long long)head,
-                      hipGetErrorString(he));
-        return AX_ERR_HIP;
-    }
-    char *img = static_cast<char *>(stage);
-    std::memset(img, 0, static_cast<size_t>(head));
+    // host image of the uploaded head of the block
+    Setup su;
+    su.image.assign(static_cast<size_t>(head), 0);
+    char *img = su.image.data();
     auto put = [&](int64_t off, const void *src, size_t bytes) {
         if (bytes) std::memcpy(img + off, src, bytes);
     };
@@ -563,7 +663,6 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
         const double *mo = nullptr, *mi = nullptr;
         ax_status st = partial_matrices_cached(ng, &mo, &mi);
         if (st != AX_OK) {
-            (void)ax::staging_release(stage_slot, ax::setup_stream());
             free_plan(plan);
             delete plan;
             return st;
@@ -622,29 +721,11 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
         put(o_segs, segs.data(), segs.size() * sizeof(ax::RkSeg));
         put(o_work, work.data(), work.size() * sizeof(ax::RkWork));
         const int d = n_split - 1;  // the default level's tables
-        {
-            const ax::JobTables &t = jobs[0];
-            put(o_jtf[0], t.tile_fastest.data(), t.tile_fastest.size() * sizeof(ax::RkJob));
-            put(o_jsf[0], t.split_fastest.data(), t.split_fastest.size() * sizeof(ax::RkJob));
-            put(o_jhv[0], t.heaviest.data(), t.heaviest.size() * sizeof(ax::RkJob));
-            put(o_tj[0], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
-            plan->jobs_tf[d] = static_cast<const ax::RkJob *>(dptr(o_jtf[0]));
-            plan->jobs_sf[d] = static_cast<const ax::RkJob *>(dptr(o_jsf[0]));
-            plan->jobs_hv[d] = static_cast<const ax::RkJob *>(dptr(o_jhv[0]));
-            plan->tile_jobs[d] = static_cast<const ax::RkTileJobs *>(dptr(o_tj[0]));
-            plan->n_jobs[d] = static_cast<int>(t.tile_fastest.size());
-        }
-        {
-            const ax::BalancedJobs &t = bal[0];
-            put(o_jb[0], t.jobs.data(), t.jobs.size() * sizeof(ax::RkJob));
-            put(o_gb[0], t.groups.data(), t.groups.size() * sizeof(int));
-            put(o_tjb[0], t.tiles.data(), t.tiles.size() * sizeof(ax::RkTileJobs));
-            plan->jobs_bal[d] = static_cast<const ax::RkJob *>(dptr(o_jb[0]));
-            plan->groups_bal[d] = static_cast<const int *>(dptr(o_gb[0]));
-            plan->tile_jobs_bal[d] = static_cast<const ax::RkTileJobs *>(dptr(o_tjb[0]));
-            plan->n_groups_bal[d] = static_cast<int>(t.groups.size()) - 1;
-            plan->reduce_bal[d] = t.jobs.size() > t.tiles.size() ? 1 : 0;
-        }
+        put(o_jtf, jobs0.tile_fastest.data(), jobs0.tile_fastest.size() * sizeof(ax::RkJob));
+        put(o_tj, jobs0.tiles.data(), jobs0.tiles.size() * sizeof(ax::RkTileJobs));
+        plan->jobs_tf[d] = static_cast<const ax::RkJob *>(dptr(o_jtf));
+        plan->tile_jobs[d] = static_cast<const ax::RkTileJobs *>(dptr(o_tj));
+        plan->n_jobs[d] = static_cast<int>(jobs0.tile_fastest.size());
         put(o_ident, ident.data(), ident.size() * sizeof(int));
         plan->identity = static_cast<const int *>(dptr(o_ident));
     }
@@ -685,27 +766,18 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
     h.counter = static_cast<int *>(dptr(o_ctr));
     apply_schedule(plan);
 
-    // one asynchronous upload and the r-power tables on the set-up stream;
-    // launches wait on the `ready` event, the host does not
-    hipStream_t ss = ax::setup_stream();
-    ax_status st = AX_OK;
-    hipError_t e = hipMemcpyAsync(blk, img, static_cast<size_t>(head), hipMemcpyHostToDevice, ss);
-    const hipError_t er = ax::staging_release(stage_slot, ss);
-    if (e == hipSuccess) e = er;
-    if (e == hipSuccess)
-        st = ax_grid_rpow(static_cast<const double *>(dptr(o_r)), n, kmax,
-                          static_cast<double *>(dptr(o_rk)), static_cast<double *>(dptr(o_rkm)),
-                          ss);
-    if (e == hipSuccess && st == AX_OK) e = hipEventCreateWithFlags(&plan->ready, hipEventDisableTiming);
-    if (e == hipSuccess && st == AX_OK) e = hipEventRecord(plan->ready, ss);
-    if (e != hipSuccess || st != AX_OK) {
-        if (e != hipSuccess) ax::set_error("R^k tensor set-up failed: %s", hipGetErrorString(e));
-        free_plan(plan);
-        delete plan;
-        return st != AX_OK ? st : AX_ERR_HIP;
+    su.plan = plan;
+    su.block = blk;
+    su.r = static_cast<const double *>(dptr(o_r));
+    su.rk = static_cast<double *>(dptr(o_rk));
+    su.rkm = static_cast<double *>(dptr(o_rkm));
+    if (setup_out) {
+        *setup_out = std::move(su);
+        return AX_OK;
     }
-    *out_plan = plan;
-    return AX_OK;
+    const ax_status st = upload_plans(&su, 1);
+    if (st == AX_OK) *out_plan = plan;
+    return st;
 }
 
 }  // namespace
@@ -717,6 +789,46 @@ ax_status ax_rk_tensor_create(const double *r_host, const double *w_host, int nv
                               ax_rk_tensor **out_plan) {
     AX_REQUIRE(out_plan != nullptr, "ax_rk_tensor_create: null plan pointer");
     return plan_create(r_host, w_host, nv, ng, n_orb, l_host, kmax, precision, out_plan, nullptr);
+}
+
+ax_status ax_rk_tensor_create_many(int n_plans, const double *const *r_host,
+                                   const double *const *w_host, const int32_t *nv,
+                                   const int32_t *ng, const int32_t *n_orb,
+                                   const int32_t *const *l_host, const int32_t *kmax,
+                                   const int32_t *precision, ax_rk_tensor **out_plans) {
+    AX_REQUIRE(n_plans >= 0 && (n_plans == 0 || (r_host && w_host && nv && ng && n_orb && l_host &&
+                                                   kmax && precision && out_plans)),
+               "ax_rk_tensor_create_many: null argument");
+    std::vector<Setup> su(static_cast<size_t>(n_plans));
+    for (int i = 0; i < n_plans; ++i) out_plans[i] = nullptr;
+    for (int i = 0; i < n_plans; ++i) {
+        const ax_status st = plan_create(r_host[i], w_host[i], nv[i], ng[i], n_orb[i], l_host[i],
+                                         kmax[i], precision[i], nullptr, nullptr, nullptr, &su[i]);
+        if (st != AX_OK) {
+            for (int j = 0; j < i; ++j) {
+                free_plan(su[j].plan);
+                delete su[j].plan;
+            }
+            return st;
+        }
+    }
+    if (n_plans == 0) return AX_OK;
+    const ax_status st = upload_plans(su.data(), n_plans);
+    if (st != AX_OK) return st;
+    for (int i = 0; i < n_plans; ++i) out_plans[i] = su[i].plan;
+    return AX_OK;
+}
+
+ax_status ax_rk_tensor_describe(const ax_rk_tensor *plan, int64_t *size, int32_t *slices,
+                                int32_t *pairs, int64_t *offsets) {
+    AX_REQUIRE(plan && size && slices && pairs && offsets, "ax_rk_tensor_describe: null argument");
+    *size = plan->out_size;
+    *slices = plan->slices;
+    for (int k = 0; k <= plan->kmax; ++k) {
+        pairs[k] = plan->m_k[k];
+        offsets[k] = plan->out_off[k];
+    }
+    return AX_OK;
 }
 
 ax_status ax_rk_tensor_create_pairs(const double *r_host, const double *w_host, int nv, int ng,
@@ -742,7 +854,7 @@ ax_status ax_rk_tensor_destroy(ax_rk_tensor *plan) {
     if (!plan) return AX_OK;
     // the blocks return to the cache: wait until neither the set-up upload
     // nor a launch can still use them
-    if (plan->ready) (void)hipEventSynchronize(plan->ready);
+    if (plan->ready) (void)hipEventSynchronize(plan->ready->ev);
     if (plan->captured)
         (void)hipDeviceSynchronize();
     else if (plan->done)

@@ -879,8 +879,13 @@ __global__ __launch_bounds__(kThreads) void sum_partials_kernel(const double *__
 // in-wave Hillis-Steele scans and the in-cell partial integrals, and the
 // two cumulative integrals back to node-per-lane for the output (or, with
 // SLICE, for the int8 slices: for a fixed r the 64 lanes hold one K-tile).
-// The per-pass wave totals are combined through LDS in a fixed order after
-// one barrier.  Same discretisation as scan_item (numerics/yk.py:87-154).
+// 8-node cells skip the transposes on the way in (each lane reads its cell's
+// 64 contiguous bytes); their FP64 output goes through a per-wave LDS stage
+// so that every store is 1 KB contiguous (cell-per-lane stores, 16 bytes
+// per lane at a 64-byte stride, measured 1.6 us slower on 1389 rows of 3200
+// nodes).  The per-pass wave totals are combined through LDS in a fixed
+// order after one barrier.  Same discretisation as scan_item
+// (numerics/yk.py:87-154).
 // ---------------------------------------------------------------------------
 constexpr int kWgScanThreads = 256;
 constexpr int kWgScanWaves = kWgScanThreads / 64;
@@ -896,6 +901,10 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
     // cells are read and written (or sliced) cell-per-lane
     constexpr int TW = NG != 8 ? kWgScanWaves : 1;
     __shared__ double tr[TW][2][64 * CS];
+    // FP64 output of 8-node cells: a per-wave stage (cell stride padded to
+    // 10 doubles) that turns the cell-per-lane results into contiguous stores
+    constexpr int YS = NG == 8 ? 10 : 1;
+    __shared__ __attribute__((aligned(16))) double ystage[NG == 8 ? kWgScanWaves : 1][64 * YS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int64_t local;
     const int p = ax::batch_find(A, blockIdx.x, local);
@@ -907,6 +916,7 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
     const double *src = g.D + (int64_t)g.item_row[local] * n;
     const double *rkp = g.rk + (int64_t)k * n;
     const double *rkmp = g.rkm + (int64_t)k * n;
+    double *ybuf = ystage[NG == 8 ? wave : 0];
     double *to = tr[TW > 1 ? wave : 0][0];
     double *ti = tr[TW > 1 ? wave : 0][1];
 
@@ -1045,10 +1055,23 @@ __global__ __launch_bounds__(kWgScanThreads) void yk_scan_wg_kernel(
                     const double y = rb * (bo + po) + ra * (bi + pi);
                     if (m & 1) y2[m >> 1].y = y; else y2[m >> 1].x = y;
                 }
-                double *yo = g.Y + (int64_t)g.item_out[local] * n + c0;
+                // through the wave's LDS stage: written cell-per-lane, read
+                // back as node pairs so that each store is 1 KB contiguous
 #pragma unroll
-                for (int h = 0; h < NG / 2; ++h) *reinterpret_cast<dd2 *>(yo + 2 * h) = y2[h];
+                for (int h = 0; h < NG / 2; ++h)
+                    *reinterpret_cast<dd2 *>(ybuf + lane * YS + 2 * h) = y2[h];
             }
+            __builtin_amdgcn_wave_barrier();
+            {
+                double *yo = g.Y + (int64_t)g.item_out[local] * n + base;
+#pragma unroll
+                for (int i = 0; i < NG / 2; ++i) {
+                    const int xl = 128 * i + 2 * lane;  // node pair of the pass's 512 nodes
+                    const dd2 v = *reinterpret_cast<const dd2 *>(ybuf + (xl / NG) * YS + xl % NG);
+                    if (base + xl + 1 < n) *reinterpret_cast<dd2 *>(yo + xl) = v;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
             continue;
         }
         if constexpr (SLICE && NG == 8) {

@@ -652,10 +652,15 @@ class GpuExecutor:
     def _pack(self, chunks, dst):
         """Concatenates the 1-D arrays ``chunks`` into ``dst``; large inputs
         are cut into equal ranges of the concatenation, one per thread."""
+        self._pack_wait(self._pack_start(chunks, dst))
+
+    def _pack_start(self, chunks, dst):
+        """Starts ``_pack``: small inputs are copied at once, large ones by
+        the worker threads; returns what ``_pack_wait`` waits for."""
         n = dst.size
         if self._pool is None or n < self._PACK_SERIAL:
             np.concatenate(chunks, out=dst)
-            return
+            return []
         starts = np.concatenate([[0], np.cumsum([c.size for c in chunks])])
         if int(starts[-1]) != n:
             raise ValueError("packed inputs do not fill the staging range")
@@ -671,8 +676,10 @@ class GpuExecutor:
                 lo += take
                 c += 1
 
-        futs = [self._pool.submit(copy_range, bounds[w], bounds[w + 1]) for w in range(parts)
+        return [self._pool.submit(copy_range, bounds[w], bounds[w + 1]) for w in range(parts)
                 if bounds[w] < bounds[w + 1]]
+
+    def _pack_wait(self, futs):
         for f in futs:
             f.result()
 
@@ -805,13 +812,11 @@ class GpuExecutor:
             if g is not None:
                 self._launch_group(atoms, g, job, stream)
                 return
-        entries = []  # (request index, atom index or -1, plan, P, Q)
+        entries = []  # [request index, atom index or -1, atom, state, P, Q]
         f64 = np.float64
-        tune = []  # new resident plans to tune on this launch set's inputs
         # the states of every request first (a request whose state cannot be
-        # had fails alone), then the plans, checks and staging request by
-        # request.  (Creating a set's plans on several threads measured
-        # slower: plan set-up is bound by HIP calls that serialise.)
+        # had fails alone), then the shape checks request by request against
+        # the request itself (plans may not exist yet)
         claimed = []
         for req in reqs:
             atoms = req.atoms if req.kind == "slater_batch" else (req,)
@@ -824,72 +829,68 @@ class GpuExecutor:
             claimed.append((atoms, sts, err))
         for i, (req, (atoms, sts, err)) in enumerate(zip(reqs, claimed)):
             batch = req.kind == "slater_batch"
-            got, fresh = [], []
+            got = []
             try:
                 if err is not None:
                     raise err
                 for j, (a, st) in enumerate(zip(atoms, sts)):
-                    plan = st.plan
-                    if plan is None:
-                        plan = st.plan = SlaterTensor(a.gr, a.grw, a.ls, a.kmax, a.precision)
-                        self.plans_created += 1
-                        # a resident state of a large tensor is tuned on its first request
-                        if st.key is not None and plan.size >= self._TUNE_MIN_RK:
-                            fresh.append((plan, st))
                     P, Q = a.P, a.Q
                     if type(P) is not np.ndarray or P.dtype != f64:
                         P = np.asarray(P, dtype=f64)
                     if Q is not None and (type(Q) is not np.ndarray or Q.dtype != f64):
                         Q = np.asarray(Q, dtype=f64)
-                    shape = (plan.n_orb, plan.n)
+                    shape = (len(a.ls), int(np.size(a.gr)))
                     if P.shape != shape or (Q is not None and Q.shape != shape):
                         raise ValueError(f"orbital samples must be {shape}")
-                    got.append((i, j if batch else -1, plan, P, Q))
+                    got.append([i, j if batch else -1, a, st, P, Q])
                 entries.extend(got)
-                # only plans whose request made it into this launch set are tuned
-                tune.extend(fresh)
             except Exception as exc:  # the request fails alone
-                job["errors"][i] = exc
-                # every state the scheduler reserved for the request is
-                # released with it, also those of atoms after the failing one
-                seen = {id(st) for st in sts}
-                for a in atoms:
-                    k = self._key(a)
-                    st = self.states.get(k) if k is not None else None
-                    if st is not None and id(st) not in seen:
-                        sts.append(st)
-                        seen.add(id(st))
+                self._fail_request(job, i, exc, atoms, sts)
             job["states"].append(sts)
         if not entries:
             return
-        chunks = []
+        # staging layout from the requests (a plan's size is its R^k count),
+        # the inputs packed on the worker threads while the new plans are made
+        chunks, in_off = [], []
+        off = 0
         for e in entries:
-            chunks.append(e[3].reshape(-1))
-            if e[4] is not None:
-                chunks.append(e[4].reshape(-1))
-        n_in = sum(c.size for c in chunks)
-        n_out = sum(e[2].size for e in entries)
-        host, dev = self._staging(job["sid"], self._host_need(n_in, n_in + n_out), n_in + n_out)
+            in_off.append(off)
+            chunks.append(e[4].reshape(-1))
+            off += e[4].size
+            if e[5] is not None:
+                chunks.append(e[5].reshape(-1))
+                off += e[5].size
+        n_in = off
+        out_off, out = [], n_in
+        for e in entries:
+            out_off.append(out)
+            out += e[3].plan.size if e[3].plan is not None else self._info(e[2])[2]
+        host, dev = self._staging(job["sid"], self._host_need(n_in, out), out)
         hv = host.numpy()
-        self._pack(chunks, hv[:n_in])
+        packing = self._pack_start(chunks, hv[:n_in])
+        tune, failed = self._make_plans(job, entries)
+        self._pack_wait(packing)
         base = dev.data_ptr()
         pp, qq, oo, plans = [], [], [], []
         results = {}
-        off, out = 0, n_in
-        for i, j, plan, P, Q in entries:
-            pp.append(base + 8 * off)
-            off += P.size
-            if Q is not None:
-                qq.append(base + 8 * off)
-                off += Q.size
-            else:
-                qq.append(None)
-            oo.append(base + 8 * out)
-            results[(i, j)] = (out, plan.size)
-            out += plan.size
+        for e, i0, o0 in zip(entries, in_off, out_off):
+            i, j, a, st = e[0], e[1], e[2], e[3]
+            if i in failed:
+                continue
+            plan = st.plan
+            if o0 + plan.size > out:
+                raise RuntimeError("R^k plan size differs from its request's count")
+            pp.append(base + 8 * i0)
+            qq.append(base + 8 * (i0 + e[4].size) if e[5] is not None else None)
+            oo.append(base + 8 * o0)
+            results[(i, j)] = (o0, plan.size)
             plans.append(plan)
+        if not plans:
+            return
         dev[:n_in].copy_(host[:n_in], non_blocking=True)
         for t, st in tune:  # times every contraction schedule on the staged inputs
+            if t not in plans:
+                continue
             k = plans.index(t)
             try:
                 t.tune_raw(pp[k], qq[k], oo[k], stream=stream)
@@ -902,6 +903,58 @@ class GpuExecutor:
         job["results"] = results
         job["n_in"] = n_in
         self._results_out(job, host, dev, n_in, out)
+
+    def _fail_request(self, job, i, exc, atoms, sts):
+        """Request i of the launch set fails alone: every state the scheduler
+        reserved for it is released with it, also those of atoms after the
+        failing one."""
+        job["errors"][i] = exc
+        seen = {id(st) for st in sts}
+        for a in atoms:
+            k = self._key(a)
+            st = self.states.get(k) if k is not None else None
+            if st is not None and id(st) not in seen:
+                sts.append(st)
+                seen.add(id(st))
+
+    def _make_plans(self, job, entries):
+        """Plans for the entries whose state has none: all of one precision in
+        one library call (one staging upload, one shared set-up event); when
+        that fails, one by one, so that a bad request fails alone.  Returns
+        (new resident plans to tune, indices of the requests that failed)."""
+        from .tensor import SlaterTensor
+        tune, failed = [], set()
+        need = {}
+        for e in entries:
+            if e[3].plan is None:
+                need.setdefault(e[2].precision, []).append(e)
+        for prec, es in need.items():
+            try:
+                made = SlaterTensor.create_many([(e[2].gr, e[2].grw, e[2].ls, e[2].kmax)
+                                                 for e in es], prec)
+            except (RuntimeError, ValueError, MemoryError):
+                made = None
+            for k, e in enumerate(es):
+                i, st = e[0], e[3]
+                if i in failed:
+                    if made is not None:
+                        made[k].close()
+                    continue
+                try:
+                    plan = made[k] if made is not None else SlaterTensor(
+                        e[2].gr, e[2].grw, e[2].ls, e[2].kmax, prec)
+                except Exception as exc:
+                    failed.add(i)
+                    req = job["reqs"][i]
+                    atoms = req.atoms if req.kind == "slater_batch" else (req,)
+                    self._fail_request(job, i, exc, atoms, [])
+                    continue
+                st.plan = plan
+                self.plans_created += 1
+                # a resident state of a large tensor is tuned on its first request
+                if st.key is not None and plan.size >= self._TUNE_MIN_RK:
+                    tune.append((plan, st))
+        return tune, failed
 
     def _launch_fock(self, req, job, stream):
         from .fock_yk import FockYk

@@ -57,12 +57,68 @@ class SlaterTensor:
                  grw.ravel().ctypes.data_as(dp), self.nv, self.ng, self.n_orb, int(pa.size),
                  pa.ctypes.data_as(ip), pb.ctypes.data_as(ip), self.kmax, PRECISIONS[precision],
                  ctypes.byref(h))
+        self._attach(lib, h)
+
+    def _attach(self, lib, h):
         self._h = h
         self._lib = lib
-        self.size = int(lib.ax_rk_tensor_size(h))
-        self.m = [int(lib.ax_rk_tensor_pairs(h, k)) for k in range(self.kmax + 1)]
-        self.offsets = [int(lib.ax_rk_tensor_offset(h, k)) for k in range(self.kmax + 1)]
-        self.slices = int(lib.ax_rk_tensor_slices(h))
+        size, slices = ctypes.c_int64(), ctypes.c_int32()
+        pairs = np.empty(self.kmax + 1, dtype=np.int32)
+        offs = np.empty(self.kmax + 1, dtype=np.int64)
+        call("ax_rk_tensor_describe", h, ctypes.byref(size), ctypes.byref(slices),
+             pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+             offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        self.size = int(size.value)
+        self.m = pairs.tolist()
+        self.offsets = offs.tolist()
+        self.slices = int(slices.value)
+
+    @classmethod
+    def create_many(cls, specs, precision="fp64"):
+        """All-pairs plans for ``specs`` = [(gr, grw, ls, kmax), ...] in one
+        library call (one staging upload, batched r-power tables, one shared
+        set-up event); each plan is then used and closed on its own."""
+        dv.require_gpu()
+        lib = load()
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {tuple(PRECISIONS)}")
+        n = len(specs)
+        if n == 0:
+            return []
+        objs, keep = [], []
+        r_arr = (ctypes.c_void_p * n)()
+        w_arr = (ctypes.c_void_p * n)()
+        l_arr = (ctypes.c_void_p * n)()
+        shape = np.empty((4, n), dtype=np.int32)  # nv, ng, n_orb, kmax
+        for i, (gr, grw, ls, kmax) in enumerate(specs):
+            gr = np.ascontiguousarray(gr, dtype=np.float64)
+            grw = np.ascontiguousarray(grw, dtype=np.float64)
+            if gr.ndim != 2 or gr.shape != grw.shape:
+                raise ValueError("grid and weights must both be (nv, n_gauss)")
+            o = cls.__new__(cls)
+            o.nv, o.ng = gr.shape
+            o.n = o.nv * o.ng
+            o.kmax = int(kmax)
+            o.precision = precision
+            o._grw = grw.ravel().copy()
+            o.ls = np.ascontiguousarray(ls, dtype=np.int32)
+            o.n_orb = int(o.ls.size)
+            o._pairs = None
+            keep += [gr, grw]
+            r_arr[i] = gr.ctypes.data
+            w_arr[i] = grw.ctypes.data
+            l_arr[i] = o.ls.ctypes.data
+            shape[:, i] = (o.nv, o.ng, o.n_orb, o.kmax)
+            objs.append(o)
+        prec = np.full(n, PRECISIONS[precision], dtype=np.int32)
+        hs = (ctypes.c_void_p * n)()
+        ip = ctypes.POINTER(ctypes.c_int32)
+        call("ax_rk_tensor_create_many", n, r_arr, w_arr, shape[0].ctypes.data_as(ip),
+             shape[1].ctypes.data_as(ip), shape[2].ctypes.data_as(ip), l_arr,
+             shape[3].ctypes.data_as(ip), prec.ctypes.data_as(ip), hs)
+        for o, h in zip(objs, hs):
+            o._attach(lib, ctypes.c_void_p(h))
+        return objs
 
     @property
     def footprint(self):

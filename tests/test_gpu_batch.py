@@ -365,3 +365,49 @@ def test_slice_count_follows_the_grid():
     pl = SlaterTensor(gr, grw, None, 6, precision="split", pairs=(pa, pb), n_fn=B.shape[0])
     assert pl.slices == 8
     pl.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_create_many_equals_single_plans(problems, precision):
+    """Plans set up by one ax_rk_tensor_create_many call (one staging upload,
+    batched r-power tables, one shared set-up event) describe and compute
+    bitwise what plans created one by one do, run alone or batched, and are
+    closed one by one in any order (the shared event outlives the first)."""
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor, run_batch
+    specs = [(gr, grw, ls, kmax) for gr, grw, P, Q, ls, kmax in problems]
+    many = SlaterTensor.create_many(specs, precision)
+    single = [SlaterTensor(gr, grw, ls, kmax, precision) for gr, grw, ls, kmax in specs]
+    for a, b in zip(many, single):
+        assert (a.size, a.m, a.offsets, a.slices, a.n_orb, a.n) == \
+            (b.size, b.m, b.offsets, b.slices, b.n_orb, b.n)
+    ins = [(P, Q) for _, _, P, Q, _, _ in problems]
+    ref = [dv.host(p.run(P, Q)) for p, (P, Q) in zip(single, ins)]
+    got = [dv.host(p.run(P, Q)) for p, (P, Q) in zip(many, ins)]
+    for r, g in zip(ref, got):
+        assert np.array_equal(r, g)
+    outs = run_batch(many, [P for P, _ in ins], [Q for _, Q in ins])
+    for r, o in zip(ref, outs):
+        assert np.array_equal(r, dv.host(o))
+    for k in (3, 0, 4, 1, 2):  # the shared set-up event is released by the last
+        many[k].close()
+    for p in single:
+        p.close()
+
+
+def test_create_many_failure(problems):
+    """A bad specification fails the whole call with the library's error (the
+    library frees the plans it built before it); a later call on good
+    specifications works."""
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    gr, grw, P, Q, ls, kmax = problems[0]
+    bad = np.array(ls, dtype=np.int32).copy()
+    bad[0] = -1
+    with pytest.raises((RuntimeError, ValueError), match="negative l"):
+        SlaterTensor.create_many([(gr, grw, ls, kmax), (gr, grw, bad, kmax)], "fp64")
+    ok = SlaterTensor.create_many([(gr, grw, ls, kmax)], "fp64")
+    single = SlaterTensor(gr, grw, ls, kmax, "fp64")
+    assert np.array_equal(dv.host(ok[0].run(P, Q)), dv.host(single.run(P, Q)))
+    ok[0].close()
+    single.close()
