@@ -34,7 +34,6 @@ class SlaterTensor:
         self.n = self.nv * self.ng
         self.kmax = int(kmax)
         self.precision = precision
-        self._grw = grw.ravel().copy()
         h = ctypes.c_void_p()
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int32)
@@ -100,7 +99,6 @@ class SlaterTensor:
             o.n = o.nv * o.ng
             o.kmax = int(kmax)
             o.precision = precision
-            o._grw = grw.ravel().copy()
             o.ls = np.ascontiguousarray(ls, dtype=np.int32)
             o.n_orb = int(o.ls.size)
             o._pairs = None
