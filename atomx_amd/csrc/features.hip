@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ax_common.h"
 
 namespace {
@@ -288,6 +290,98 @@ __global__ void cpc_gemm_kernel(int64_t K, int chunk, const T *__restrict__ L, i
     }
 }
 
+// The same product for real operands on the FP64 MFMA: one 64x64 output tile
+// and one K chunk per work-group.  K is staged through LDS 16 rows at a time
+// (row li[k] of L scaled by coef[k], row ri[k] of R, columns of the tile,
+// zero outside the matrix or past the chunk; rows padded by 8 doubles), and
+// each wave accumulates a 32x32 quadrant as 2x2 blocks of
+// v_mfma_f64_16x16x4_f64: lane (f, g) feeds L^T row f and R column f at K
+// row g of each group of 4 and holds output rows g + 4 r, column f.
+constexpr int kCpcTile = 64;
+constexpr int kCpcK = 16;
+constexpr int kCpcPad = kCpcTile + 8;
+
+typedef double cpc_d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void cpc_gemm_mfma_kernel(
+    int64_t K, int chunk, const double *__restrict__ L, int64_t ldl, int m,
+    const int *__restrict__ li, const double *__restrict__ R, int64_t ldr, int n,
+    const int *__restrict__ ri, const double *__restrict__ coef, double *__restrict__ part) {
+    __shared__ double Ls[kCpcK][kCpcPad], Rs[kCpcK][kCpcPad];
+    const int tiles_n = (n + kCpcTile - 1) / kCpcTile;
+    const int m0 = (int)(blockIdx.x / tiles_n) * kCpcTile;
+    const int n0 = (int)(blockIdx.x % tiles_n) * kCpcTile;
+    const int split = blockIdx.y;
+    const int64_t k0 = (int64_t)split * chunk;
+    const int64_t k1 = min(K, k0 + chunk);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int fr = lane & 15, fg = lane >> 4;
+    cpc_d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = cpc_d4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t kb = k0; kb < k1; kb += kCpcK) {
+#pragma unroll
+        for (int q = 0; q < kCpcK * kCpcTile / 256; ++q) {
+            const int e = tid + 256 * q, kk = e / kCpcTile, c = e % kCpcTile;
+            const int64_t k = kb + kk;
+            double lv = 0.0, rv = 0.0;
+            if (k < k1) {
+                const int64_t ra = li ? (int64_t)li[k] : k, rb = ri ? (int64_t)ri[k] : k;
+                if (m0 + c < m) lv = L[ra * ldl + m0 + c] * (coef ? coef[k] : 1.0);
+                if (n0 + c < n) rv = R[rb * ldr + n0 + c];
+            }
+            Ls[kk][c] = lv;
+            Rs[kk][c] = rv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < kCpcK / 4; ++s) {
+            const int kk = 4 * s + fg;
+            double av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = Ls[kk][wr + 16 * a + fr];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = Rs[kk][wc + 16 * b + fr];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0,
+                                                                     0);
+        }
+        __syncthreads();
+    }
+    double *dst = part + (int64_t)split * m * n;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = m0 + wr + 16 * a + fg + 4 * r, j = n0 + wc + 16 * b + fr;
+                if (i < m && j < n) dst[(int64_t)i * n + j] = acc[a][b][r];
+            }
+}
+
+// cpc_gemm_kernel for complex operands, the MFMA form for real ones.
+template <typename T>
+void launch_cpc_gemm(int64_t K, int chunk, int n_split, const T *L, int64_t ldl, int m,
+                     const int *li, const T *R, int64_t ldr, int n, const int *ri, const T *coef,
+                     T *part, hipStream_t st) {
+    if constexpr (std::is_same<T, double>::value) {
+        const int tiles = ((m + kCpcTile - 1) / kCpcTile) * ((n + kCpcTile - 1) / kCpcTile);
+        hipLaunchKernelGGL(cpc_gemm_mfma_kernel, dim3(tiles, n_split), dim3(256), 0, st, K, chunk,
+                           L, ldl, m, li, R, ldr, n, ri, coef, part);
+    } else {
+        const dim3 gmn(ax::grid_for((int64_t)m * n, 256, 64));
+        hipLaunchKernelGGL(cpc_gemm_kernel<T>, dim3(gmn.x, n_split), dim3(256), 0, st, K,
+                           chunk, L, ldl, m, li, R, ldr, n, ri, coef, part);
+    }
+}
+
 // out[t] (+)= sum over splits in order.
 template <typename T>
 __global__ void cpc_reduce_kernel(const T *__restrict__ part, int n_split, int64_t mn,
@@ -490,14 +584,12 @@ int cpc_apply(const void *rho, int nr, int ns_, const void *A, int64_t lda, int 
                        static_cast<const T *>(pfx), nv, n, Z + zn);
     AX_CHECK_LAUNCH();
     const int64_t mn = (int64_t)m * n;
-    const dim3 gmn(ax::grid_for(mn, kThreads, 64));
     // cross-cell part: [L_rise; L_fall] (2 nv nr, m) against [Z_sfx; Z_pfx]
     {
         const int64_t K = 2 * zn / n;
         const int chunk = (int)((K + n_split - 1) / n_split);
-        hipLaunchKernelGGL(cpc_gemm_kernel<T>, dim3(gmn.x, n_split), dim3(kThreads), 0, st, K,
-                           chunk, static_cast<const T *>(left_stack), (int64_t)m, m, nullptr, Z,
-                           (int64_t)n, n, nullptr, nullptr, part);
+        launch_cpc_gemm<T>(K, chunk, n_split, static_cast<const T *>(left_stack), (int64_t)m, m,
+                           nullptr, Z, (int64_t)n, n, nullptr, nullptr, part, st);
         hipLaunchKernelGGL(cpc_reduce_kernel<T>, dim3(ax::grid_for(mn, kThreads)),
                            dim3(kThreads), 0, st, part, n_split, mn, 0, out_);
         AX_CHECK_LAUNCH();
@@ -510,9 +602,8 @@ int cpc_apply(const void *rho, int nr, int ns_, const void *A, int64_t lda, int 
                            dim3(kThreads), 0, st, BP, ns_, static_cast<const T *>(C), ldc, src,
                            dst, w, n_edges, coef);
         const int chunk = (int)((n_edges + n_split - 1) / n_split);
-        hipLaunchKernelGGL(cpc_gemm_kernel<T>, dim3(gmn.x, n_split), dim3(kThreads), 0, st,
-                           n_edges, chunk, static_cast<const T *>(A), lda, m, src,
-                           static_cast<const T *>(D), ldd, n, dst, coef, part);
+        launch_cpc_gemm<T>(n_edges, chunk, n_split, static_cast<const T *>(A), lda, m, src,
+                           static_cast<const T *>(D), ldd, n, dst, coef, part, st);
         hipLaunchKernelGGL(cpc_reduce_kernel<T>, dim3(ax::grid_for(mn, kThreads)),
                            dim3(kThreads), 0, st, part, n_split, mn, 1, out_);
         AX_CHECK_LAUNCH();
