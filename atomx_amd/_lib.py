@@ -72,6 +72,16 @@ SIGNATURES = {
     "ax_rk_tensor_create_many": (_int, [_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _ip,
                                         _ip, ctypes.POINTER(_vp), _ip, _ip,
                                         ctypes.POINTER(_vp)]),
+    "ax_server_create": (_int, [_i64, _int, _int, ctypes.POINTER(_vp)]),
+    "ax_server_destroy": (_int, [_vp]),
+    "ax_server_submit": (_int, [_vp, _vp, _vp, _int, _int, _int, _vp, _int, _int, _vp, _vp,
+                                ctypes.POINTER(_i64)]),
+    "ax_server_poll": (_int, [_vp, _i64, ctypes.POINTER(ctypes.c_int32)]),
+    "ax_server_wait": (_int, [_vp, _i64]),
+    "ax_server_result": (_int, [_vp, _i64, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
+                                ctypes.POINTER(ctypes.c_int32)]),
+    "ax_server_release": (_int, [_vp, _i64]),
+    "ax_server_stats": (_int, [_vp, ctypes.POINTER(_i64), _int]),
     "ax_rk_tensor_describe": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int32),
                                      _ip, ctypes.POINTER(_i64)]),
     "ax_rk_tensor_estimate": (_i64, [_int, _int, _int, _ip, _int, _int]),

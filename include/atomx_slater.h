@@ -200,6 +200,48 @@ ax_status ax_rk_tensor_create_many(int n_plans, const double *const *r_host,
  * ax_rk_tensor_size / _slices / _pairs / _offset). */
 ax_status ax_rk_tensor_describe(const ax_rk_tensor *plan, int64_t *size, int32_t *slices,
                                 int32_t *pairs, int64_t *offsets);
+/* ---- native serving loop (continuous batching of R^k requests) -------- */
+
+/* A server owns one scheduler thread on the calling thread's device.  It
+ * admits submitted all-pairs R^k requests in arrival order into launch sets
+ * of at most max_batch plans while one of max_inflight slots is free and the
+ * in-flight device bytes (plans, staged inputs and outputs) stay within
+ * budget_bytes; each set's plans come from one ax_rk_tensor_create_many
+ * call, its orbitals are packed into pinned staging by worker threads and
+ * moved with one copy, its plans run as one ax_rk_tensor_run_batch and its
+ * results come back with one copy into a pinned buffer of the set.  No
+ * reference counterpart: the reference evaluates integrals one call at a
+ * time (numerics/bspline/slater.py:118-130); this is the serving loop of the
+ * same values.  destroy() finishes every queued and in-flight request; the
+ * values of tickets not released by then stay valid until the process ends. */
+typedef struct ax_server ax_server;
+ax_status ax_server_create(int64_t budget_bytes, int max_inflight, int max_batch,
+                           ax_server **server);
+ax_status ax_server_destroy(ax_server *server);
+/* Queues the request of ax_rk_tensor_create(r_host, w_host, nv, ng, n_orb,
+ * l_host, kmax, precision) run on P_host, Q_host (n_orb x nv*ng FP64 host
+ * arrays, Q_host may be NULL).  r_host, w_host, P_host and Q_host are read
+ * when the request is launched and must stay valid until its ticket is
+ * complete; l_host is copied.  AX_ERR_MEMORY when the request alone needs
+ * more than the budget. */
+ax_status ax_server_submit(ax_server *server, const double *r_host, const double *w_host, int nv,
+                           int ng, int n_orb, const int32_t *l_host, int kmax, int precision,
+                           const double *P_host, const double *Q_host, int64_t *ticket);
+ax_status ax_server_poll(ax_server *server, int64_t ticket, int32_t *done);
+ax_status ax_server_wait(ax_server *server, int64_t ticket);
+/* A complete ticket's packed R^k values (the layout of ax_rk_tensor_run:
+ * per k the packed upper triangle over the valid pairs) in server-owned
+ * pinned memory, valid until ax_server_release; its status is the request's
+ * (a failed request returns its error here).  slices: int8 slices the
+ * contraction used (0: FP64). */
+ax_status ax_server_result(ax_server *server, int64_t ticket, const double **values,
+                           int64_t *size, int32_t *slices);
+ax_status ax_server_release(ax_server *server, int64_t ticket);
+/* Counters since creation, the first n of: launch sets, plans, peak
+ * in-flight device bytes, and host nanoseconds of the scheduler thread in
+ * plan set-up, waiting for the packing, allocations, and queueing copies
+ * and launches. */
+ax_status ax_server_stats(ax_server *server, int64_t *out, int n);
 /* Pair-product tensor (the CI / CCSD B-spline product workload): n_fn
  * functions sampled on the grid (P rows; Q may be NULL), an explicit list of
  * n_pairs pairs (pa_host[p], pb_host[p]), every pair at every rank

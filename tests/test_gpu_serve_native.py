@@ -1,0 +1,75 @@
+"""GPU: the native serving loop (ax_server_*, NativeSlaterServer).
+
+Every request's values equal bitwise those of its plan run alone (a batch
+runs each plan's tiles, K splits and summation order unchanged), for mixed
+grids, orbital counts, k ranges and both precision modes, with and without
+the small component, through more launch sets than the server holds in
+flight; admission rejects a request above the budget; a bad request fails
+at submission; values held by the caller outlive the server.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(n, seed):
+    from atomx_amd.serve import SlaterRequest
+    from atomx_amd.workloads import synthetic_grid, synthetic_orbitals
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nv = int(rng.integers(30, 180))
+        n_orb = int(rng.integers(2, 9))
+        kmax = int(rng.integers(1, 7))
+        gr, grw = synthetic_grid(nv=nv, ng=8, r_min=1e-5, r_max=float(rng.uniform(30, 60)))
+        P, Q, ls = synthetic_orbitals(gr, grw, n_orb=n_orb, seed=seed * 1000 + i)
+        prec = "split" if i % 2 else "fp64"
+        out.append(SlaterRequest(gr=gr, grw=grw, ls=ls, kmax=kmax, P=P,
+                                 Q=Q if i % 3 else None, precision=prec))
+    return out
+
+
+def _alone(req):
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    pl = SlaterTensor(req.gr, req.grw, req.ls, req.kmax, req.precision)
+    try:
+        return dv.host(pl.run(req.P, req.Q)), pl.slices, pl.m, pl.offsets
+    finally:
+        pl.close()
+
+
+def test_native_server_bitwise_equals_plans_alone():
+    from atomx_amd.serve_native import NativeSlaterServer
+    reqs = _requests(37, seed=11)
+    with NativeSlaterServer(max_inflight=3, max_batch=5) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        res = [srv.result(t) for t in tickets]
+        st = srv.stats()
+    assert st["plans"] == len(reqs) and st["launches"] >= len(reqs) // 5
+    for r, x in zip(reqs, res):
+        ref, slices, m, offs = _alone(r)
+        assert np.array_equal(x.values, ref)
+        assert (x.slices, x.m, x.offsets) == (slices, m, offs)
+
+
+def test_native_server_budget_errors_and_held_values():
+    from atomx_amd.serve_native import NativeSlaterServer
+    reqs = _requests(3, seed=12)
+    srv = NativeSlaterServer(budget_bytes=1 << 20, max_inflight=2, max_batch=4)
+    with pytest.raises(MemoryError):  # one request alone above the budget
+        srv.submit(reqs[0])
+    srv.close()
+    with NativeSlaterServer(max_inflight=2, max_batch=4) as srv:
+        bad = _requests(1, seed=13)[0]
+        bad.ls = np.array(bad.ls).copy()
+        bad.ls[0] = -1
+        with pytest.raises(ValueError):
+            srv.submit(bad)
+        tickets = [srv.submit(r) for r in reqs]
+        held = [srv.result(t) for t in tickets]
+    # the server is gone; the values stay valid
+    for r, x in zip(reqs, held):
+        assert np.array_equal(x.values, _alone(r)[0])
