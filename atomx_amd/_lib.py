@@ -70,7 +70,7 @@ SIGNATURES = {
                                          _int, ctypes.POINTER(_vp)]),
     "ax_rk_tensor_destroy": (_int, [_vp]),
     "ax_rk_tensor_create_many": (_int, [_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _ip,
-                                        _ip, ctypes.POINTER(_vp), _ip, _ip,
+                                        _ip, ctypes.POINTER(_vp), _ip, _ip, _vp,
                                         ctypes.POINTER(_vp)]),
     "ax_server_create": (_int, [_i64, _int, _int, ctypes.POINTER(_vp)]),
     "ax_server_destroy": (_int, [_vp]),

@@ -36,8 +36,11 @@ size_t pool_class(size_t bytes);
 hipStream_t setup_stream();
 // Pinned host staging buffer of at least `bytes` (reused once the event of
 // its previous release has completed); staging_release records that event
-// on `stream` after the copies out of the buffer have been queued there.
-hipError_t staging_acquire(size_t bytes, void **ptr, int *slot);
+// on `stream` after the copies out of the buffer, and the reads of its
+// device mirror, have been queued there.  With `dev` non-null the slot also
+// hands out a device buffer of the same size on the current device (kept
+// with the slot), so a whole staging buffer moves with one copy.
+hipError_t staging_acquire(size_t bytes, void **ptr, int *slot, void **dev = nullptr);
 hipError_t staging_release(int slot, hipStream_t stream);
 
 }  // namespace ax

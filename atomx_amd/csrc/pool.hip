@@ -95,6 +95,8 @@ struct Staging {
     size_t bytes;
     hipEvent_t ready;  // recorded after the last copy out of the buffer
     bool busy;
+    void *dev;       // device mirror (null until a caller asks for one)
+    int dev_device;  // the device that holds it
 };
 
 std::mutex g_smu;
@@ -102,25 +104,45 @@ std::vector<Staging> g_staging;
 
 }  // namespace
 
-hipError_t staging_acquire(size_t bytes, void **ptr, int *slot) {
+hipError_t staging_acquire(size_t bytes, void **ptr, int *slot, void **dev) {
     const size_t cls = size_class(bytes == 0 ? 1 : bytes);
+    int device = 0;
+    if (dev) (void)hipGetDevice(&device);
     std::lock_guard<std::mutex> lk(g_smu);
+    auto mirror = [&](Staging &st) -> hipError_t {
+        if (!dev) return hipSuccess;
+        if (!st.dev) {
+            const hipError_t e = hipMalloc(&st.dev, cls);
+            if (e != hipSuccess) {
+                st.dev = nullptr;
+                return e;
+            }
+            st.dev_device = device;
+        }
+        *dev = st.dev;
+        return hipSuccess;
+    };
     for (size_t i = 0; i < g_staging.size(); ++i) {
         Staging &st = g_staging[i];
         if (st.busy || st.bytes != cls) continue;
+        if (dev && st.dev && st.dev_device != device) continue;
         const hipError_t q = hipEventQuery(st.ready);
         if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) return q;
+        const hipError_t e = mirror(st);
+        if (e != hipSuccess) return e;
         st.busy = true;
         *ptr = st.ptr;
         *slot = static_cast<int>(i);
         return hipSuccess;
     }
-    Staging st{nullptr, cls, nullptr, true};
+    Staging st{nullptr, cls, nullptr, true, nullptr, device};
     hipError_t e = hipHostMalloc(&st.ptr, cls, hipHostMallocDefault);
     if (e != hipSuccess) return e;
     e = hipEventCreateWithFlags(&st.ready, hipEventDisableTiming);
+    if (e == hipSuccess) e = mirror(st);
     if (e != hipSuccess) {
+        if (st.ready) (void)hipEventDestroy(st.ready);
         (void)hipHostFree(st.ptr);
         return e;
     }

@@ -364,21 +364,32 @@ struct Setup {
     double *rk = nullptr, *rkm = nullptr;
 };
 
-// r^k and r^-(k+1) tables of up to kMaxBatch plans in one launch (the
-// values of rpow_kernel, numerics/yk.py:113-117).
-struct RpowArg {
-    const double *r;
+// Set-up of up to kMaxBatch plans in one launch: each plan's uploaded image
+// is copied from the device mirror of the staging buffer into its block, 16
+// bytes per item, and its r^k and r^-(k+1) tables (the values of
+// rpow_kernel, numerics/yk.py:113-117) are formed from the r values of that
+// same mirror, so neither part waits for the other.
+struct SetupArg {
+    const uint4 *src;  // the plan's image in the device mirror
+    uint4 *dst;        // the plan's block
+    const double *r;   // its r values in the mirror
     double *rk, *rkm;
-    int64_t n;
-    int kmax, pad;
+    int64_t n16;  // 16-byte units of the image (images are 256-byte multiples)
+    int64_t n;    // nodes
+    int64_t kmax;
 };
-__global__ void rpow_batch_kernel(const ax::BatchOf<RpowArg> A) {
+__global__ void setup_batch_kernel(const ax::BatchOf<SetupArg> A) {
     const int64_t total = A.end[A.n - 1];
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
         int64_t local;
         const int p = ax::batch_find(A, t, local);
-        const RpowArg &g = A.a[p];
+        const SetupArg &g = A.a[p];
+        if (local < g.n16) {
+            g.dst[local] = g.src[local];
+            continue;
+        }
+        local -= g.n16;
         const int k = static_cast<int>(local / g.n);
         const double rv = g.r[local - (int64_t)k * g.n];
         g.rk[local] = pow(rv, static_cast<double>(k));
@@ -386,11 +397,12 @@ __global__ void rpow_batch_kernel(const ax::BatchOf<RpowArg> A) {
     }
 }
 
-// Uploads the set-up images of n built plans: one pinned staging buffer and
-// one copy per plan on the creating thread's set-up stream, the r-power
-// tables in launches of up to kMaxBatch plans, and one `ready` mark that
-// every plan shares.  On failure every plan is freed.
-ax_status upload_plans(Setup *su, int n) {
+// Uploads the set-up images of n built plans on `ss`: one pinned staging
+// buffer moved with one copy into its device mirror, then launches of up to
+// kMaxBatch plans that scatter the images into the plans' blocks and form
+// the r-power tables, and one `ready` mark that every plan shares.  On
+// failure every plan is freed.
+ax_status upload_plans(Setup *su, int n, hipStream_t ss) {
     auto fail_all = [&](ax_status st) {
         for (int i = 0; i < n; ++i) {
             if (!su[i].plan) continue;
@@ -402,38 +414,46 @@ ax_status upload_plans(Setup *su, int n) {
     };
     std::vector<int64_t> off(static_cast<size_t>(n) + 1, 0);
     for (int i = 0; i < n; ++i) off[i + 1] = off[i] + align256((int64_t)su[i].image.size());
-    void *stage = nullptr;
+    void *stage = nullptr, *mirror = nullptr;
     int slot = -1;
-    hipError_t e = ax::staging_acquire(static_cast<size_t>(off[n]), &stage, &slot);
+    hipError_t e = ax::staging_acquire(static_cast<size_t>(off[n]), &stage, &slot, &mirror);
     if (e != hipSuccess) {
-        ax::set_error("pinned staging of %lld bytes failed: %s", (long long)off[n],
+        ax::set_error("set-up staging of %lld bytes failed: %s", (long long)off[n],
                       hipGetErrorString(e));
-        return fail_all(AX_ERR_HIP);
+        return fail_all(e == hipErrorOutOfMemory ? AX_ERR_MEMORY : AX_ERR_HIP);
     }
     char *img = static_cast<char *>(stage);
     for (int i = 0; i < n; ++i)
         if (!su[i].image.empty()) std::memcpy(img + off[i], su[i].image.data(), su[i].image.size());
-    hipStream_t ss = ax::setup_stream();
-    for (int i = 0; i < n && e == hipSuccess; ++i)
-        if (!su[i].image.empty())
-            e = hipMemcpyAsync(su[i].block, img + off[i], su[i].image.size(),
-                               hipMemcpyHostToDevice, ss);
-    const hipError_t er = ax::staging_release(slot, ss);
-    if (e == hipSuccess) e = er;
+    e = hipMemcpyAsync(mirror, stage, static_cast<size_t>(off[n]), hipMemcpyHostToDevice, ss);
+    const char *dimg = static_cast<const char *>(mirror);
     for (int i0 = 0; i0 < n && e == hipSuccess; i0 += ax::kMaxBatch) {
-        ax::BatchOf<RpowArg> A{};
+        ax::BatchOf<SetupArg> A{};
         A.n = std::min(ax::kMaxBatch, n - i0);
         int64_t total = 0;
         for (int j = 0; j < A.n; ++j) {
-            const ax_rk_tensor *p = su[i0 + j].plan;
-            A.a[j] = RpowArg{su[i0 + j].r, su[i0 + j].rk, su[i0 + j].rkm, p->n, p->kmax, 0};
-            total += p->n * (p->kmax + 1);
+            const Setup &u = su[i0 + j];
+            const ax_rk_tensor *p = u.plan;
+            const int64_t o_r = reinterpret_cast<const char *>(u.r) -
+                                static_cast<const char *>(u.block);
+            SetupArg &g = A.a[j];
+            g.src = reinterpret_cast<const uint4 *>(dimg + off[i0 + j]);
+            g.dst = static_cast<uint4 *>(u.block);
+            g.r = reinterpret_cast<const double *>(dimg + off[i0 + j] + o_r);
+            g.rk = u.rk;
+            g.rkm = u.rkm;
+            g.n16 = static_cast<int64_t>(u.image.size()) / 16;
+            g.n = p->n;
+            g.kmax = p->kmax;
+            total += g.n16 + p->n * (p->kmax + 1);
             A.end[j] = total;
         }
-        hipLaunchKernelGGL(rpow_batch_kernel, dim3(ax::grid_for(total, 256, 4096)), dim3(256), 0,
+        hipLaunchKernelGGL(setup_batch_kernel, dim3(ax::grid_for(total, 256, 4096)), dim3(256), 0,
                            ss, A);
         e = hipGetLastError();
     }
+    const hipError_t er = ax::staging_release(slot, ss);
+    if (e == hipSuccess) e = er;
     DoneMark *m = e == hipSuccess ? mark_acquire() : nullptr;
     if (e == hipSuccess && !m) e = hipErrorOutOfMemory;
     if (e == hipSuccess) e = hipEventRecord(m->ev, ss);
@@ -775,7 +795,7 @@ ax_status plan_create(const double *r_host, const double *w_host, int nv, int ng
         *setup_out = std::move(su);
         return AX_OK;
     }
-    const ax_status st = upload_plans(&su, 1);
+    const ax_status st = upload_plans(&su, 1, ax::setup_stream());
     if (st == AX_OK) *out_plan = plan;
     return st;
 }
@@ -795,7 +815,8 @@ ax_status ax_rk_tensor_create_many(int n_plans, const double *const *r_host,
                                    const double *const *w_host, const int32_t *nv,
                                    const int32_t *ng, const int32_t *n_orb,
                                    const int32_t *const *l_host, const int32_t *kmax,
-                                   const int32_t *precision, ax_rk_tensor **out_plans) {
+                                   const int32_t *precision, void *stream,
+                                   ax_rk_tensor **out_plans) {
     AX_REQUIRE(n_plans >= 0 && (n_plans == 0 || (r_host && w_host && nv && ng && n_orb && l_host &&
                                                    kmax && precision && out_plans)),
                "ax_rk_tensor_create_many: null argument");
@@ -813,7 +834,8 @@ ax_status ax_rk_tensor_create_many(int n_plans, const double *const *r_host,
         }
     }
     if (n_plans == 0) return AX_OK;
-    const ax_status st = upload_plans(su.data(), n_plans);
+    const ax_status st = upload_plans(
+        su.data(), n_plans, stream ? static_cast<hipStream_t>(stream) : ax::setup_stream());
     if (st != AX_OK) return st;
     for (int i = 0; i < n_plans; ++i) out_plans[i] = su[i].plan;
     return AX_OK;

@@ -343,7 +343,8 @@ bool ax_server::launch(std::vector<Request> &batch, int slot_id, int64_t bytes, 
     std::vector<std::string> errs(n);
     std::vector<ax_status> sts(n, AX_OK);
     if (ax_rk_tensor_create_many(n, rr.data(), ww.data(), nv.data(), ng.data(), no.data(),
-                                 ll.data(), km.data(), pr.data(), plans.data()) != AX_OK) {
+                                 ll.data(), km.data(), pr.data(), slot.stream,
+                                 plans.data()) != AX_OK) {
         for (int i = 0; i < n; ++i) {
             sts[i] = ax_rk_tensor_create(rr[i], ww[i], nv[i], ng[i], no[i], ll[i], km[i], pr[i],
                                          &plans[i]);

@@ -868,7 +868,7 @@ class GpuExecutor:
         host, dev = self._staging(job["sid"], self._host_need(n_in, out), out)
         hv = host.numpy()
         packing = self._pack_start(chunks, hv[:n_in])
-        tune, failed = self._make_plans(job, entries)
+        tune, failed = self._make_plans(job, entries, stream)
         self._pack_wait(packing)
         base = dev.data_ptr()
         pp, qq, oo, plans = [], [], [], []
@@ -917,11 +917,12 @@ class GpuExecutor:
                 sts.append(st)
                 seen.add(id(st))
 
-    def _make_plans(self, job, entries):
+    def _make_plans(self, job, entries, stream):
         """Plans for the entries whose state has none: all of one precision in
-        one library call (one staging upload, one shared set-up event); when
-        that fails, one by one, so that a bad request fails alone.  Returns
-        (new resident plans to tune, indices of the requests that failed)."""
+        one library call (one staging upload, one shared set-up event, queued
+        on the launch set's own stream); when that fails, one by one, so that
+        a bad request fails alone.  Returns (new resident plans to tune,
+        indices of the requests that failed)."""
         from .tensor import SlaterTensor
         tune, failed = [], set()
         need = {}
@@ -931,7 +932,7 @@ class GpuExecutor:
         for prec, es in need.items():
             try:
                 made = SlaterTensor.create_many([(e[2].gr, e[2].grw, e[2].ls, e[2].kmax)
-                                                 for e in es], prec)
+                                                 for e in es], prec, stream=stream)
             except (RuntimeError, ValueError, MemoryError):
                 made = None
             for k, e in enumerate(es):

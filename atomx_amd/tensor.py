@@ -73,10 +73,13 @@ class SlaterTensor:
         self.slices = int(slices.value)
 
     @classmethod
-    def create_many(cls, specs, precision="fp64"):
+    def create_many(cls, specs, precision="fp64", stream=None):
         """All-pairs plans for ``specs`` = [(gr, grw, ls, kmax), ...] in one
-        library call (one staging upload, batched r-power tables, one shared
-        set-up event); each plan is then used and closed on its own."""
+        library call (one staging upload, batched scatter and r-power tables,
+        one shared set-up event); each plan is then used and closed on its
+        own.  ``stream``: the torch stream to queue the set-up on (the one
+        the plans will run on, say); None = the library's own set-up stream
+        of the calling thread."""
         dv.require_gpu()
         lib = load()
         if precision not in PRECISIONS:
@@ -113,7 +116,8 @@ class SlaterTensor:
         ip = ctypes.POINTER(ctypes.c_int32)
         call("ax_rk_tensor_create_many", n, r_arr, w_arr, shape[0].ctypes.data_as(ip),
              shape[1].ctypes.data_as(ip), shape[2].ctypes.data_as(ip), l_arr,
-             shape[3].ctypes.data_as(ip), prec.ctypes.data_as(ip), hs)
+             shape[3].ctypes.data_as(ip), prec.ctypes.data_as(ip),
+             None if stream is None else dv.stream_handle(stream), hs)
         for o, h in zip(objs, hs):
             o._attach(lib, ctypes.c_void_p(h))
         return objs

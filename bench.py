@@ -504,6 +504,11 @@ def serving_leg(args):
         "native_saturated_cold_rk_per_s": total_rk / t_nat[0],
         "native_saturated_vs_sequential_fp64": t_seq / float(np.median(t_nat[1:])),
         "native_launches": st_nat["launches"] // len(t_nat),
+        "native_pass_ms": [round(1e3 * t, 2) for t in t_nat],
+        "native_host_ms_per_pass": {k: round(st_nat[k] / len(t_nat), 2) for k in
+                                    ("setup_ms", "pack_wait_ms", "alloc_ms", "queue_ms")},
+        "native_peak_device_bytes": st_nat["peak_bytes"],
+        "saturated_pass_ms": [round(1e3 * t, 2) for t in t_sat_all],
         "native_vs_python_server_max_abs_diff": nat_diff,
         "saturated_cold_vs_sequential_fp64": t_seq / t_sat_cold,
         "split_requests_sliced": sum(1 for (_, q), r in zip(stream, res)

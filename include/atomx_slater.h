@@ -184,17 +184,23 @@ ax_status ax_rk_tensor_create(const double *r_host, const double *w_host, int nv
 ax_status ax_rk_tensor_destroy(ax_rk_tensor *plan);
 /* n_plans plans in one call, plan i as ax_rk_tensor_create(r_host[i],
  * w_host[i], nv[i], ng[i], n_orb[i], l_host[i], kmax[i], precision[i]): the
- * set-up images go through one pinned staging buffer, the r-power tables of
- * up to 16 plans are formed by one launch and the plans share one set-up
- * completion event (serving: every one-shot plan of a launch set).  Each
- * plan is destroyed on its own.  On failure no plan is left and every
- * out_plans[i] is NULL.  No reference counterpart: it batches the set-up of
- * plans that numerics/bspline/slater.py:118-130 would evaluate one by one. */
+ * set-up images go up with one copy through one pinned staging buffer, one
+ * launch per 16 plans scatters them into the plans' blocks and forms their
+ * r-power tables, and the plans share one set-up completion event (serving:
+ * every one-shot plan of a launch set).  That set-up is queued on `stream`
+ * (NULL here means the calling thread's own set-up stream, not the default
+ * stream); launches of the plans on any stream wait for it on the device.
+ * Passing the stream the plans will run on keeps the set-up off other
+ * streams' hardware queues.  Each plan is destroyed on its own.  On failure
+ * no plan is left and every out_plans[i] is NULL.  No reference
+ * counterpart: it batches the set-up of plans that
+ * numerics/bspline/slater.py:118-130 would evaluate one by one. */
 ax_status ax_rk_tensor_create_many(int n_plans, const double *const *r_host,
                                    const double *const *w_host, const int32_t *nv,
                                    const int32_t *ng, const int32_t *n_orb,
                                    const int32_t *const *l_host, const int32_t *kmax,
-                                   const int32_t *precision, ax_rk_tensor **out_plans);
+                                   const int32_t *precision, void *stream,
+                                   ax_rk_tensor **out_plans);
 /* Output size, slice count (0: FP64 contraction) and, for k = 0..kmax, the
  * valid pairs and the packed offset of rank k, in one call (the values of
  * ax_rk_tensor_size / _slices / _pairs / _offset). */

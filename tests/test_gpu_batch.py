@@ -395,6 +395,34 @@ def test_create_many_equals_single_plans(problems, precision):
         p.close()
 
 
+def test_create_many_on_stream(problems):
+    """Set-up queued on a caller's stream for more plans than one set-up
+    launch takes (16 per launch; 20 here): every plan computes bitwise what
+    a plan created alone does, run on that stream or on another one, which
+    waits for the set-up on the device."""
+    import torch
+
+    from atomx_amd import device as dv
+    from atomx_amd.tensor import SlaterTensor
+    specs = [(gr, grw, ls, kmax) for gr, grw, P, Q, ls, kmax in problems] * 4
+    ins = [(dv.to_dev(P), dv.to_dev(Q)) for _, _, P, Q, _, _ in problems]
+    single = [SlaterTensor(gr, grw, ls, kmax, "split") for gr, grw, ls, kmax in specs[:5]]
+    ref = [dv.host(p.run(P, Q)) for p, (P, Q) in zip(single, ins)]
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    many = SlaterTensor.create_many(specs, "split", stream=s1)
+    outs = []
+    for i, p in enumerate(many):
+        st = s1 if i % 2 == 0 else s2
+        with torch.cuda.stream(st):
+            outs.append(p.run(*ins[i % 5], stream=st))
+    torch.cuda.synchronize()
+    for i, o in enumerate(outs):
+        assert np.array_equal(dv.host(o), ref[i % 5]), i
+    for p in many + single:
+        p.close()
+
+
 def test_create_many_failure(problems):
     """A bad specification fails the whole call with the library's error (the
     library frees the plans it built before it); a later call on good
