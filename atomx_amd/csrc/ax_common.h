@@ -34,14 +34,21 @@ size_t pool_class(size_t bytes);
 // Non-blocking stream of the calling thread on the current device, for
 // set-up copies (never destroyed).
 hipStream_t setup_stream();
-// Pinned host staging buffer of at least `bytes` (reused once the event of
-// its previous release has completed); staging_release records that event
-// on `stream` after the copies out of the buffer, and the reads of its
-// device mirror, have been queued there.  With `dev` non-null the slot also
-// hands out a device buffer of the same size on the current device (kept
-// with the slot), so a whole staging buffer moves with one copy.
-hipError_t staging_acquire(size_t bytes, void **ptr, int *slot, void **dev = nullptr);
-hipError_t staging_release(int slot, hipStream_t stream);
+// Pinned host staging buffer of at least `bytes` for work queued on
+// `stream`; staging_release records the slot's event on that stream after
+// the copies out of the buffer, and the reads of its device mirror, have been
+// queued there, and the slot is handed out again (to users of the same
+// stream only) once that event has completed.  A slot is bound to one stream
+// because an event last recorded on a stream that has since been destroyed
+// can no longer be queried.  With `dev` non-null the slot also hands out a
+// device buffer of the same size on the current device (kept with the slot),
+// so a whole staging buffer moves with one copy.
+hipError_t staging_acquire(size_t bytes, hipStream_t stream, void **ptr, int *slot,
+                           void **dev = nullptr);
+hipError_t staging_release(int slot);
+// Frees the idle staging slots bound to `stream`, whose queued work must have
+// completed; called before a library-owned stream is destroyed.
+void staging_forget_stream(hipStream_t stream);
 
 }  // namespace ax
 

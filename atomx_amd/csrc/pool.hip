@@ -95,8 +95,9 @@ struct Staging {
     size_t bytes;
     hipEvent_t ready;  // recorded after the last copy out of the buffer
     bool busy;
-    void *dev;       // device mirror (null until a caller asks for one)
-    int dev_device;  // the device that holds it
+    void *dev;           // device mirror (null until a caller asks for one)
+    int dev_device;      // the device that holds it
+    hipStream_t stream;  // the stream its releases are recorded on
 };
 
 std::mutex g_smu;
@@ -104,7 +105,7 @@ std::vector<Staging> g_staging;
 
 }  // namespace
 
-hipError_t staging_acquire(size_t bytes, void **ptr, int *slot, void **dev) {
+hipError_t staging_acquire(size_t bytes, hipStream_t stream, void **ptr, int *slot, void **dev) {
     const size_t cls = size_class(bytes == 0 ? 1 : bytes);
     int device = 0;
     if (dev) (void)hipGetDevice(&device);
@@ -124,7 +125,7 @@ hipError_t staging_acquire(size_t bytes, void **ptr, int *slot, void **dev) {
     };
     for (size_t i = 0; i < g_staging.size(); ++i) {
         Staging &st = g_staging[i];
-        if (st.busy || st.bytes != cls) continue;
+        if (st.busy || st.bytes != cls || st.stream != stream || !st.ptr) continue;
         if (dev && st.dev && st.dev_device != device) continue;
         const hipError_t q = hipEventQuery(st.ready);
         if (q == hipErrorNotReady) continue;
@@ -136,7 +137,7 @@ hipError_t staging_acquire(size_t bytes, void **ptr, int *slot, void **dev) {
         *slot = static_cast<int>(i);
         return hipSuccess;
     }
-    Staging st{nullptr, cls, nullptr, true, nullptr, device};
+    Staging st{nullptr, cls, nullptr, true, nullptr, device, stream};
     hipError_t e = hipHostMalloc(&st.ptr, cls, hipHostMallocDefault);
     if (e != hipSuccess) return e;
     e = hipEventCreateWithFlags(&st.ready, hipEventDisableTiming);
@@ -146,18 +147,35 @@ hipError_t staging_acquire(size_t bytes, void **ptr, int *slot, void **dev) {
         (void)hipHostFree(st.ptr);
         return e;
     }
-    g_staging.push_back(st);
+    size_t at = 0;  // an entry freed by staging_forget_stream, else a new one
+    while (at < g_staging.size() && g_staging[at].ptr) ++at;
+    if (at == g_staging.size())
+        g_staging.push_back(st);
+    else
+        g_staging[at] = st;
     *ptr = st.ptr;
-    *slot = static_cast<int>(g_staging.size() - 1);
+    *slot = static_cast<int>(at);
     return hipSuccess;
 }
 
-hipError_t staging_release(int slot, hipStream_t stream) {
+hipError_t staging_release(int slot) {
     std::lock_guard<std::mutex> lk(g_smu);
     Staging &st = g_staging[static_cast<size_t>(slot)];
-    const hipError_t e = hipEventRecord(st.ready, stream);
+    const hipError_t e = hipEventRecord(st.ready, st.stream);
     st.busy = false;
     return e;
+}
+
+void staging_forget_stream(hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g_smu);
+    for (Staging &st : g_staging) {
+        if (st.busy || st.stream != stream || !st.ptr) continue;
+        (void)hipEventSynchronize(st.ready);
+        (void)hipEventDestroy(st.ready);
+        (void)hipHostFree(st.ptr);
+        if (st.dev) (void)hipFree(st.dev);
+        st = Staging{nullptr, 0, nullptr, false, nullptr, 0, nullptr};  // slot index stays
+    }
 }
 
 hipStream_t setup_stream() {

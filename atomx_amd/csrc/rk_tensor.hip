@@ -416,7 +416,7 @@ ax_status upload_plans(Setup *su, int n, hipStream_t ss) {
     for (int i = 0; i < n; ++i) off[i + 1] = off[i] + align256((int64_t)su[i].image.size());
     void *stage = nullptr, *mirror = nullptr;
     int slot = -1;
-    hipError_t e = ax::staging_acquire(static_cast<size_t>(off[n]), &stage, &slot, &mirror);
+    hipError_t e = ax::staging_acquire(static_cast<size_t>(off[n]), ss, &stage, &slot, &mirror);
     if (e != hipSuccess) {
         ax::set_error("set-up staging of %lld bytes failed: %s", (long long)off[n],
                       hipGetErrorString(e));
@@ -452,7 +452,7 @@ ax_status upload_plans(Setup *su, int n, hipStream_t ss) {
                            ss, A);
         e = hipGetLastError();
     }
-    const hipError_t er = ax::staging_release(slot, ss);
+    const hipError_t er = ax::staging_release(slot);
     if (e == hipSuccess) e = er;
     DoneMark *m = e == hipSuccess ? mark_acquire() : nullptr;
     if (e == hipSuccess && !m) e = hipErrorOutOfMemory;

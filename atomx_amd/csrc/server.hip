@@ -222,12 +222,7 @@ double *ax_server::pinned_acquire(size_t bytes, size_t *got) {
         }
     }
     void *p = nullptr;
-    const int64_t t0 = now_ns();
     const hipError_t e = hipHostMalloc(&p, cls, hipHostMallocDefault);
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        ns_alloc += now_ns() - t0;
-    }
     if (e != hipSuccess) return nullptr;
     *got = cls;
     return static_cast<double *>(p);
@@ -298,7 +293,9 @@ bool ax_server::launch(std::vector<Request> &batch, int slot_id, int64_t bytes, 
     for (const Request &r : batch) n_in += r.n_in;
     const size_t in_bytes = static_cast<size_t>(n_in) * 8;
     size_t staging_bytes = 0;
+    const int64_t t_stage = now_ns();
     double *staging = pinned_acquire(in_bytes, &staging_bytes);
+    const int64_t ns_stage = now_ns() - t_stage;
     if (!staging) {
         fail_all(AX_ERR_MEMORY, "pinned input staging allocation failed");
         return false;
@@ -414,7 +411,7 @@ bool ax_server::launch(std::vector<Request> &batch, int slot_id, int64_t bytes, 
     std::lock_guard<std::mutex> lk(mu);
     ns_setup += t_pack - t_setup;
     ns_pack_wait += t_alloc - t_pack;
-    ns_alloc += t_queue - t_alloc;  // device staging and result buffers
+    ns_alloc += ns_stage + (t_queue - t_alloc);  // input staging, device and result buffers
     ns_queue += t_end - t_queue;
     for (int i = 0; i < n; ++i)
         if (!plans[i]) fail_ticket_locked(batch[i].ticket, sts[i], errs[i]);
@@ -537,7 +534,12 @@ ax_status ax_server_destroy(ax_server *s) {
     for (PackPool *p : s->pack) delete p;
     for (Slot &sl : s->slots) {
         if (sl.dev) ax::pool_free(sl.dev);
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.stream) {
+            // plan set-up staging bound to this stream goes with it
+            (void)hipStreamSynchronize(sl.stream);
+            ax::staging_forget_stream(sl.stream);
+            (void)hipStreamDestroy(sl.stream);
+        }
     }
     // results not released yet stay valid: their pinned buffers are left to
     // the process (a caller may still hold views of them)

@@ -191,7 +191,8 @@ ax_status ax_rk_tensor_destroy(ax_rk_tensor *plan);
  * (NULL here means the calling thread's own set-up stream, not the default
  * stream); launches of the plans on any stream wait for it on the device.
  * Passing the stream the plans will run on keeps the set-up off other
- * streams' hardware queues.  Each plan is destroyed on its own.  On failure
+ * streams' hardware queues; that stream must stay alive until each plan has
+ * run once or been destroyed.  Each plan is destroyed on its own.  On failure
  * no plan is left and every out_plans[i] is NULL.  No reference
  * counterpart: it batches the set-up of plans that
  * numerics/bspline/slater.py:118-130 would evaluate one by one. */

@@ -5,7 +5,8 @@ runs each plan's tiles, K splits and summation order unchanged), for mixed
 grids, orbital counts, k ranges and both precision modes, with and without
 the small component, through more launch sets than the server holds in
 flight; admission rejects a request above the budget; a bad request fails
-at submission; values held by the caller outlive the server.
+at submission; values held by the caller outlive the server; plans set up
+after the server has closed work.
 """
 
 import numpy as np
@@ -73,3 +74,24 @@ def test_native_server_budget_errors_and_held_values():
     # the server is gone; the values stay valid
     for r, x in zip(reqs, held):
         assert np.array_equal(x.values, _alone(r)[0])
+
+
+def test_plans_after_server_close():
+    """Plans set up after a server has closed (its streams destroyed) work:
+    the server's one-plan launch sets used set-up staging of the same size
+    class as the single plans made afterwards, and that staging went away
+    with the server's streams instead of being handed out again."""
+    from atomx_amd.serve_native import NativeSlaterServer
+    from atomx_amd.tensor import SlaterTensor
+    from atomx_amd import device as dv
+    reqs = _requests(6, seed=14)
+    with NativeSlaterServer(max_inflight=2, max_batch=1) as srv:
+        res = [srv.result(t) for t in [srv.submit(r) for r in reqs]]
+    for r, x in zip(reqs, res):
+        assert np.array_equal(x.values, _alone(r)[0])
+    many = SlaterTensor.create_many([(r.gr, r.grw, r.ls, r.kmax) for r in reqs[:2]], "fp64")
+    for pl, r in zip(many, reqs[:2]):
+        one = SlaterTensor(r.gr, r.grw, r.ls, r.kmax, "fp64")
+        assert np.array_equal(dv.host(pl.run(r.P, r.Q)), dv.host(one.run(r.P, r.Q)))
+        one.close()
+        pl.close()
