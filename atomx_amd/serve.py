@@ -407,9 +407,12 @@ class GpuExecutor:
 
     # -- footprints ----------------------------------------------------------------
     def _info(self, req):
-        """(shape key, device bytes of a fresh state's plan, R^k count,
-        device staging bytes of the request's inputs and outputs), computed
-        once per request (the estimate once per request shape)."""
+        """(device bytes of a fresh state's plan, R^k count, device staging
+        bytes of the request's inputs and outputs), computed once per request
+        (the estimate once per request shape).  The state's shape key, which
+        hashes the grid, is left to state creation (``_new_state``): a
+        request of an existing state is matched by the identity of its
+        arrays."""
         info = getattr(req, "_ax_info", None)
         if info is None:
             if req.kind == "slater":
@@ -424,9 +427,9 @@ class GpuExecutor:
                     self._estimates[sk] = est
                     if len(self._estimates) > 4096:
                         self._estimates.pop(next(iter(self._estimates)))
-                info = (self._slater_shape(req), est[0], est[1], est[2])
+                info = est
             else:
-                info = (None, self._fock_bytes(req), 0, 0)
+                info = (self._fock_bytes(req), 0, 0)
             req._ax_info = info
         return info
 
@@ -472,20 +475,20 @@ class GpuExecutor:
             for a in req.atoms:
                 st = states.get((a.kind, a.state)) if a.state is not None else None
                 info = self._info(a)
-                total += (st.nbytes if st is not None else info[1]) + info[3]
+                total += (st.nbytes if st is not None else info[0]) + info[2]
             return total
         info = self._info(req)
         key = self._key(req)
         if key is not None and key in self.states:
-            return self.states[key].nbytes + info[3]
-        return info[1] + info[3]
+            return self.states[key].nbytes + info[2]
+        return info[0] + info[2]
 
     def _staging_need(self, reqs):
         """Device staging bytes a launch set's requests were charged."""
         total = 0
         for r in reqs:
             for a in (r.atoms if r.kind == "slater_batch" else (r,)):
-                total += self._info(a)[3]
+                total += self._info(a)[2]
         return total
 
     def ready(self, req):
@@ -526,11 +529,7 @@ class GpuExecutor:
             return
         st = self.states.get(key)
         if st is None:  # placeholder until the launch builds it
-            shape, nbytes = self._info(req)[:2]
-            st = _State(key, shape, nbytes)
-            st.busy = True
-            if req.kind == "slater":
-                st.match = self._match(req)
+            st = self._new_state(req, key)
             self.states[key] = st
         else:
             self._set_busy(st, True)
@@ -603,13 +602,19 @@ class GpuExecutor:
                 st.match = self._match(req)
             self._set_busy(st, True)
             return st
-        shape, nbytes = self._info(req)[:2]
-        st = _State(key, shape, nbytes)
-        st.busy = True
-        if req.kind == "slater":
-            st.match = self._match(req)
+        st = self._new_state(req, key)
         if key is not None:
             self.states[key] = st
+        return st
+
+    def _new_state(self, req, key):
+        """A busy state for ``req``, with its shape key (the grid hashed
+        once per state) and the plan bytes of its estimate."""
+        slater = req.kind == "slater"
+        st = _State(key, self._slater_shape(req) if slater else None, self._info(req)[0])
+        st.busy = True
+        if slater:
+            st.match = self._match(req)
         return st
 
     def launch(self, reqs):
@@ -864,7 +869,7 @@ class GpuExecutor:
         out_off, out = [], n_in
         for e in entries:
             out_off.append(out)
-            out += e[3].plan.size if e[3].plan is not None else self._info(e[2])[2]
+            out += e[3].plan.size if e[3].plan is not None else self._info(e[2])[1]
         host, dev = self._staging(job["sid"], self._host_need(n_in, out), out)
         hv = host.numpy()
         packing = self._pack_start(chunks, hv[:n_in])
