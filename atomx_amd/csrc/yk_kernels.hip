@@ -1277,9 +1277,16 @@ ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t str
         return AX_ERR_VALUE;
     }
     // rows of up to kWgScanMaxCells cells: one register-resident scan launch
-    // per (n_gauss, in-scan slicing); longer rows: the LDS scan, and rows the
-    // LDS scan does not hold go through the generic kernel one plan at a time
-    for (int ng : {7, 8, 9}) {
+    // per (n_gauss, in-scan slicing, passes per wave), so that a short row
+    // never carries the registers and loads of the batch's longest (the
+    // passes per wave do not change a row's sums); longer rows: the LDS scan,
+    // and rows the LDS scan does not hold go through the generic kernel one
+    // plan at a time
+    auto q_class = [](int nv) {
+        return nv <= 64 * kWgScanWaves ? 1 : nv <= 128 * kWgScanWaves ? 2 : 4;
+    };
+    for (int c = 0; c < 9; ++c) {
+        const int ng = 7 + c / 3, qc = 1 << (c % 3);
         for (int sl = 0; sl < (ng == 8 ? 2 : 1); ++sl) {
             BatchOf<ScanArg> A{};
             int64_t total = 0;
@@ -1288,6 +1295,7 @@ ax_status scan_batch_launch(const PlanHost *const *plans, int n, hipStream_t str
             for (int i = 0; i < n; ++i) {
                 const PlanHost *h = plans[i];
                 if (h->ng != ng || h->nv > kWgScanMaxCells || h->n_items == 0) continue;
+                if (q_class(h->nv) != qc) continue;
                 const bool fused = h->precision == AX_PREC_SPLIT && h->Y == nullptr;
                 if (fused != (sl == 1)) continue;
                 total += h->n_items;
