@@ -80,7 +80,7 @@ struct DensArg {             // weighted pair densities (+ their slices)
     int8_t *Ds;              // split precision: slice planes of D, else null
     int32_t *De;
     int64_t npad, ds_plane;
-    int slices, pad;
+    int slices, chunks;      // chunks: 512-node blocks of one pair row
 };
 struct ScanArg {             // Y^k items of one plan
     const double *D;

@@ -83,67 +83,71 @@ __device__ inline void slice_pair(double v0, double v1, int64_t x, int8_t *__res
     if ((threadIdx.x & 31) == 0) *e_slot = ex;
 }
 
-// Grid (node chunks of 512, pairs of the batch).  Each thread forms two
-// consecutive nodes; rows of even length with 16-byte aligned bases are
-// read and written as double2.  Split plans also slice their rows (npad is
-// a multiple of 64, so every half-wave holds one whole K-tile).
+// One work-group per (plan, pair, 512-node chunk) of the batch, taken from
+// the prefix of each plan's pairs x chunks, so a plan's rows launch only
+// their own chunks (a short row beside a long one in the batch launches no
+// empty work-groups).  Each thread forms two consecutive nodes; rows of even
+// length with 16-byte aligned bases are read and written as double2.  Split
+// plans also slice their rows (npad is a multiple of 64 and a chunk starts
+// at a multiple of 512, so every half-wave holds one whole K-tile or lies
+// wholly past npad).
 __global__ __launch_bounds__(256) void pair_density_batch_kernel(
     const ax::BatchOf<ax::DensArg> A) {
     typedef double d2 __attribute__((ext_vector_type(2)));
     const int64_t total = A.end[A.n - 1];
-    for (int64_t gp = blockIdx.y; gp < total; gp += gridDim.y) {
-        int64_t p;
-        const int i = ax::batch_find(A, gp, p);
+    for (int64_t gb = blockIdx.x; gb < total; gb += gridDim.x) {
+        int64_t local;
+        const int i = ax::batch_find(A, gb, local);
         const ax::DensArg &g = A.a[i];
+        const int64_t p = local / g.chunks;
+        const int64_t x = (local - p * g.chunks) * 512 + (int64_t)threadIdx.x * 2;
         const int64_t n = g.n, ld = g.ld;
+        const int64_t lim = g.Ds ? g.npad : n;
+        if (x >= lim) continue;  // uniform over each half-wave (see above)
         const int64_t a = g.pa[p], b = g.pb[p];
         const double *Pa = g.P + a * ld;
         const double *Pb = g.P + b * ld;
         const double *Qa = g.Q ? g.Q + a * ld : nullptr;
         const double *Qb = g.Q ? g.Q + b * ld : nullptr;
         double *o = g.D + p * n;
-        const int64_t lim = g.Ds ? g.npad : n;
         const bool vec = (n % 2 == 0) && (ld % 2 == 0) &&
                          ((reinterpret_cast<uintptr_t>(g.P) | reinterpret_cast<uintptr_t>(g.w) |
                            reinterpret_cast<uintptr_t>(g.D) |
                            (g.Q ? reinterpret_cast<uintptr_t>(g.Q) : 0)) % 16 == 0);
-        for (int64_t x = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; x < lim;
-             x += (int64_t)gridDim.x * blockDim.x * 2) {
-            double v0 = 0.0, v1 = 0.0;
-            if (vec) {
-                const int64_t xc = x < n ? x : n - 2;
-                const d2 pa = *reinterpret_cast<const d2 *>(Pa + xc);
-                const d2 pb = *reinterpret_cast<const d2 *>(Pb + xc);
-                const d2 w = *reinterpret_cast<const d2 *>(g.w + xc);
-                d2 t = pa * pb;
-                if (Qa) {
-                    const d2 qa = *reinterpret_cast<const d2 *>(Qa + xc);
-                    const d2 qb = *reinterpret_cast<const d2 *>(Qb + xc);
-                    t = t + qa * qb;
-                }
-                t = w * t;
-                if (x < n) {
-                    *reinterpret_cast<d2 *>(o + x) = t;
-                    v0 = t.x;
-                    v1 = t.y;
-                }
-            } else {
+        double v0 = 0.0, v1 = 0.0;
+        if (vec) {
+            const int64_t xc = x < n ? x : n - 2;
+            const d2 pa = *reinterpret_cast<const d2 *>(Pa + xc);
+            const d2 pb = *reinterpret_cast<const d2 *>(Pb + xc);
+            const d2 w = *reinterpret_cast<const d2 *>(g.w + xc);
+            d2 t = pa * pb;
+            if (Qa) {
+                const d2 qa = *reinterpret_cast<const d2 *>(Qa + xc);
+                const d2 qb = *reinterpret_cast<const d2 *>(Qb + xc);
+                t = t + qa * qb;
+            }
+            t = w * t;
+            if (x < n) {
+                *reinterpret_cast<d2 *>(o + x) = t;
+                v0 = t.x;
+                v1 = t.y;
+            }
+        } else {
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int64_t xu = x + u;
-                    if (xu < n) {
-                        double t = Pa[xu] * Pb[xu];
-                        if (Qa) t += Qa[xu] * Qb[xu];
-                        t = g.w[xu] * t;
-                        o[xu] = t;
-                        (u ? v1 : v0) = t;
-                    }
+            for (int u = 0; u < 2; ++u) {
+                const int64_t xu = x + u;
+                if (xu < n) {
+                    double t = Pa[xu] * Pb[xu];
+                    if (Qa) t += Qa[xu] * Qb[xu];
+                    t = g.w[xu] * t;
+                    o[xu] = t;
+                    (u ? v1 : v0) = t;
                 }
             }
-            if (g.Ds)
-                slice_pair(v0, v1, x, g.Ds + p * g.npad, g.ds_plane,
-                           g.De + p * (g.npad / ax::kSplitKTile) + x / ax::kSplitKTile, g.slices);
         }
+        if (g.Ds)
+            slice_pair(v0, v1, x, g.Ds + p * g.npad, g.ds_plane,
+                       g.De + p * (g.npad / ax::kSplitKTile) + x / ax::kSplitKTile, g.slices);
     }
 }
 
@@ -1246,25 +1250,24 @@ ax_status density_batch_launch(const PlanHost *const *plans, int n, const double
     }
     BatchOf<DensArg> A{};
     A.n = n;
-    int64_t total = 0, nmax = 1;
+    int64_t total = 0;  // work-groups: pairs x 512-node chunks, plan after plan
     for (int i = 0; i < n; ++i) {
         const PlanHost *h = plans[i];
         if (!P[i]) {
             set_error("density_batch_launch: null orbital samples for plan %d", i);
             return AX_ERR_VALUE;
         }
-        total += h->n_pairs;
-        A.end[i] = total;
         const bool sl = h->precision == AX_PREC_SPLIT;
+        const int64_t len = sl ? h->npad : h->n;
+        const int chunks = static_cast<int>((len + 511) / 512);
+        total += (int64_t)h->n_pairs * chunks;
+        A.end[i] = total;
         A.a[i] = DensArg{P[i], Q ? Q[i] : nullptr, h->n, h->n, h->pa, h->pb, h->w, h->D,
                          sl ? h->Ds : nullptr, sl ? h->De : nullptr, h->npad,
-                         (int64_t)h->n_pairs * h->npad, h->slices, 0};
-        const int64_t len = sl ? h->npad : h->n;
-        if (len > nmax) nmax = len;
+                         (int64_t)h->n_pairs * h->npad, h->slices, chunks};
     }
     if (total == 0) return AX_OK;
-    const dim3 grid(static_cast<unsigned>((nmax + 511) / 512),
-                    total < 65535 ? static_cast<unsigned>(total) : 65535u);
+    const dim3 grid(static_cast<unsigned>(total < (1 << 22) ? total : (1 << 22)));
     hipLaunchKernelGGL(pair_density_batch_kernel, grid, dim3(256), 0, stream, A);
     AX_CHECK_LAUNCH();
     return AX_OK;
