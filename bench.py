@@ -428,11 +428,11 @@ def serving_leg(args):
         res = [f.result() for f in futs]
         t_open = time.perf_counter() - t0
         st_open = srv.stats()
-        # saturated: every Slater request submitted at once, four passes; the
+        # saturated: every Slater request submitted at once, six passes; the
         # first grows the server's pinned and device staging to these launch
         # sets (reported as the cold rate), the median of the others is timed
         t_sat_all = []
-        for _ in range(4):
+        for _ in range(6):
             srv.reset_stats()
             t0 = time.perf_counter()
             for f in [srv.submit(r) for r in slater]:
@@ -444,7 +444,7 @@ def serving_leg(args):
     from atomx_amd.serve_native import NativeSlaterServer
     t_nat = []
     with NativeSlaterServer(max_inflight=4, max_batch=16) as nsrv:
-        for i in range(4):  # the first pass grows the server's staging (cold)
+        for i in range(6):  # the first pass grows the server's staging (cold)
             nres = None
             if i == 1:
                 st_cold = nsrv.stats()
