@@ -4,6 +4,7 @@ server, for traces: every Slater request of the 300-request Poisson stream
 
     python -m atomx_amd.native_bench.serve_passes [--server native|python]
                                                    [--passes N] [--requests N]
+                                                   [--max-batch N] [--max-inflight N]
 
 Before each pass the device is synchronised and a one-element fill runs on
 the current stream (a ``FillFunctor`` dispatch in a kernel trace), so
@@ -26,6 +27,8 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--max-batch", type=int, default=16, help="plans per launch set")
+    ap.add_argument("--max-inflight", type=int, default=4, help="launch sets in flight")
     args = ap.parse_args(argv)
 
     import numpy as np
@@ -45,10 +48,10 @@ def main(argv=None):
     gc.freeze()
     if args.server == "native":
         from atomx_amd.serve_native import NativeSlaterServer
-        srv = NativeSlaterServer(max_inflight=4, max_batch=16)
+        srv = NativeSlaterServer(max_inflight=args.max_inflight, max_batch=args.max_batch)
     else:
         from atomx_amd.serve import SlaterServer
-        srv = SlaterServer(max_inflight=4, max_batch=16)
+        srv = SlaterServer(max_inflight=args.max_inflight, max_batch=args.max_batch)
     keys = ("setup_ms", "pack_wait_ms", "alloc_ms", "queue_ms")
     try:
         prev = srv.stats() if args.server == "native" else None
