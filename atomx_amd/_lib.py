@@ -75,7 +75,8 @@ SIGNATURES = {
     "ax_server_create": (_int, [_i64, _int, _int, ctypes.POINTER(_vp)]),
     "ax_server_destroy": (_int, [_vp]),
     "ax_server_submit": (_int, [_vp, _vp, _vp, _int, _int, _int, _vp, _int, _int, _vp, _vp,
-                                ctypes.POINTER(_i64)]),
+                                _i64, ctypes.POINTER(_i64)]),
+    "ax_server_drop_state": (_int, [_vp, _i64]),
     "ax_server_poll": (_int, [_vp, _i64, ctypes.POINTER(ctypes.c_int32)]),
     "ax_server_wait": (_int, [_vp, _i64]),
     "ax_server_result": (_int, [_vp, _i64, ctypes.POINTER(_vp), ctypes.POINTER(_i64),

@@ -11,6 +11,7 @@
 //     the pack threads,
 //   * creates the set's plans with one ax_rk_tensor_create_many call (a
 //     failing set is retried plan by plan, so a bad request fails alone),
+//     except for requests that name a resident state holding a plan,
 //   * waits for the packing, and on the slot's stream copies the inputs to
 //     the device with one copy, runs every plan as one batch and copies all
 //     results into a pinned buffer of the set with one copy, then records
@@ -20,6 +21,12 @@
 // ticket of the set has been released.  Several sets are in flight on
 // separate streams.  Each plan's results are bitwise those of the same plan
 // run alone (the batch does not change a plan's tiles, splits or order).
+// A request may name a resident state (one atom's requests, SCF iteration
+// after SCF iteration): the state keeps the plan its first request made and
+// later requests of the same shape reuse it; a state runs one request at a
+// time (later requests of other states go ahead of a waiting one), its plan
+// counts against the budget while it is held, and idle states give their
+// plans up, least recently used first, when a request does not fit.
 // The request surface is that of the reference's per-integral entry points
 // (numerics/bspline/slater.py:118-130) taken over all quartets at once.
 #include <algorithm>
@@ -28,6 +35,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -45,11 +53,32 @@ struct Request {
     std::vector<int32_t> l;
     int64_t n_in = 0;   // staged input values (P, and Q when given)
     int64_t bytes = 0;  // device bytes while in flight: plan + staged inputs and outputs
+    int64_t plan_bytes = 0;  // the plan's part of `bytes`
+    int64_t state = 0;       // resident state key (0: none)
+    struct State *st = nullptr;  // that state, set at admission
+};
+
+// A resident state: the plan of the requests that name one key (the next
+// SCF iteration of an atom), kept between them, and the shape it was made
+// for.  At most one request of a state is admitted at a time.
+struct State {
+    ax_rk_tensor *plan = nullptr;
+    int nv = 0, ng = 0, n_orb = 0, kmax = 0, precision = 0;
+    bool has_q = false;
+    std::vector<int32_t> l;
+    std::vector<double> r, w;  // the grid the plan was made for
+    int64_t plan_bytes = 0;    // charged to the resident bytes while reserved
+    bool charged = false;      // plan_bytes reserved (plan made or being made)
+    bool busy = false;         // a request of it is admitted and not complete
+    bool drop = false;         // free it once idle with nothing queued
+    int queued = 0;            // its requests still in the queue
+    int64_t last = 0;          // use stamp (least recently used goes first)
 };
 
 struct LaunchSet {
     int slot = -1;
     std::vector<ax_rk_tensor *> plans;
+    std::vector<State *> owner;    // per plan: its resident state (null: freed at completion)
     std::vector<int64_t> tickets;  // one per plan
     std::vector<int64_t> offset;   // result offset of each plan in `results`
     hipEvent_t done = nullptr;
@@ -173,6 +202,9 @@ struct ax_server {
     std::vector<LaunchSet *> inflight;
     std::vector<Slot> slots;
     std::multimap<size_t, double *> pinned_free;  // result buffers by size class
+    std::unordered_map<int64_t, std::unique_ptr<State>> states;
+    int64_t state_bytes = 0;  // plan bytes of the resident states
+    int64_t use_clock = 0, plans_made = 0;
     int64_t next_ticket = 1, inflight_bytes = 0;
     int preparing = 0;  // launch sets a scheduler thread is preparing
     int64_t launches = 0, plans_created = 0, peak_bytes = 0;
@@ -188,6 +220,8 @@ struct ax_server {
 
     void run(int id);
     bool complete_locked();
+    bool evict_locked(int64_t need);
+    void free_state_plan_locked(State *st);
     bool launch(std::vector<Request> &batch, int slot, int64_t bytes, PackPool *pk);
     void fail_ticket_locked(int64_t t, ax_status st, const std::string &msg);
     double *pinned_acquire(size_t bytes, size_t *got);
@@ -257,7 +291,14 @@ bool ax_server::complete_locked() {
                 k.error = std::string("launch set failed: ") + hipGetErrorString(q);
             }
             k.done = true;
-            (void)ax_rk_tensor_destroy(s->plans[j]);
+            State *st = s->owner[j];
+            if (!st) {
+                (void)ax_rk_tensor_destroy(s->plans[j]);
+                continue;
+            }
+            st->busy = false;
+            st->last = ++use_clock;
+            if (q != hipSuccess || (st->drop && st->queued == 0)) free_state_plan_locked(st);
         }
         (void)hipEventDestroy(s->done);
         pinned_release_locked(s->staging, s->staging_bytes);
@@ -273,6 +314,37 @@ bool ax_server::complete_locked() {
         any = true;
     }
     return any;
+}
+
+// Frees a state's plan and its charge (the state keeps its shape; a state
+// marked for dropping with nothing queued goes away).
+void ax_server::free_state_plan_locked(State *st) {
+    if (st->plan) (void)ax_rk_tensor_destroy(st->plan);
+    st->plan = nullptr;
+    if (st->charged) state_bytes -= st->plan_bytes;
+    st->charged = false;
+    if (st->drop && st->queued == 0 && !st->busy)
+        for (auto it = states.begin(); it != states.end(); ++it)
+            if (it->second.get() == st) {
+                states.erase(it);
+                break;
+            }
+}
+
+// Frees the plans of idle resident states, least recently used first, until
+// `need` more bytes fit the budget; returns whether they do.
+bool ax_server::evict_locked(int64_t need) {
+    while (inflight_bytes + state_bytes + need > budget) {
+        State *lru = nullptr;
+        for (auto &kv : states) {
+            State *st = kv.second.get();
+            // busy first: the set holding a busy state sets its plan unlocked
+            if (!st->busy && st->charged && st->plan && (!lru || st->last < lru->last)) lru = st;
+        }
+        if (!lru) return false;
+        free_state_plan_locked(lru);
+    }
+    return true;
 }
 
 // Prepares and queues one launch set on slot `slot_id` (called without `mu`).
@@ -320,36 +392,53 @@ bool ax_server::launch(std::vector<Request> &batch, int slot_id, int64_t bytes, 
     }
     pk->start(&segs);
 
-    // plans: one call for the whole set, plan by plan when that fails
-    std::vector<const double *> rr(n), ww(n);
-    std::vector<const int32_t *> ll(n);
-    std::vector<int32_t> nv(n), ng(n), no(n), km(n), pr(n);
-    for (int i = 0; i < n; ++i) {
-        const Request &q = batch[i];
-        rr[i] = q.r;
-        ww[i] = q.w;
-        ll[i] = q.l.data();
-        nv[i] = q.nv;
-        ng[i] = q.ng;
-        no[i] = q.n_orb;
-        km[i] = q.kmax;
-        pr[i] = q.precision;
-    }
+    // plans: a resident state's own (this set holds the state busy), the
+    // others made by one call for the set, plan by plan when that fails
     const int64_t t_setup = now_ns();
     std::vector<ax_rk_tensor *> plans(n, nullptr);
     std::vector<std::string> errs(n);
     std::vector<ax_status> sts(n, AX_OK);
-    if (ax_rk_tensor_create_many(n, rr.data(), ww.data(), nv.data(), ng.data(), no.data(),
-                                 ll.data(), km.data(), pr.data(), slot.stream,
-                                 plans.data()) != AX_OK) {
-        for (int i = 0; i < n; ++i) {
-            sts[i] = ax_rk_tensor_create(rr[i], ww[i], nv[i], ng[i], no[i], ll[i], km[i], pr[i],
-                                         &plans[i]);
+    std::vector<int> make;
+    for (int i = 0; i < n; ++i) {
+        if (batch[i].st && batch[i].st->plan)
+            plans[i] = batch[i].st->plan;
+        else
+            make.push_back(i);
+    }
+    const int nm = static_cast<int>(make.size());
+    std::vector<const double *> rr(nm), ww(nm);
+    std::vector<const int32_t *> ll(nm);
+    std::vector<int32_t> nv(nm), ng(nm), no(nm), km(nm), pr(nm);
+    std::vector<ax_rk_tensor *> made(nm, nullptr);
+    for (int j = 0; j < nm; ++j) {
+        const Request &q = batch[make[j]];
+        rr[j] = q.r;
+        ww[j] = q.w;
+        ll[j] = q.l.data();
+        nv[j] = q.nv;
+        ng[j] = q.ng;
+        no[j] = q.n_orb;
+        km[j] = q.kmax;
+        pr[j] = q.precision;
+    }
+    if (nm > 0 && ax_rk_tensor_create_many(nm, rr.data(), ww.data(), nv.data(), ng.data(),
+                                           no.data(), ll.data(), km.data(), pr.data(), slot.stream,
+                                           made.data()) != AX_OK) {
+        for (int j = 0; j < nm; ++j) {
+            const int i = make[j];
+            sts[i] = ax_rk_tensor_create(rr[j], ww[j], nv[j], ng[j], no[j], ll[j], km[j], pr[j],
+                                         &made[j]);
             if (sts[i] != AX_OK) {
                 errs[i] = ax_last_error();
-                plans[i] = nullptr;
+                made[j] = nullptr;
             }
         }
+    }
+    int made_ok = 0;
+    for (int j = 0; j < nm; ++j) {
+        plans[make[j]] = made[j];
+        if (batch[make[j]].st) batch[make[j]].st->plan = made[j];  // null if it failed
+        made_ok += made[j] != nullptr;
     }
     const int64_t t_pack = now_ns();
     pk->wait();
@@ -394,6 +483,7 @@ bool ax_server::launch(std::vector<Request> &batch, int slot_id, int64_t bytes, 
             qq.push_back(batch[i].Q ? dev + in_off[i] + m : nullptr);
             oo.push_back(dev + n_in + s->offset[j]);
             s->plans.push_back(plans[i]);
+            s->owner.push_back(batch[i].st);
             s->tickets.push_back(batch[i].ticket);
         }
         e = hipMemcpyAsync(dev, staging, in_bytes, hipMemcpyHostToDevice, slot.stream);
@@ -413,8 +503,21 @@ bool ax_server::launch(std::vector<Request> &batch, int slot_id, int64_t bytes, 
     ns_pack_wait += t_alloc - t_pack;
     ns_alloc += ns_stage + (t_queue - t_alloc);  // input staging, device and result buffers
     ns_queue += t_end - t_queue;
+    plans_made += made_ok;
+    // a state whose plan could not be made, or whose set failed, is idle
+    // again and gives back its charge (its next request makes a new plan)
+    auto release_state = [&](State *st) {
+        if (!st) return;
+        st->plan = nullptr;
+        st->busy = false;
+        st->last = ++use_clock;
+        free_state_plan_locked(st);
+    };
     for (int i = 0; i < n; ++i)
-        if (!plans[i]) fail_ticket_locked(batch[i].ticket, sts[i], errs[i]);
+        if (!plans[i]) {
+            fail_ticket_locked(batch[i].ticket, sts[i], errs[i]);
+            release_state(batch[i].st);
+        }
     if (keep.empty() || e != hipSuccess || run != AX_OK) {
         const std::string msg = run != AX_OK ? std::string(ax_last_error())
                                              : std::string("launch set failed: ") +
@@ -423,11 +526,13 @@ bool ax_server::launch(std::vector<Request> &batch, int slot_id, int64_t bytes, 
         for (size_t j = 0; j < s->plans.size(); ++j) {
             fail_ticket_locked(s->tickets[j], run != AX_OK ? run : AX_ERR_HIP, msg);
             (void)ax_rk_tensor_destroy(s->plans[j]);
+            release_state(s->owner[j]);
         }
         for (int i : keep)
             if (std::find(s->plans.begin(), s->plans.end(), plans[i]) == s->plans.end()) {
                 fail_ticket_locked(batch[i].ticket, AX_ERR_MEMORY, "staging allocation failed");
                 (void)ax_rk_tensor_destroy(plans[i]);
+                release_state(batch[i].st);
             }
         if (s->done) (void)hipEventDestroy(s->done);
         pinned_release_locked(s->results, s->result_bytes);
@@ -460,17 +565,41 @@ void ax_server::run(int id) {
             }
         std::vector<Request> batch;
         if (slot >= 0) {
+            // a request whose state is busy (its previous request is in flight
+            // or in this set) waits and later ones go ahead, so a state's own
+            // requests keep their order; a resident plan is charged to the
+            // state's bytes, not to the set's, and idle states lose their
+            // plans (least recently used first) when a request does not fit
             int64_t bytes = 0;
-            while (!queue.empty() && (int)batch.size() < max_batch &&
-                   inflight_bytes + bytes + queue.front().bytes <= budget) {
-                bytes += queue.front().bytes;
-                batch.push_back(std::move(queue.front()));
-                queue.pop_front();
+            for (auto it = queue.begin(); it != queue.end() && (int)batch.size() < max_batch;) {
+                State *st = it->state ? states.at(it->state).get() : nullptr;
+                if (st && st->busy) {
+                    ++it;
+                    continue;
+                }
+                const bool new_plan = st && !st->charged;
+                const int64_t need = st ? it->bytes - it->plan_bytes : it->bytes;
+                const int64_t charge = need + (new_plan ? st->plan_bytes : 0);
+                if (inflight_bytes + state_bytes + bytes + charge > budget &&
+                    !evict_locked(bytes + charge))
+                    break;
+                if (st) {
+                    st->busy = true;
+                    --st->queued;
+                    if (new_plan) {
+                        st->charged = true;
+                        state_bytes += st->plan_bytes;
+                    }
+                }
+                bytes += need;
+                it->st = st;
+                batch.push_back(std::move(*it));
+                it = queue.erase(it);
             }
             if (!batch.empty()) {
                 slots[slot].busy = true;
                 inflight_bytes += bytes;  // reserved while the set is prepared
-                peak_bytes = std::max(peak_bytes, inflight_bytes);
+                peak_bytes = std::max(peak_bytes, inflight_bytes + state_bytes);
                 ++preparing;
                 lk.unlock();
                 const bool queued = launch(batch, slot, bytes, pack[id]);
@@ -532,6 +661,9 @@ ax_status ax_server_destroy(ax_server *s) {
     s->work_cv.notify_all();
     for (auto &w : s->worker) w.join();  // drain the queue and every in-flight set
     for (PackPool *p : s->pack) delete p;
+    for (auto &kv : s->states)
+        if (kv.second->plan) (void)ax_rk_tensor_destroy(kv.second->plan);
+    s->states.clear();
     for (Slot &sl : s->slots) {
         if (sl.dev) ax::pool_free(sl.dev);
         if (sl.stream) {
@@ -552,10 +684,24 @@ ax_status ax_server_destroy(ax_server *s) {
 
 ax_status ax_server_submit(ax_server *s, const double *r_host, const double *w_host, int nv,
                            int ng, int n_orb, const int32_t *l_host, int kmax, int precision,
-                           const double *P_host, const double *Q_host, int64_t *ticket) {
+                           const double *P_host, const double *Q_host, int64_t state,
+                           int64_t *ticket) {
     AX_REQUIRE(s && r_host && w_host && l_host && P_host && ticket,
                "ax_server_submit: null argument");
-    const int64_t plan_bytes = ax_rk_tensor_estimate(nv, ng, n_orb, l_host, kmax, precision);
+    // a request of an existing state of the same scalars has its plan size
+    // (its full shape is checked below); others are estimated
+    int64_t plan_bytes = -1;
+    if (state != 0) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->states.find(state);
+        if (it != s->states.end()) {
+            const State &st = *it->second;
+            if (st.nv == nv && st.ng == ng && st.n_orb == n_orb && st.kmax == kmax &&
+                st.precision == precision)
+                plan_bytes = st.plan_bytes;
+        }
+    }
+    if (plan_bytes < 0) plan_bytes = ax_rk_tensor_estimate(nv, ng, n_orb, l_host, kmax, precision);
     AX_REQUIRE(plan_bytes >= 0, "ax_server_submit: invalid request (%s)", ax_last_error());
     const int64_t n = (int64_t)nv * ng;
     // R^k count of the request (the plan's output size), by angular class
@@ -583,11 +729,43 @@ ax_status ax_server_submit(ax_server *s, const double *r_host, const double *w_h
     q.l.assign(l_host, l_host + n_orb);
     q.n_in = (int64_t)n_orb * n * (Q_host ? 2 : 1);
     q.bytes = plan_bytes + 8 * (q.n_in + n_rk);
+    q.plan_bytes = plan_bytes;
+    q.state = state;
     std::lock_guard<std::mutex> lk(s->mu);
     AX_REQUIRE_CODE(!s->stop, AX_ERR_VALUE, "ax_server_submit: the server is closing");
     AX_REQUIRE_CODE(q.bytes <= s->budget, AX_ERR_MEMORY,
                     "ax_server_submit: request needs %lld device bytes, above the budget %lld",
                     (long long)q.bytes, (long long)s->budget);
+    if (state != 0) {
+        // the state's shape is that of its first request; later ones must
+        // match it (grid values included), since they reuse its plan
+        auto &slot = s->states[state];
+        if (!slot) {
+            slot.reset(new State());
+            State &st = *slot;
+            st.nv = nv;
+            st.ng = ng;
+            st.n_orb = n_orb;
+            st.kmax = kmax;
+            st.precision = precision;
+            st.has_q = Q_host != nullptr;
+            st.l = q.l;
+            st.r.assign(r_host, r_host + n);
+            st.w.assign(w_host, w_host + n);
+            st.plan_bytes = plan_bytes;
+        } else {
+            const State &st = *slot;
+            const bool same = st.nv == nv && st.ng == ng && st.n_orb == n_orb &&
+                              st.kmax == kmax && st.precision == precision &&
+                              st.has_q == (Q_host != nullptr) && st.l == q.l &&
+                              std::memcmp(st.r.data(), r_host, n * sizeof(double)) == 0 &&
+                              std::memcmp(st.w.data(), w_host, n * sizeof(double)) == 0;
+            AX_REQUIRE(same, "ax_server_submit: state %lld was created with another shape",
+                       (long long)state);
+        }
+        slot->drop = false;
+        ++slot->queued;
+    }
     q.ticket = s->next_ticket++;
     s->tickets[q.ticket] = Ticket{};
     s->tickets[q.ticket].size = n_rk;
@@ -655,9 +833,24 @@ ax_status ax_server_release(ax_server *s, int64_t ticket) {
 ax_status ax_server_stats(ax_server *s, int64_t *out, int n) {
     AX_REQUIRE(s && out && n >= 0, "ax_server_stats: null argument");
     std::lock_guard<std::mutex> lk(s->mu);
-    const int64_t v[7] = {s->launches,     s->plans_created, s->peak_bytes, s->ns_setup,
-                          s->ns_pack_wait, s->ns_alloc,      s->ns_queue};
-    for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
+    int64_t resident = 0;
+    for (auto &kv : s->states)
+        if (kv.second->plan) ++resident;
+    const int64_t v[10] = {s->launches,     s->plans_created, s->peak_bytes, s->ns_setup,
+                           s->ns_pack_wait, s->ns_alloc,      s->ns_queue,   s->plans_made,
+                           resident,        s->state_bytes};
+    for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];
+    return AX_OK;
+}
+
+ax_status ax_server_drop_state(ax_server *s, int64_t state) {
+    AX_REQUIRE(s != nullptr, "ax_server_drop_state: null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->states.find(state);
+    if (it == s->states.end()) return AX_OK;
+    State *st = it->second.get();
+    st->drop = true;
+    if (!st->busy && st->queued == 0) s->free_state_plan_locked(st);  // erases it
     return AX_OK;
 }
 

@@ -609,6 +609,28 @@ def _served(reqs, warm):
     return dt, st
 
 
+def _served_native(reqs, warm):
+    """Seconds to serve ``reqs`` submitted at once through the native
+    serving loop with resident states (after ``warm`` untimed, which makes
+    the states' plans); the same collector treatment as ``_served``."""
+    import gc
+    import time
+
+    from atomx_amd.serve_native import NativeSlaterServer
+    with NativeSlaterServer(max_inflight=4, max_batch=16) as srv:
+        for t in [srv.submit(r) for r in warm]:
+            srv.result(t)
+        gc.collect()
+        gc.freeze()
+        t0 = time.perf_counter()
+        res = [srv.result(t) for t in [srv.submit(r) for r in reqs]]
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        del res
+    gc.unfreeze()
+    return dt, st
+
+
 def scf_serving(args, slater, iterations=6):
     """SCF-style serving: every atom sends one all-R^k request per SCF
     iteration (new orbitals, same grid and shells) under a resident state;
@@ -618,7 +640,8 @@ def scf_serving(args, slater, iterations=6):
     to 60 orbitals, GPU-bound per atom) and 256 small atoms and ions (40-160
     cells, 1-10 orbitals: launch-bound per atom), the latter served both as
     one request per atom and as one request per group of 32 atoms per
-    iteration (``SlaterBatchRequest``)."""
+    iteration (``SlaterBatchRequest``); per-atom requests also through the
+    native serving loop with its resident states."""
     from atomx_amd.serve import SlaterBatchRequest
     from atomx_amd.workloads import count_rk, small_atom_requests
 
@@ -629,6 +652,7 @@ def scf_serving(args, slater, iterations=6):
         n = len(atoms)
         n_rk = sum(count_rk(r.ls, r.kmax) for r in reqs[n:])
         t_req, st = _served(reqs[n:], reqs[:n])
+        t_nat, st_nat = _served_native(reqs[n:], reqs[:n])
         t_plain = _plain_per_atom(atoms, reqs[n:])
         row = {
             "atoms": n, "timed_iterations": iterations - 1, "rk_total": n_rk,
@@ -636,6 +660,9 @@ def scf_serving(args, slater, iterations=6):
             "served_per_atom_requests_rk_per_s": n_rk / t_req,
             "served_per_atom_requests_vs_plain": t_plain / t_req,
             "served_per_atom_p50_ms": st["p50_ms"], "served_per_atom_p99_ms": st["p99_ms"],
+            "native_served_per_atom_rk_per_s": n_rk / t_nat,
+            "native_served_per_atom_vs_plain": t_plain / t_nat,
+            "native_plans_made": st_nat["plans_made"],
         }
         if name == "small256":
             group = 32

@@ -230,10 +230,17 @@ ax_status ax_server_destroy(ax_server *server);
  * arrays, Q_host may be NULL).  r_host, w_host, P_host and Q_host are read
  * when the request is launched and must stay valid until its ticket is
  * complete; l_host is copied.  AX_ERR_MEMORY when the request alone needs
- * more than the budget. */
+ * more than the budget.  state != 0 names a resident state (one atom's
+ * requests, SCF iteration after SCF iteration): its first request fixes the
+ * shape (grid values, l, kmax, precision, Q given or not) and its plan is
+ * kept for the later ones, which must have that shape (AX_ERR_VALUE
+ * otherwise); one request of a state runs at a time, requests of other
+ * states may go ahead of a waiting one, and idle states give up their plans
+ * (least recently used first) when a request does not fit the budget. */
 ax_status ax_server_submit(ax_server *server, const double *r_host, const double *w_host, int nv,
                            int ng, int n_orb, const int32_t *l_host, int kmax, int precision,
-                           const double *P_host, const double *Q_host, int64_t *ticket);
+                           const double *P_host, const double *Q_host, int64_t state,
+                           int64_t *ticket);
 ax_status ax_server_poll(ax_server *server, int64_t ticket, int32_t *done);
 ax_status ax_server_wait(ax_server *server, int64_t ticket);
 /* A complete ticket's packed R^k values (the layout of ax_rk_tensor_run:
@@ -244,11 +251,16 @@ ax_status ax_server_wait(ax_server *server, int64_t ticket);
 ax_status ax_server_result(ax_server *server, int64_t ticket, const double **values,
                            int64_t *size, int32_t *slices);
 ax_status ax_server_release(ax_server *server, int64_t ticket);
-/* Counters since creation, the first n of: launch sets, plans, peak
- * in-flight device bytes, and host nanoseconds of the scheduler thread in
- * plan set-up, waiting for the packing, allocations, and queueing copies
- * and launches. */
+/* Counters since creation, the first n of: launch sets, plans run, peak
+ * device bytes (in flight plus resident), host nanoseconds of the scheduler
+ * thread in plan set-up, waiting for the packing, allocations, and queueing
+ * copies and launches; plans made; and now: resident states holding a plan,
+ * their bytes. */
 ax_status ax_server_stats(ax_server *server, int64_t *out, int n);
+/* Frees a resident state's plan once no request of it is queued or in
+ * flight (at once if none is); a later request naming the key starts a new
+ * state.  Unknown keys are ignored. */
+ax_status ax_server_drop_state(ax_server *server, int64_t state);
 /* Pair-product tensor (the CI / CCSD B-spline product workload): n_fn
  * functions sampled on the grid (P rows; Q may be NULL), an explicit list of
  * n_pairs pairs (pa_host[p], pb_host[p]), every pair at every rank
