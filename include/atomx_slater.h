@@ -257,9 +257,10 @@ ax_status ax_server_release(ax_server *server, int64_t ticket);
  * copies and launches; plans made; and now: resident states holding a plan,
  * their bytes. */
 ax_status ax_server_stats(ax_server *server, int64_t *out, int n);
-/* Frees a resident state's plan once no request of it is queued or in
- * flight (at once if none is); a later request naming the key starts a new
- * state.  Unknown keys are ignored. */
+/* Frees a resident state once no request of it is queued or in flight (at
+ * once if none is); a request naming the key before that keeps the state
+ * (the drop is cancelled), one after it starts a new state.  Unknown keys
+ * are ignored. */
 ax_status ax_server_drop_state(ax_server *server, int64_t state);
 /* Pair-product tensor (the CI / CCSD B-spline product workload): n_fn
  * functions sampled on the grid (P rows; Q may be NULL), an explicit list of
