@@ -4,7 +4,8 @@
 // hipFree of their dozen buffers would dominate small requests.  Freed blocks
 // are kept per (device, size class) and reused; on allocation failure the
 // cache is released and the allocation retried once.  Size classes: powers
-// of two up to 2 MiB, multiples of 2 MiB above.
+// of two up to 2 MiB, multiples of 2 MiB above (served by a cached block up
+// to 1/8 larger).
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -48,12 +49,17 @@ hipError_t pool_alloc(void **ptr, size_t bytes) {
     (void)hipGetDevice(&dev);
     const size_t cls = size_class(bytes == 0 ? 1 : bytes);
     std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_free.find({dev, cls});
-    if (it != g_free.end()) {
+    // exact class, or for the 2 MiB-multiple classes the smallest cached
+    // block at most 1/8 larger (the sizes of one-shot plans vary from set to
+    // set; a miss costs a hipMalloc); a block keeps its own class
+    auto it = g_free.lower_bound({dev, cls});
+    const size_t slack = cls >= (size_t(2) << 20) ? cls / 8 : 0;
+    if (it != g_free.end() && it->first.first == dev && it->first.second <= cls + slack) {
+        const size_t got = it->first.second;
         *ptr = it->second;
         g_free.erase(it);
-        g_cached -= cls;
-        g_live[*ptr] = Block{cls, dev};
+        g_cached -= got;
+        g_live[*ptr] = Block{got, dev};
         return hipSuccess;
     }
     hipError_t e = hipMalloc(ptr, cls);
