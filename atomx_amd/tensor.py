@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import device as dv
-from ._lib import PRECISIONS, call, load
+from ._lib import PRECISIONS, call, check, load
 from .workloads import pair_list, valid_pairs
 
 
@@ -61,16 +61,16 @@ class SlaterTensor:
     def _attach(self, lib, h):
         self._h = h
         self._lib = lib
+        k1 = self.kmax + 1
         size, slices = ctypes.c_int64(), ctypes.c_int32()
-        pairs = np.empty(self.kmax + 1, dtype=np.int32)
-        offs = np.empty(self.kmax + 1, dtype=np.int64)
-        call("ax_rk_tensor_describe", h, ctypes.byref(size), ctypes.byref(slices),
-             pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-             offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
-        self.size = int(size.value)
-        self.m = pairs.tolist()
-        self.offsets = offs.tolist()
-        self.slices = int(slices.value)
+        pairs, offs = (ctypes.c_int32 * k1)(), (ctypes.c_int64 * k1)()
+        rc = lib.ax_rk_tensor_describe(h, ctypes.byref(size), ctypes.byref(slices), pairs, offs)
+        if rc:
+            check(rc, "ax_rk_tensor_describe")
+        self.size = size.value
+        self.m = list(pairs)
+        self.offsets = list(offs)
+        self.slices = slices.value
 
     @classmethod
     def create_many(cls, specs, precision="fp64", stream=None):
@@ -88,13 +88,12 @@ class SlaterTensor:
         if n == 0:
             return []
         objs, keep = [], []
-        r_arr = (ctypes.c_void_p * n)()
-        w_arr = (ctypes.c_void_p * n)()
-        l_arr = (ctypes.c_void_p * n)()
-        shape = np.empty((4, n), dtype=np.int32)  # nv, ng, n_orb, kmax
-        for i, (gr, grw, ls, kmax) in enumerate(specs):
-            gr = np.ascontiguousarray(gr, dtype=np.float64)
-            grw = np.ascontiguousarray(grw, dtype=np.float64)
+        ptrs = [], [], []  # grid, weights, l addresses
+        nvs, ngs, nos, kms = [], [], [], []
+        f64, i32 = np.float64, np.int32
+        for gr, grw, ls, kmax in specs:
+            gr = np.ascontiguousarray(gr, dtype=f64)
+            grw = np.ascontiguousarray(grw, dtype=f64)
             if gr.ndim != 2 or gr.shape != grw.shape:
                 raise ValueError("grid and weights must both be (nv, n_gauss)")
             o = cls.__new__(cls)
@@ -102,21 +101,21 @@ class SlaterTensor:
             o.n = o.nv * o.ng
             o.kmax = int(kmax)
             o.precision = precision
-            o.ls = np.ascontiguousarray(ls, dtype=np.int32)
+            o.ls = np.ascontiguousarray(ls, dtype=i32)
             o.n_orb = int(o.ls.size)
             o._pairs = None
             keep += [gr, grw]
-            r_arr[i] = gr.ctypes.data
-            w_arr[i] = grw.ctypes.data
-            l_arr[i] = o.ls.ctypes.data
-            shape[:, i] = (o.nv, o.ng, o.n_orb, o.kmax)
+            for lst, a in zip(ptrs, (gr, grw, o.ls)):
+                lst.append(a.__array_interface__["data"][0])
+            nvs.append(o.nv)
+            ngs.append(o.ng)
+            nos.append(o.n_orb)
+            kms.append(o.kmax)
             objs.append(o)
-        prec = np.full(n, PRECISIONS[precision], dtype=np.int32)
+        vp, ia = ctypes.c_void_p * n, ctypes.c_int32 * n
         hs = (ctypes.c_void_p * n)()
-        ip = ctypes.POINTER(ctypes.c_int32)
-        call("ax_rk_tensor_create_many", n, r_arr, w_arr, shape[0].ctypes.data_as(ip),
-             shape[1].ctypes.data_as(ip), shape[2].ctypes.data_as(ip), l_arr,
-             shape[3].ctypes.data_as(ip), prec.ctypes.data_as(ip),
+        call("ax_rk_tensor_create_many", n, vp(*ptrs[0]), vp(*ptrs[1]), ia(*nvs), ia(*ngs),
+             ia(*nos), vp(*ptrs[2]), ia(*kms), ia(*([PRECISIONS[precision]] * n)),
              None if stream is None else dv.stream_handle(stream), hs)
         for o, h in zip(objs, hs):
             o._attach(lib, ctypes.c_void_p(h))
