@@ -299,21 +299,51 @@ __global__ void density_sym_kernel(const double *__restrict__ A, int64_t lda,
     }
 }
 
-// out[t] = sum_j coef[t, j] * dens[j]  (rows of length L), j ascending.
+// out[t][e] = sum_j coef[t][j] dens[j][e] (rows of length L, j ascending):
+// work-group row blockIdx.y takes kCombineT targets and each thread one
+// column e of them, so the coefficients are work-group uniform (scalar
+// loads, and a uniform branch skips a zero one) and each density value is
+// loaded once per kCombineT targets, four j ahead of their FMAs; every
+// output sums over j in order, skipping zero coefficients (the values of one
+// target at a time, bit for bit).
+constexpr int kCombineT = 8;
 __global__ void combine_rows_kernel(const double *__restrict__ dens, int64_t L, int n_src,
                                     const double *__restrict__ coef, int nt,
                                     double *__restrict__ out) {
-    const int64_t total = L * nt;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int tt = static_cast<int>(t / L);
-        const int64_t e = t - (int64_t)tt * L;
-        double s = 0.0;
-        for (int j = 0; j < n_src; ++j) {
-            const double c = coef[(int64_t)tt * n_src + j];
-            if (c != 0.0) s += c * dens[(int64_t)j * L + e];
+    const int t0 = blockIdx.y * kCombineT;
+    const int tn = min(kCombineT, nt - t0);
+    const double *cr = coef + (int64_t)t0 * n_src;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < L;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        double s[kCombineT];
+#pragma unroll
+        for (int u = 0; u < kCombineT; ++u) s[u] = 0.0;
+        int j = 0;
+        for (; j + 4 <= n_src; j += 4) {
+            double d[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) d[q] = dens[(int64_t)(j + q) * L + e];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int u = 0; u < kCombineT; ++u) {
+                    if (u >= tn) break;
+                    const double c = cr[(int64_t)u * n_src + j + q];
+                    if (c != 0.0) s[u] += c * d[q];
+                }
         }
-        out[t] = s;
+        for (; j < n_src; ++j) {
+            const double d = dens[(int64_t)j * L + e];
+#pragma unroll
+            for (int u = 0; u < kCombineT; ++u) {
+                if (u >= tn) break;
+                const double c = cr[(int64_t)u * n_src + j];
+                if (c != 0.0) s[u] += c * d;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCombineT; ++u)
+            if (u < tn) out[(int64_t)(t0 + u) * L + e] = s[u];
     }
 }
 
@@ -964,7 +994,9 @@ ax_status ax_combine_rows(const double *dens, int64_t L, int n_src, const double
                           double *out, void *stream) {
     AX_REQUIRE(L > 0 && n_src >= 0 && nt >= 0, "ax_combine_rows: bad sizes");
     if (nt == 0) return AX_OK;
-    hipLaunchKernelGGL(combine_rows_kernel, dim3(blocks(L * nt)), dim3(kT), 0,
+    const int groups = (nt + kCombineT - 1) / kCombineT;
+    AX_REQUIRE(groups <= 65535, "ax_combine_rows: %d targets exceed one launch", nt);
+    hipLaunchKernelGGL(combine_rows_kernel, dim3(blocks(L), groups), dim3(kT), 0,
                        ax::as_stream(stream), dens, L, n_src, coef, nt, out);
     AX_CHECK_LAUNCH();
     return AX_OK;
